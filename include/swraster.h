@@ -1,0 +1,377 @@
+/*
+ * swraster.h -- C ABI of the MI355X (gfx950) rasterization pipeline.
+ *
+ * The pipeline replaces the per-frame path of the reference software
+ * rasterizer: clip-space projection with frustum clipping, tile binning,
+ * per-tile rasterization into a visibility buffer, visibility-buffer PBR
+ * shading, auto exposure and the tonemapping blit.
+ *
+ * Reference interfaces replaced (paths are relative to the reference tree):
+ *   Renderer::new                  src/renderer.rs:165   -> sr_renderer_create
+ *   Renderer::render_scene         src/renderer.rs:201   -> sr_renderer_render_scene
+ *   Renderer::update_auto_exposure src/renderer.rs:258   -> sr_renderer_update_auto_exposure
+ *   Renderer::blit_to_buffer       src/renderer.rs:293   -> sr_renderer_blit_to_buffer
+ *   TileRasterizer::render_tile    src/tilerasterizer.rs:72 (runs inside
+ *                                  sr_renderer_render_scene, one GPU
+ *                                  workgroup per screen tile)
+ *   RenderCamera::new              src/rendercamera.rs:28 -> sr_camera_new
+ *   RenderCamera::update_matrices  src/rendercamera.rs:78 -> sr_camera_update_matrices
+ *   Scene (constructed by Scene::from_gltf, src/scene.rs:145, and
+ *   load_scene, src/main.rs:100-283)   -> sr_scene_desc + sr_scene_create
+ *
+ * Conventions
+ *   - every function returns int: SR_OK (0) or a negative SR_ERR_* code and
+ *     never throws across the ABI; sr_last_error() gives thread-local text.
+ *   - a renderer owns its frame buffers and one HIP stream; it is not
+ *     re-entrant (the reference takes &mut self).  Distinct renderers may
+ *     run concurrently.  A scene is immutable after creation and may be
+ *     shared by any number of renderers, batches and servers.
+ *   - pixels are packed r<<24 | g<<16 | b<<8 | a (src/util.rs:98-100).
+ *   - matrices are column-major float[16] (glam Mat4 layout).
+ */
+#ifndef SWRASTER_H
+#define SWRASTER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SR_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------ */
+#define SR_OK 0
+#define SR_ERR_INVALID_ARGUMENT (-1)
+#define SR_ERR_HIP (-2)
+#define SR_ERR_OUT_OF_MEMORY (-3)
+#define SR_ERR_UNSUPPORTED (-4)
+#define SR_ERR_CAPACITY (-5)
+#define SR_ERR_BUSY (-6)
+#define SR_ERR_NOT_FOUND (-7)
+
+/* ---- textures and materials (src/texture.rs:19-42, src/scene.rs:104-121) */
+#define SR_TEX_SRGB 0
+#define SR_TEX_NORMAL 1
+#define SR_TEX_METALLIC_ROUGHNESS 2
+#define SR_TEX_LINEAR 3
+
+#define SR_WRAP_REPEAT 0
+#define SR_WRAP_MIRRORED_REPEAT 1
+#define SR_WRAP_CLAMP_TO_EDGE 2
+
+#define SR_ALPHA_OPAQUE 0
+#define SR_ALPHA_MASK 1
+#define SR_ALPHA_BLEND 2
+
+/* Base level of an RGBA8 texture; the mip chain is generated on load exactly
+ * as Texture::generate_mipmaps (src/texture.rs:45-128). */
+typedef struct sr_texture_desc {
+    uint32_t width, height;
+    int32_t type;             /* SR_TEX_* */
+    const uint32_t *texels;   /* width*height packed RGBA8 */
+} sr_texture_desc;
+
+typedef struct sr_texref {
+    int32_t texture;          /* index into sr_scene_desc.textures, -1 = none */
+    int32_t wrap_s, wrap_t;   /* SR_WRAP_* */
+} sr_texref;
+
+typedef struct sr_material_desc {
+    float base_color_factor[4];
+    float metallic_factor;
+    float roughness_factor;
+    float emissive_factor[3];
+    float occlusion_strength;
+    float alpha_cutoff;
+    float transmission;       /* KHR_materials_transmission factor */
+    int32_t alpha_mode;       /* SR_ALPHA_* */
+    int32_t is_translucent;   /* material has KHR_materials_transmission */
+    sr_texref base_color;
+    sr_texref metallic_roughness;
+    sr_texref normal;
+    sr_texref emissive;
+    sr_texref occlusion;
+    sr_texref transmission_texture;
+} sr_material_desc;
+
+/* One glTF primitive (src/scene.rs:94-102).  Positions, normals and texture
+ * coordinates are tightly packed float arrays (xyz, xyz, uv).  texcoords may
+ * be NULL (zeros are used, src/scene.rs:455-464). */
+typedef struct sr_primitive_desc {
+    const float *positions;   /* num_vertices * 3 */
+    const float *normals;     /* num_vertices * 3 */
+    const float *texcoords;   /* num_vertices * 2 or NULL */
+    const float *tangents;    /* num_vertices * 4 or NULL (computed as
+                                 compute_tangents, src/scene.rs:548-635) */
+    const uint32_t *indices;  /* num_indices (triangle list) */
+    uint32_t num_vertices;
+    uint32_t num_indices;
+    int32_t material;         /* index into materials; absent -> 0 (src/scene.rs:493) */
+} sr_primitive_desc;
+
+typedef struct sr_mesh_desc {
+    int32_t first_primitive;  /* primitives[first .. first+count) */
+    int32_t num_primitives;
+} sr_mesh_desc;
+
+/* A flattened node: world transform and optional mesh (src/scene.rs:79-85). */
+typedef struct sr_node_desc {
+    float transform[16];
+    int32_t mesh;             /* -1 = none */
+} sr_node_desc;
+
+/* Image-based lighting inputs derived from the skybox cubemap at load time
+ * (src/scene.rs:152-228): the skybox faces (mip 0 only is ever sampled), the
+ * GGX-prefiltered specular cubemap (every mip at full face resolution,
+ * src/texture.rs:330-420), the 4-coefficient irradiance SH
+ * (src/texture.rs:289-328) and the split-sum BRDF LUT (src/texture.rs:199-235).
+ * sr_prepare_environment() fills one from the skybox faces. */
+typedef struct sr_environment {
+    uint32_t cubemap_size;              /* face width == height */
+    const uint32_t *cubemap;            /* 6 * size^2, faces +X,-X,+Y,-Y,+Z,-Z */
+    uint32_t specular_size;             /* face size of every specular mip */
+    uint32_t specular_mips;             /* 1 + ilog2(size) */
+    const uint32_t *specular;           /* specular_mips * 6 * size^2 */
+    uint32_t brdf_lut_size;
+    const uint32_t *brdf_lut;           /* size^2 (mip 0 only is sampled) */
+    float irradiance_sh[4][3];
+} sr_environment;
+
+/* Voxel lighting grid (src/voxelgrid.rs:6-18).  Without a GI bake the
+ * reference fills every voxel with SH = scale * irradiance_sh, w0 = sun
+ * visibility and w1 = sky visibility (src/gi.rs:123-149, src/main.rs:280);
+ * sh4 == NULL selects exactly that form and only the per-voxel sun
+ * visibility is stored.  sr_prepare_voxel_grid() builds it. */
+typedef struct sr_voxel_grid {
+    int32_t dims[3];
+    float world_min[3];
+    float world_max[3];
+    const float *light_intensity;       /* dims product: blurred sun visibility */
+    float sh_scale;                     /* 0.25 (GI_FALLBACK_SCENE_SH_SCALE) */
+    float sky_visibility;               /* 1.0 */
+    const float *sh4;                   /* optional dims product * 16 floats, NULL */
+} sr_voxel_grid;
+
+typedef struct sr_light {
+    float direction[3];       /* normalized (-0.2, 1, 0.5) by default */
+    float color[3];           /* (1, 1, 0.95) * 5 by default */
+} sr_light;
+
+typedef struct sr_scene_desc {
+    const sr_primitive_desc *primitives; int32_t num_primitives;
+    const sr_mesh_desc *meshes;          int32_t num_meshes;
+    const sr_node_desc *nodes;           int32_t num_nodes;
+    const sr_material_desc *materials;   int32_t num_materials;
+    const sr_texture_desc *textures;     int32_t num_textures;
+    const sr_environment *environment;
+    const sr_voxel_grid *voxels;
+    sr_light light;
+} sr_scene_desc;
+
+/* ---- camera (src/rendercamera.rs:5-25) --------------------------------- */
+typedef struct sr_camera {
+    float position[3];
+    float fov, width, height, one_over_width, one_over_height;
+    float near_plane, far_plane;
+    float yaw, pitch, roll;
+    float view_matrix[16];
+    float projection_matrix[16];
+    float view_project_matrix[16];
+    float inverse_view_project_matrix[16];
+    float skybox_matrix_transposed[16];
+    float view_clip_planes[6][4];
+    float view_normal[3];
+} sr_camera;
+
+/* ---- renderer options and statistics ------------------------------------ */
+typedef struct sr_renderer_opts {
+    int32_t tile_size;        /* GPU screen tile: 8, 16 or 32 (0 = tuned default) */
+    int32_t raster_mode;      /* 0 = tuned default; see DESIGN */
+    int32_t device;           /* HIP device ordinal */
+    int32_t reserved[5];
+} sr_renderer_opts;
+
+typedef struct sr_frame_stats {
+    uint64_t input_triangles;      /* submitted, pre-cull */
+    uint64_t setup_triangles;      /* after frustum clipping (fan output) */
+    uint64_t binned_pairs;         /* (triangle, tile) references */
+    uint32_t tiles;
+    uint32_t draws;
+    float ms_setup;                /* transform+clip+setup+bin count */
+    float ms_bin;                  /* scans + scatter */
+    float ms_raster_shade;         /* per-tile raster + vbuffer shading */
+    float ms_blit;
+    float ms_total;
+} sr_frame_stats;
+
+/* ---- batched views and render server ------------------------------------ */
+typedef struct sr_view {
+    const struct sr_scene *scene;
+    sr_camera camera;
+    int32_t width, height;
+    float exposure;           /* exposure applied by the fused blit */
+} sr_view;
+
+typedef struct sr_batch_opts {
+    int32_t tile_size;        /* 0 = tuned default */
+    int32_t use_graph;        /* capture the launch sequence in a hipGraph */
+    int32_t persistent_blocks;/* 0 = 8 per CU */
+    int32_t device;
+    int32_t reserved[4];
+} sr_batch_opts;
+
+typedef struct sr_batch_stats {
+    uint64_t input_triangles;
+    uint64_t setup_triangles;
+    uint64_t binned_pairs;
+    uint64_t pixels;
+    uint32_t views;
+    uint32_t work_items;      /* (view, tile) items drained by the persistent kernel */
+    float ms_total;
+} sr_batch_stats;
+
+typedef struct sr_server_opts {
+    uint64_t admission_bytes; /* framebuffer + bin budget (0 = 64 GiB) */
+    int32_t max_batch_views;  /* 0 = 64 */
+    int32_t streams;          /* batches in flight (0 = 2) */
+    int32_t tile_size;        /* 0 = tuned default */
+    int32_t device;
+    int32_t reserved[4];
+} sr_server_opts;
+
+typedef struct sr_request {
+    int32_t scene;            /* index returned by sr_server_add_scene */
+    sr_camera camera;
+    int32_t width, height;
+    float exposure;
+    uint64_t user_tag;
+} sr_request;
+
+typedef struct sr_completion {
+    uint64_t request_id;
+    uint64_t user_tag;
+    int32_t status;
+    int32_t width, height;
+    double submit_time_s;     /* server clock */
+    double start_time_s;
+    double complete_time_s;
+} sr_completion;
+
+typedef struct sr_scene sr_scene;
+typedef struct sr_renderer sr_renderer;
+typedef struct sr_batch sr_batch;
+typedef struct sr_server sr_server;
+
+/* ---- library ------------------------------------------------------------ */
+SR_API const char *sr_last_error(void);
+SR_API int sr_version(int32_t *major, int32_t *minor);
+SR_API int sr_device_count(int32_t *count);
+
+/* ---- load-time input producers (host, C++) ------------------------------- */
+/* Builds the GGX specular cubemap, irradiance SH and BRDF LUT from the
+ * skybox faces (src/texture.rs:199-420; sample counts 64 and 128 as in
+ * src/scene.rs:179, :221).  Arrays are owned by the library until
+ * sr_environment_free. */
+SR_API int sr_prepare_environment(const uint32_t *cubemap_faces, uint32_t face_size,
+                                  int32_t threads, sr_environment *out);
+SR_API int sr_environment_free(sr_environment *env);
+/* Builds the 128^3 voxel grid over the scene bounds: occupancy and active
+ * mask (src/gi.rs:151-265), sun visibility by ray casting
+ * (src/gi.rs:267-314, src/raytracer.rs:172-260), 3x3x3 blur
+ * (src/voxelgrid.rs:371-419).  scene->voxels is ignored. */
+SR_API int sr_prepare_voxel_grid(const sr_scene_desc *scene, int32_t grid_size,
+                                 int32_t threads, sr_voxel_grid *out);
+SR_API int sr_voxel_grid_free(sr_voxel_grid *grid);
+/* Scene bounds over all node-transformed positions (src/scene.rs:333-351). */
+SR_API int sr_scene_desc_bounds(const sr_scene_desc *scene, float min[3], float max[3],
+                                float center[3], float *diagonal);
+/* Mip chain of a texture (src/texture.rs:45-128): writes the level offsets,
+ * widths and heights (at most 32 levels) and, if texels_out != NULL, the
+ * whole chain (*total_texels entries). */
+SR_API int sr_texture_mip_chain(const sr_texture_desc *tex, uint32_t *texels_out,
+                                uint32_t *total_texels, uint32_t *num_levels,
+                                uint32_t offsets[32], uint32_t widths[32],
+                                uint32_t heights[32]);
+
+/* ---- camera ------------------------------------------------------------- */
+/* RenderCamera::new (src/rendercamera.rs:28-63): yaw/pitch from the look-at
+ * direction, near = 0.01 * far.  Matrices are computed by update_matrices. */
+SR_API int sr_camera_new(const float position[3], const float look_at[3], float fov,
+                         float width, float height, float far_plane, sr_camera *out);
+/* A camera given directly by its reference-convention pose. */
+SR_API int sr_camera_from_pose(const float position[3], float yaw, float pitch, float roll,
+                               float fov, float width, float height, float far_plane,
+                               sr_camera *out);
+/* RenderCamera::update_matrices (src/rendercamera.rs:78-86). */
+SR_API int sr_camera_update_matrices(sr_camera *camera);
+
+/* ---- scene (resident in HBM) -------------------------------------------- */
+SR_API int sr_scene_create(const sr_scene_desc *desc, int32_t device, sr_scene **out);
+SR_API int sr_scene_destroy(sr_scene *scene);
+SR_API int sr_scene_info(const sr_scene *scene, uint64_t *triangles, uint64_t *vertices,
+                         uint64_t *device_bytes);
+
+/* ---- renderer: the drop-in per-frame path -------------------------------- */
+SR_API int sr_renderer_create(int32_t width, int32_t height, const sr_renderer_opts *opts,
+                              sr_renderer **out);
+SR_API int sr_renderer_destroy(sr_renderer *renderer);
+/* Renderer::render_scene: transform, clip, bin, raster and shade one frame.
+ * Returns when the frame is complete on the device. */
+SR_API int sr_renderer_render_scene(sr_renderer *renderer, const sr_scene *scene,
+                                    const sr_camera *camera);
+/* Renderer::update_auto_exposure: trimmed mean of the per-64px-tile centre
+ * luminances, exponential adaptation with tau = 1 s. */
+SR_API int sr_renderer_update_auto_exposure(sr_renderer *renderer, float delta_time);
+/* Renderer::blit_to_buffer: exposure, tonemap, quantize, pack into a host
+ * buffer of width*height u32 owned by the caller.  Rows are written in
+ * pairs; an odd last row is left untouched as in the reference. */
+SR_API int sr_renderer_blit_to_buffer(sr_renderer *renderer, uint32_t *pixels,
+                                      size_t width, size_t height);
+/* The same blit into device memory (width*height u32); nothing crosses PCIe. */
+SR_API int sr_renderer_blit_to_device(sr_renderer *renderer, void *device_pixels);
+SR_API float sr_renderer_exposure(const sr_renderer *renderer);
+SR_API int sr_renderer_set_exposure(sr_renderer *renderer, float exposure);
+/* Visibility buffer of the last frame: canonical triangle id (0xFFFFFFFF =
+ * no fragment) and depth (+inf = sky), width*height each; either may be NULL. */
+SR_API int sr_renderer_read_vbuffer(sr_renderer *renderer, uint32_t *tri_id, float *depth);
+/* Linear shaded colour of the last frame, width*height*3 floats. */
+SR_API int sr_renderer_read_color(sr_renderer *renderer, float *rgb);
+/* Centre-quad luminance of every 64px tile (src/tilerasterizer.rs:103-106). */
+SR_API int sr_renderer_read_tile_luminance(sr_renderer *renderer, float *lum, int32_t *count);
+SR_API int sr_renderer_get_stats(const sr_renderer *renderer, sr_frame_stats *stats);
+/* Device pointers of the last frame's visibility buffer and colour (for
+ * in-HBM consumers); valid until the next render call. */
+SR_API int sr_renderer_device_buffers(sr_renderer *renderer, void **tri_id, void **depth,
+                                      void **color_rgb);
+
+/* ---- batched views (one launch sequence for many views) ------------------ */
+SR_API int sr_batch_create(const sr_batch_opts *opts, sr_batch **out);
+SR_API int sr_batch_destroy(sr_batch *batch);
+/* Renders every view; per-view outputs are written to device memory owned
+ * by the batch and read back with sr_batch_read_*. */
+SR_API int sr_batch_render(sr_batch *batch, const sr_view *views, int32_t num_views);
+SR_API int sr_batch_read_vbuffer(sr_batch *batch, int32_t view, uint32_t *tri_id, float *depth);
+SR_API int sr_batch_read_pixels(sr_batch *batch, int32_t view, uint32_t *pixels);
+SR_API int sr_batch_get_stats(const sr_batch *batch, sr_batch_stats *stats);
+
+/* ---- render server: continuous batching over resident scenes ------------- */
+SR_API int sr_server_create(const sr_server_opts *opts, sr_server **out);
+SR_API int sr_server_destroy(sr_server *server);
+SR_API int sr_server_add_scene(sr_server *server, const sr_scene *scene, int32_t *scene_index);
+SR_API int sr_server_submit(sr_server *server, const sr_request *request, uint64_t *request_id);
+/* Pops up to max completions; *count receives how many were written. */
+SR_API int sr_server_poll(sr_server *server, sr_completion *out, int32_t max, int32_t *count);
+SR_API int sr_server_read_pixels(sr_server *server, uint64_t request_id, uint32_t *pixels);
+SR_API int sr_server_read_vbuffer(sr_server *server, uint64_t request_id, uint32_t *tri_id,
+                                  float *depth);
+SR_API int sr_server_release(sr_server *server, uint64_t request_id);
+SR_API int sr_server_drain(sr_server *server);
+SR_API double sr_server_clock(const sr_server *server);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWRASTER_H */

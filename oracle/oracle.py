@@ -1,0 +1,144 @@
+"""ctypes wrapper of the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker / CPU baseline.  The product path
+never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_scene_create.restype = P
+        L.orc_scene_create.argtypes = [P]
+        L.orc_scene_destroy.argtypes = [P]
+        L.orc_renderer_create.restype = P
+        L.orc_renderer_create.argtypes = [C.c_int, C.c_int]
+        L.orc_renderer_destroy.argtypes = [P]
+        L.orc_render.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int]
+        L.orc_stage_ms.restype = C.c_double
+        L.orc_stage_ms.argtypes = [P, C.c_int]
+        L.orc_setup_count.restype = C.c_uint64
+        L.orc_setup_count.argtypes = [P]
+        L.orc_read_frame.argtypes = [P, P, P, P, P, P]
+        L.orc_update_auto_exposure.argtypes = [P, C.c_float]
+        L.orc_exposure.restype = C.c_float
+        L.orc_exposure.argtypes = [P]
+        L.orc_blit.argtypes = [P, P, C.c_size_t, C.c_size_t]
+        L.orc_tile_luminance.argtypes = [P, P]
+        L.orc_camera_new.argtypes = [P, P, C.c_float, C.c_float, C.c_float, C.c_float, P]
+        L.orc_camera_update.argtypes = [P]
+        L.orc_quantize.restype = C.c_uint32
+        L.orc_quantize.argtypes = [C.c_float, C.c_float]
+        L.orc_mip_chain.argtypes = [P, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class OracleScene:
+    def __init__(self, scene_data):
+        self.data = scene_data
+        self.handle = lib().orc_scene_create(C.addressof(scene_data.desc))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().orc_scene_destroy(self.handle)
+            self.handle = None
+
+
+class OracleRenderer:
+    """CPU restatement of Renderer; canonical=True gives the deterministic
+    submission order the GPU vbuffer is compared against."""
+
+    def __init__(self, width, height):
+        self.width, self.height = width, height
+        self.handle = lib().orc_renderer_create(width, height)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().orc_renderer_destroy(self.handle)
+            self.handle = None
+
+    def render_scene(self, scene, camera, threads=1, fp64=False, canonical=True):
+        lib().orc_render(self.handle, scene.handle, C.addressof(camera.c), threads, int(fp64), int(canonical))
+
+    def stage_ms(self):
+        return lib().orc_stage_ms(self.handle, 0), lib().orc_stage_ms(self.handle, 1)
+
+    def setup_count(self):
+        return lib().orc_setup_count(self.handle)
+
+    def read_frame(self, fp64=False, bary=False):
+        n = self.width * self.height
+        ids = np.empty(n, np.uint32)
+        depth = np.empty(n, np.float32)
+        rgb = np.empty(n * 3, np.float32)
+        rgb64 = np.empty(n * 3, np.float64) if fp64 else None
+        b = np.empty(n * 2, np.float32) if bary else None
+        lib().orc_read_frame(self.handle, _p(ids), _p(depth), _p(rgb), _p(rgb64), _p(b))
+        out = {"ids": ids.reshape(self.height, self.width), "depth": depth.reshape(self.height, self.width),
+               "rgb": rgb.reshape(self.height, self.width, 3)}
+        if fp64:
+            out["rgb64"] = rgb64.reshape(self.height, self.width, 3)
+        if bary:
+            out["bary"] = b.reshape(self.height, self.width, 2)
+        return out
+
+    def update_auto_exposure(self, dt):
+        lib().orc_update_auto_exposure(self.handle, float(dt))
+
+    @property
+    def exposure(self):
+        return lib().orc_exposure(self.handle)
+
+    def blit(self):
+        px = np.zeros((self.height, self.width), np.uint32)
+        lib().orc_blit(self.handle, _p(px), self.width, self.height)
+        return px
+
+    def tile_luminance(self):
+        tx = (self.width + 63) // 64
+        ty = (self.height + 63) // 64
+        out = np.empty(tx * ty, np.float32)
+        lib().orc_tile_luminance(self.handle, _p(out))
+        return out
+
+
+def camera_new(sr_camera_cls, position, look_at, fov, width, height, far_plane):
+    c = sr_camera_cls()
+    lib().orc_camera_new((C.c_float * 3)(*position), (C.c_float * 3)(*look_at), fov, width, height, far_plane,
+                         C.byref(c))
+    return c
+
+
+def camera_update(cam_struct):
+    lib().orc_camera_update(C.byref(cam_struct))
+    return cam_struct
+
+
+def quantize(c, exposure):
+    return lib().orc_quantize(c, exposure)
+
+
+def mip_chain(tex_desc):
+    n = C.c_uint32()
+    lib().orc_mip_chain(C.byref(tex_desc), None, C.byref(n))
+    out = np.empty(n.value, np.uint32)
+    lib().orc_mip_chain(C.byref(tex_desc), _p(out), C.byref(n))
+    return out

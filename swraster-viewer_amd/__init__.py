@@ -1,0 +1,640 @@
+"""swraster-viewer_amd -- Python mirror of the MI355X rasterization pipeline.
+
+The product is ``lib/libswraster.so`` (C ABI in ``include/swraster.h``): a C++
+host runtime plus hand-written gfx950 HIP kernels.  This module binds it with
+ctypes and mirrors the reference's per-frame interface:
+
+  RenderCamera.new / update_matrices   src/rendercamera.rs:28, :78
+  Renderer(width, height)              src/renderer.rs:165
+  Renderer.render_scene(scene, camera) src/renderer.rs:201
+  Renderer.update_auto_exposure(dt)    src/renderer.rs:258
+  Renderer.blit_to_buffer(buffer)      src/renderer.rs:293
+
+Errors raise ``SwrasterError`` carrying the library's status and text.  There
+is no CPU fallback: without the built library every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libswraster.so")
+
+SR_OK = 0
+SR_TEX_SRGB, SR_TEX_NORMAL, SR_TEX_METALLIC_ROUGHNESS, SR_TEX_LINEAR = 0, 1, 2, 3
+SR_WRAP_REPEAT, SR_WRAP_MIRRORED_REPEAT, SR_WRAP_CLAMP_TO_EDGE = 0, 1, 2
+SR_ALPHA_OPAQUE, SR_ALPHA_MASK, SR_ALPHA_BLEND = 0, 1, 2
+NO_ID = 0xFFFFFFFF
+
+
+class SwrasterError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"swraster error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# C structures (include/swraster.h)
+# ---------------------------------------------------------------------------
+class sr_texture_desc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("type", C.c_int32),
+                ("texels", C.POINTER(C.c_uint32))]
+
+
+class sr_texref(C.Structure):
+    _fields_ = [("texture", C.c_int32), ("wrap_s", C.c_int32), ("wrap_t", C.c_int32)]
+
+
+class sr_material_desc(C.Structure):
+    _fields_ = [("base_color_factor", C.c_float * 4), ("metallic_factor", C.c_float),
+                ("roughness_factor", C.c_float), ("emissive_factor", C.c_float * 3),
+                ("occlusion_strength", C.c_float), ("alpha_cutoff", C.c_float),
+                ("transmission", C.c_float), ("alpha_mode", C.c_int32), ("is_translucent", C.c_int32),
+                ("base_color", sr_texref), ("metallic_roughness", sr_texref), ("normal", sr_texref),
+                ("emissive", sr_texref), ("occlusion", sr_texref), ("transmission_texture", sr_texref)]
+
+
+class sr_primitive_desc(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("texcoords", C.POINTER(C.c_float)), ("tangents", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("num_vertices", C.c_uint32),
+                ("num_indices", C.c_uint32), ("material", C.c_int32)]
+
+
+class sr_mesh_desc(C.Structure):
+    _fields_ = [("first_primitive", C.c_int32), ("num_primitives", C.c_int32)]
+
+
+class sr_node_desc(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("mesh", C.c_int32)]
+
+
+class sr_environment(C.Structure):
+    _fields_ = [("cubemap_size", C.c_uint32), ("cubemap", C.POINTER(C.c_uint32)),
+                ("specular_size", C.c_uint32), ("specular_mips", C.c_uint32),
+                ("specular", C.POINTER(C.c_uint32)), ("brdf_lut_size", C.c_uint32),
+                ("brdf_lut", C.POINTER(C.c_uint32)), ("irradiance_sh", (C.c_float * 3) * 4)]
+
+
+class sr_voxel_grid(C.Structure):
+    _fields_ = [("dims", C.c_int32 * 3), ("world_min", C.c_float * 3), ("world_max", C.c_float * 3),
+                ("light_intensity", C.POINTER(C.c_float)), ("sh_scale", C.c_float),
+                ("sky_visibility", C.c_float), ("sh4", C.POINTER(C.c_float))]
+
+
+class sr_light(C.Structure):
+    _fields_ = [("direction", C.c_float * 3), ("color", C.c_float * 3)]
+
+
+class sr_scene_desc(C.Structure):
+    _fields_ = [("primitives", C.POINTER(sr_primitive_desc)), ("num_primitives", C.c_int32),
+                ("meshes", C.POINTER(sr_mesh_desc)), ("num_meshes", C.c_int32),
+                ("nodes", C.POINTER(sr_node_desc)), ("num_nodes", C.c_int32),
+                ("materials", C.POINTER(sr_material_desc)), ("num_materials", C.c_int32),
+                ("textures", C.POINTER(sr_texture_desc)), ("num_textures", C.c_int32),
+                ("environment", C.POINTER(sr_environment)), ("voxels", C.POINTER(sr_voxel_grid)),
+                ("light", sr_light)]
+
+
+class sr_camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("fov", C.c_float), ("width", C.c_float),
+                ("height", C.c_float), ("one_over_width", C.c_float), ("one_over_height", C.c_float),
+                ("near_plane", C.c_float), ("far_plane", C.c_float), ("yaw", C.c_float),
+                ("pitch", C.c_float), ("roll", C.c_float), ("view_matrix", C.c_float * 16),
+                ("projection_matrix", C.c_float * 16), ("view_project_matrix", C.c_float * 16),
+                ("inverse_view_project_matrix", C.c_float * 16),
+                ("skybox_matrix_transposed", C.c_float * 16), ("view_clip_planes", (C.c_float * 4) * 6),
+                ("view_normal", C.c_float * 3)]
+
+
+class sr_renderer_opts(C.Structure):
+    _fields_ = [("tile_size", C.c_int32), ("raster_mode", C.c_int32), ("device", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+class sr_frame_stats(C.Structure):
+    _fields_ = [("input_triangles", C.c_uint64), ("setup_triangles", C.c_uint64),
+                ("binned_pairs", C.c_uint64), ("tiles", C.c_uint32), ("draws", C.c_uint32),
+                ("ms_setup", C.c_float), ("ms_bin", C.c_float), ("ms_raster_shade", C.c_float),
+                ("ms_blit", C.c_float), ("ms_total", C.c_float)]
+
+
+class sr_view(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", sr_camera), ("width", C.c_int32),
+                ("height", C.c_int32), ("exposure", C.c_float)]
+
+
+class sr_batch_opts(C.Structure):
+    _fields_ = [("tile_size", C.c_int32), ("use_graph", C.c_int32), ("persistent_blocks", C.c_int32),
+                ("device", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class sr_batch_stats(C.Structure):
+    _fields_ = [("input_triangles", C.c_uint64), ("setup_triangles", C.c_uint64),
+                ("binned_pairs", C.c_uint64), ("pixels", C.c_uint64), ("views", C.c_uint32),
+                ("work_items", C.c_uint32), ("ms_total", C.c_float)]
+
+
+class sr_server_opts(C.Structure):
+    _fields_ = [("admission_bytes", C.c_uint64), ("max_batch_views", C.c_int32), ("streams", C.c_int32),
+                ("tile_size", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class sr_request(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("camera", sr_camera), ("width", C.c_int32), ("height", C.c_int32),
+                ("exposure", C.c_float), ("user_tag", C.c_uint64)]
+
+
+class sr_completion(C.Structure):
+    _fields_ = [("request_id", C.c_uint64), ("user_tag", C.c_uint64), ("status", C.c_int32),
+                ("width", C.c_int32), ("height", C.c_int32), ("submit_time_s", C.c_double),
+                ("start_time_s", C.c_double), ("complete_time_s", C.c_double)]
+
+
+# Every entry point declared in include/swraster.h, with its argument types.
+_P = C.c_void_p
+_SIGS = {
+    "sr_last_error": (C.c_char_p, []),
+    "sr_version": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "sr_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "sr_prepare_environment": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32, C.c_int32, C.POINTER(sr_environment)]),
+    "sr_environment_free": (C.c_int, [C.POINTER(sr_environment)]),
+    "sr_prepare_voxel_grid": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32, C.POINTER(sr_voxel_grid)]),
+    "sr_voxel_grid_free": (C.c_int, [C.POINTER(sr_voxel_grid)]),
+    "sr_scene_desc_bounds": (C.c_int, [C.POINTER(sr_scene_desc), C.c_float * 3, C.c_float * 3, C.c_float * 3,
+                                       C.POINTER(C.c_float)]),
+    "sr_texture_mip_chain": (C.c_int, [C.POINTER(sr_texture_desc), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                       C.POINTER(C.c_uint32), C.c_uint32 * 32, C.c_uint32 * 32, C.c_uint32 * 32]),
+    "sr_camera_new": (C.c_int, [C.c_float * 3, C.c_float * 3, C.c_float, C.c_float, C.c_float, C.c_float,
+                                C.POINTER(sr_camera)]),
+    "sr_camera_from_pose": (C.c_int, [C.c_float * 3, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, C.POINTER(sr_camera)]),
+    "sr_camera_update_matrices": (C.c_int, [C.POINTER(sr_camera)]),
+    "sr_scene_create": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.POINTER(_P)]),
+    "sr_scene_destroy": (C.c_int, [_P]),
+    "sr_scene_info": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "sr_renderer_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(sr_renderer_opts), C.POINTER(_P)]),
+    "sr_renderer_destroy": (C.c_int, [_P]),
+    "sr_renderer_render_scene": (C.c_int, [_P, _P, C.POINTER(sr_camera)]),
+    "sr_renderer_update_auto_exposure": (C.c_int, [_P, C.c_float]),
+    "sr_renderer_blit_to_buffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t]),
+    "sr_renderer_blit_to_device": (C.c_int, [_P, _P]),
+    "sr_renderer_exposure": (C.c_float, [_P]),
+    "sr_renderer_set_exposure": (C.c_int, [_P, C.c_float]),
+    "sr_renderer_read_vbuffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "sr_renderer_read_color": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "sr_renderer_read_tile_luminance": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
+    "sr_renderer_get_stats": (C.c_int, [_P, C.POINTER(sr_frame_stats)]),
+    "sr_renderer_device_buffers": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "sr_batch_create": (C.c_int, [C.POINTER(sr_batch_opts), C.POINTER(_P)]),
+    "sr_batch_destroy": (C.c_int, [_P]),
+    "sr_batch_render": (C.c_int, [_P, C.POINTER(sr_view), C.c_int32]),
+    "sr_batch_read_vbuffer": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "sr_batch_read_pixels": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32)]),
+    "sr_batch_get_stats": (C.c_int, [_P, C.POINTER(sr_batch_stats)]),
+    "sr_server_create": (C.c_int, [C.POINTER(sr_server_opts), C.POINTER(_P)]),
+    "sr_server_destroy": (C.c_int, [_P]),
+    "sr_server_add_scene": (C.c_int, [_P, _P, C.POINTER(C.c_int32)]),
+    "sr_server_submit": (C.c_int, [_P, C.POINTER(sr_request), C.POINTER(C.c_uint64)]),
+    "sr_server_poll": (C.c_int, [_P, C.POINTER(sr_completion), C.c_int32, C.POINTER(C.c_int32)]),
+    "sr_server_read_pixels": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_uint32)]),
+    "sr_server_read_vbuffer": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+    "sr_server_release": (C.c_int, [_P, C.c_uint64]),
+    "sr_server_drain": (C.c_int, [_P]),
+    "sr_server_clock": (C.c_double, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded product library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SwrasterError(-1, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != SR_OK:
+        raise SwrasterError(code, lib().sr_last_error().decode(errors="replace"))
+    return code
+
+
+def device_count():
+    n = C.c_int32(0)
+    check(lib().sr_device_count(C.byref(n)))
+    return n.value
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+def pack_rgba(rgba):
+    """(..., 4) uint8 -> packed r<<24 | g<<16 | b<<8 | a (src/util.rs:98-100)."""
+    rgba = np.asarray(rgba, dtype=np.uint32)
+    return ((rgba[..., 0] << 24) | (rgba[..., 1] << 16) | (rgba[..., 2] << 8) | rgba[..., 3]).astype(np.uint32)
+
+
+def unpack_rgba(px):
+    px = np.asarray(px, dtype=np.uint32)
+    return np.stack([(px >> 24) & 255, (px >> 16) & 255, (px >> 8) & 255, px & 255], -1).astype(np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# Camera (RenderCamera, src/rendercamera.rs)
+# ---------------------------------------------------------------------------
+class RenderCamera:
+    """Mirror of RenderCamera: ``new`` derives yaw/pitch from a look-at point
+    (with the reference's pitch convention), ``from_pose`` takes them
+    directly; ``update_matrices`` fills view/projection/skybox matrices and
+    the view-space cull planes."""
+
+    def __init__(self, cam):
+        self.c = cam
+
+    @classmethod
+    def new(cls, position, look_at, fov, width, height, far_plane):
+        c = sr_camera()
+        check(lib().sr_camera_new((C.c_float * 3)(*position), (C.c_float * 3)(*look_at), fov, width, height,
+                                  far_plane, C.byref(c)))
+        return cls(c)
+
+    @classmethod
+    def from_pose(cls, position, yaw, pitch, fov, width, height, far_plane, roll=0.0):
+        c = sr_camera()
+        check(lib().sr_camera_from_pose((C.c_float * 3)(*position), yaw, pitch, roll, fov, width, height,
+                                        far_plane, C.byref(c)))
+        return cls(c)
+
+    def update_matrices(self):
+        check(lib().sr_camera_update_matrices(C.byref(self.c)))
+        return self
+
+    def matrix(self, name):
+        return np.array(getattr(self.c, name), dtype=np.float32).reshape(4, 4).T  # row-major view
+
+
+# ---------------------------------------------------------------------------
+# Environment and voxel inputs
+# ---------------------------------------------------------------------------
+CACHE_DIR = os.path.join(REPO_DIR, "assets", ".cache")
+
+
+def cubemap_faces():
+    """Skybox faces (+X,-X,+Y,-Y,+Z,-Z) packed RGBA8 with alpha 255."""
+    faces = np.load(os.path.join(REPO_DIR, "assets", "cubemap_faces.npz"))["faces"]
+    rgba = np.concatenate([faces, np.full(faces.shape[:-1] + (1,), 255, np.uint8)], -1)
+    return np.ascontiguousarray(pack_rgba(rgba).reshape(6, faces.shape[1], faces.shape[2]))
+
+
+class Environment:
+    """Image-based lighting inputs (src/scene.rs:152-228), cached on disk."""
+
+    def __init__(self, cubemap, specular, specular_mips, lut, sh):
+        self.cubemap = np.ascontiguousarray(cubemap, np.uint32)
+        self.specular = np.ascontiguousarray(specular, np.uint32)
+        self.specular_mips = int(specular_mips)
+        self.lut = np.ascontiguousarray(lut, np.uint32)
+        self.sh = np.ascontiguousarray(sh, np.float32).reshape(4, 3)
+        self.desc = sr_environment()
+        self.desc.cubemap_size = self.cubemap.shape[-1]
+        self.desc.cubemap = _uptr(self.cubemap)
+        self.desc.specular_size = self.cubemap.shape[-1]
+        self.desc.specular_mips = self.specular_mips
+        self.desc.specular = _uptr(self.specular)
+        self.desc.brdf_lut_size = int(round(np.sqrt(self.lut.size)))
+        self.desc.brdf_lut = _uptr(self.lut)
+        for i in range(4):
+            for k in range(3):
+                self.desc.irradiance_sh[i][k] = float(self.sh[i, k])
+
+    @classmethod
+    def load(cls, threads=0, cache=True):
+        path = os.path.join(CACHE_DIR, "environment_v1.npz")
+        faces = cubemap_faces()
+        if cache and os.path.exists(path):
+            z = np.load(path)
+            if np.array_equal(z["cubemap"], faces):
+                return cls(z["cubemap"], z["specular"], int(z["mips"]), z["lut"], z["sh"])
+        env = sr_environment()
+        check(lib().sr_prepare_environment(_uptr(faces), faces.shape[-1], threads, C.byref(env)))
+        size = env.cubemap_size
+        spec = np.ctypeslib.as_array(env.specular, shape=(env.specular_mips * 6 * size * size,)).copy()
+        lut = np.ctypeslib.as_array(env.brdf_lut, shape=(env.brdf_lut_size ** 2,)).copy()
+        sh = np.array([[env.irradiance_sh[i][k] for k in range(3)] for i in range(4)], np.float32)
+        mips = env.specular_mips
+        lib().sr_environment_free(C.byref(env))
+        if cache:
+            os.makedirs(CACHE_DIR, exist_ok=True)
+            np.savez(path, cubemap=faces, specular=spec, mips=mips, lut=lut, sh=sh)
+        return cls(faces, spec, mips, lut, sh)
+
+
+class VoxelGrid:
+    def __init__(self, dims, wmin, wmax, light, sh_scale=0.25, sky=1.0):
+        self.light = np.ascontiguousarray(light, np.float32).ravel()
+        self.desc = sr_voxel_grid()
+        for k in range(3):
+            self.desc.dims[k] = int(dims[k])
+            self.desc.world_min[k] = float(wmin[k])
+            self.desc.world_max[k] = float(wmax[k])
+        self.desc.light_intensity = _fptr(self.light)
+        self.desc.sh_scale = sh_scale
+        self.desc.sky_visibility = sky
+        self.desc.sh4 = None
+        self.dims = tuple(int(d) for d in dims)
+        self.wmin = np.array(wmin, np.float32)
+        self.wmax = np.array(wmax, np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Scene description (host arrays) and the HBM-resident scene
+# ---------------------------------------------------------------------------
+def default_light():
+    d = np.array([-0.2, 1.0, 0.5], np.float32)
+    # Vec3A::normalize: v * (1 / sqrt(dot)) in f32 (src/scene.rs:237)
+    dot = np.float32(np.float32(d[0] * d[0]) + np.float32(d[1] * d[1])) + np.float32(d[2] * d[2])
+    r = np.float32(1.0) / np.float32(np.sqrt(dot))
+    col = np.array([1.0, 1.0, 0.95], np.float32) * np.float32(5.0)
+    return (d * r).astype(np.float32), col.astype(np.float32)
+
+
+class SceneData:
+    """Host-side scene: primitives, meshes, nodes, materials, textures.
+
+    primitives: list of dicts with positions (N,3) f32, normals (N,3) f32,
+    texcoords (N,2) f32 or None, tangents (N,4) f32 or None, indices (M,) u32,
+    material int.  meshes: list of (first_primitive, num_primitives).
+    nodes: list of (transform (4,4) row-major f32, mesh index or -1).
+    materials: list of dicts (see material()).  textures: list of
+    (width, height, type, packed u32 texels)."""
+
+    def __init__(self, primitives, meshes, nodes, materials, textures, name="scene"):
+        self.primitives = primitives
+        self.meshes = meshes
+        self.nodes = nodes
+        self.materials = materials
+        self.textures = textures
+        self.name = name
+        self.light_dir, self.light_color = default_light()
+        self.environment = None
+        self.voxels = None
+        self._build_desc()
+
+    @property
+    def triangles(self):
+        return sum(len(p["indices"]) // 3 for p in self.primitives)
+
+    def _build_desc(self):
+        keep = []
+        prims = (sr_primitive_desc * max(1, len(self.primitives)))()
+        for i, p in enumerate(self.primitives):
+            for k in ("positions", "normals", "texcoords", "tangents"):
+                if p.get(k) is not None:
+                    p[k] = np.ascontiguousarray(p[k], np.float32)
+            p["indices"] = np.ascontiguousarray(p["indices"], np.uint32).ravel()
+            prims[i].positions = _fptr(p["positions"])
+            prims[i].normals = _fptr(p["normals"])
+            prims[i].texcoords = _fptr(p.get("texcoords"))
+            prims[i].tangents = _fptr(p.get("tangents"))
+            prims[i].indices = _uptr(p["indices"])
+            prims[i].num_vertices = len(p["positions"])
+            prims[i].num_indices = len(p["indices"])
+            prims[i].material = int(p.get("material", 0))
+        meshes = (sr_mesh_desc * max(1, len(self.meshes)))()
+        for i, (f, n) in enumerate(self.meshes):
+            meshes[i].first_primitive = f
+            meshes[i].num_primitives = n
+        nodes = (sr_node_desc * max(1, len(self.nodes)))()
+        for i, (t, m) in enumerate(self.nodes):
+            col = np.asarray(t, np.float32).reshape(4, 4).T.ravel()  # column-major
+            for k in range(16):
+                nodes[i].transform[k] = float(col[k])
+            nodes[i].mesh = int(m)
+        mats = (sr_material_desc * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            d = mats[i]
+            for k in range(4):
+                d.base_color_factor[k] = float(m["base_color"][k])
+            d.metallic_factor = float(m.get("metallic", 1.0))
+            d.roughness_factor = float(m.get("roughness", 1.0))
+            for k in range(3):
+                d.emissive_factor[k] = float(m.get("emissive", (0, 0, 0))[k])
+            d.occlusion_strength = float(m.get("occlusion_strength", 1.0))
+            d.alpha_cutoff = float(m.get("alpha_cutoff", 0.5))
+            d.transmission = float(m.get("transmission", 0.0))
+            d.alpha_mode = int(m.get("alpha_mode", SR_ALPHA_OPAQUE))
+            d.is_translucent = int(m.get("is_translucent", 0))
+            for slot in ("base_color", "metallic_roughness", "normal", "emissive", "occlusion",
+                         "transmission_texture"):
+                ref = m.get(slot + "_tex")
+                r = getattr(d, slot)
+                if ref is None:
+                    r.texture, r.wrap_s, r.wrap_t = -1, SR_WRAP_REPEAT, SR_WRAP_REPEAT
+                else:
+                    r.texture, r.wrap_s, r.wrap_t = ref[0], ref[1], ref[2]
+        texs = (sr_texture_desc * max(1, len(self.textures)))()
+        for i, (w, h, ty, data) in enumerate(self.textures):
+            data = np.ascontiguousarray(data, np.uint32).ravel()
+            self.textures[i] = (w, h, ty, data)
+            texs[i].width, texs[i].height, texs[i].type = w, h, ty
+            texs[i].texels = _uptr(data)
+        d = sr_scene_desc()
+        d.primitives, d.num_primitives = prims, len(self.primitives)
+        d.meshes, d.num_meshes = meshes, len(self.meshes)
+        d.nodes, d.num_nodes = nodes, len(self.nodes)
+        d.materials, d.num_materials = mats, len(self.materials)
+        d.textures, d.num_textures = texs, len(self.textures)
+        for k in range(3):
+            d.light.direction[k] = float(self.light_dir[k])
+            d.light.color[k] = float(self.light_color[k])
+        keep += [prims, meshes, nodes, mats, texs]
+        self._keep = keep
+        self.desc = d
+
+    def bounds(self):
+        mn, mx, ce = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
+        dg = C.c_float()
+        check(lib().sr_scene_desc_bounds(C.byref(self.desc), mn, mx, ce, C.byref(dg)))
+        return np.array(mn[:], np.float32), np.array(mx[:], np.float32), np.array(ce[:], np.float32), dg.value
+
+    def attach(self, environment, voxels):
+        self.environment = environment
+        self.voxels = voxels
+        self.desc.environment = C.pointer(environment.desc)
+        self.desc.voxels = C.pointer(voxels.desc)
+        return self
+
+    def prepare(self, environment=None, grid=128, threads=0, cache_key=None):
+        """Attach the IBL environment and build (or load) the voxel grid."""
+        environment = environment or Environment.load(threads=threads)
+        path = None
+        if cache_key:
+            path = os.path.join(CACHE_DIR, f"voxels_{cache_key}_{grid}.npz")
+            if os.path.exists(path):
+                z = np.load(path)
+                return self.attach(environment, VoxelGrid(z["dims"], z["wmin"], z["wmax"], z["light"]))
+        g = sr_voxel_grid()
+        check(lib().sr_prepare_voxel_grid(C.byref(self.desc), grid, threads, C.byref(g)))
+        n = g.dims[0] * g.dims[1] * g.dims[2]
+        light = np.ctypeslib.as_array(g.light_intensity, shape=(n,)).copy()
+        dims, wmin, wmax = list(g.dims), list(g.world_min), list(g.world_max)
+        lib().sr_voxel_grid_free(C.byref(g))
+        if path:
+            os.makedirs(CACHE_DIR, exist_ok=True)
+            np.savez(path, dims=dims, wmin=wmin, wmax=wmax, light=light)
+        return self.attach(environment, VoxelGrid(dims, wmin, wmax, light))
+
+
+class Scene:
+    """A scene resident in HBM (sr_scene_create)."""
+
+    def __init__(self, data, device=0):
+        if data.environment is None or data.voxels is None:
+            data.prepare()
+        self.data = data
+        self.handle = C.c_void_p()
+        check(lib().sr_scene_create(C.byref(data.desc), device, C.byref(self.handle)))
+
+    def info(self):
+        t, v, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().sr_scene_info(self.handle, C.byref(t), C.byref(v), C.byref(b)))
+        return {"triangles": t.value, "vertices": v.value, "device_bytes": b.value}
+
+    def close(self):
+        if self.handle:
+            lib().sr_scene_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Renderer:
+    """Mirror of Renderer (src/renderer.rs:145-355) on one GPU."""
+
+    def __init__(self, width, height, tile_size=0, device=0):
+        self.width, self.height = int(width), int(height)
+        o = sr_renderer_opts()
+        o.tile_size = tile_size
+        o.device = device
+        self.handle = C.c_void_p()
+        check(lib().sr_renderer_create(self.width, self.height, C.byref(o), C.byref(self.handle)))
+
+    def render_scene(self, scene, camera):
+        check(lib().sr_renderer_render_scene(self.handle, scene.handle, C.byref(camera.c)))
+
+    def update_auto_exposure(self, delta_time):
+        check(lib().sr_renderer_update_auto_exposure(self.handle, float(delta_time)))
+
+    @property
+    def exposure(self):
+        return lib().sr_renderer_exposure(self.handle)
+
+    def blit_to_buffer(self, buffer=None):
+        if buffer is None:
+            buffer = np.zeros((self.height, self.width), np.uint32)
+        check(lib().sr_renderer_blit_to_buffer(self.handle, _uptr(buffer), self.width, self.height))
+        return buffer
+
+    def read_vbuffer(self):
+        ids = np.empty((self.height, self.width), np.uint32)
+        depth = np.empty((self.height, self.width), np.float32)
+        check(lib().sr_renderer_read_vbuffer(self.handle, _uptr(ids), _fptr(depth)))
+        return ids, depth
+
+    def read_color(self):
+        rgb = np.empty((self.height, self.width, 3), np.float32)
+        check(lib().sr_renderer_read_color(self.handle, _fptr(rgb)))
+        return rgb
+
+    def read_tile_luminance(self):
+        n = C.c_int32()
+        check(lib().sr_renderer_read_tile_luminance(self.handle, None, C.byref(n)))
+        out = np.empty(n.value, np.float32)
+        check(lib().sr_renderer_read_tile_luminance(self.handle, _fptr(out), C.byref(n)))
+        return out
+
+    def stats(self):
+        s = sr_frame_stats()
+        check(lib().sr_renderer_get_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in sr_frame_stats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().sr_renderer_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many views (scene, camera, resolution) in one launch sequence with a
+    persistent raster kernel draining (view, tile) items."""
+
+    def __init__(self, tile_size=0, use_graph=False, persistent_blocks=0, device=0):
+        o = sr_batch_opts()
+        o.tile_size, o.use_graph, o.persistent_blocks, o.device = tile_size, int(use_graph), persistent_blocks, device
+        self.handle = C.c_void_p()
+        check(lib().sr_batch_create(C.byref(o), C.byref(self.handle)))
+        self.sizes = []
+
+    def render(self, views):
+        arr = (sr_view * len(views))()
+        for i, (scene, cam, w, h, exposure) in enumerate(views):
+            arr[i].scene = scene.handle.value
+            arr[i].camera = cam.c
+            arr[i].width, arr[i].height, arr[i].exposure = w, h, exposure
+        check(lib().sr_batch_render(self.handle, arr, len(views)))
+        self.sizes = [(w, h) for (_, _, w, h, _) in views]
+
+    def read_vbuffer(self, i):
+        w, h = self.sizes[i]
+        ids = np.empty((h, w), np.uint32)
+        depth = np.empty((h, w), np.float32)
+        check(lib().sr_batch_read_vbuffer(self.handle, i, _uptr(ids), _fptr(depth)))
+        return ids, depth
+
+    def read_pixels(self, i):
+        w, h = self.sizes[i]
+        px = np.zeros((h, w), np.uint32)
+        check(lib().sr_batch_read_pixels(self.handle, i, _uptr(px)))
+        return px
+
+    def stats(self):
+        s = sr_batch_stats()
+        check(lib().sr_batch_get_stats(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in sr_batch_stats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().sr_batch_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,56 @@
+// kernels.h -- host-callable launchers for the gfx950 kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "common.h"
+
+namespace sr {
+
+struct FrameArgs {
+    // inputs
+    const DevView *views;         // device
+    int32_t num_views;
+    const DevDraw *draws;         // device
+    int32_t num_draws;
+    uint32_t total_xv;            // transformed vertices (sum over draws)
+    uint32_t total_in;            // input triangles (sum over draws)
+    uint32_t total_items;         // (view, tile) items == bins
+    int32_t tile_size;            // 8, 16 or 32
+    int32_t any_tangents;
+    // transient buffers
+    float4 *xv;                   // 3 float4 per transformed vertex
+    float4 *xt;                   // tangent float4 per transformed vertex (or null)
+    uint32_t *tri_counts;         // total_in (+1)
+    uint32_t *tri_base;           // total_in (+1)
+    uint32_t *bin_count;          // total_items
+    uint32_t *bin_start;          // total_items
+    uint32_t *bin_end;            // total_items (cursor, then end)
+    uint32_t *scan_partials;      // scratch for scans
+    uint32_t *list;               // pair_cap
+    uint4 *rec_geo;               // rec_cap
+    float4 *rec_raster;           // rec_cap * 4
+    float4 *rec_shade;            // rec_cap * 8
+    float4 *rec_tan;              // rec_cap * 3 (or null)
+    uint32_t rec_cap, pair_cap;
+    DevStatus *status;
+    // outputs (per view at DevView.fb_offset)
+    uint32_t *out_ids;
+    float *out_depth;
+    float *out_color;             // three planes of color_plane floats
+    uint64_t color_plane;
+    uint32_t *out_rgba;           // fused blit target (views with flags & 1)
+    float *out_lum;
+    int32_t persistent_blocks;    // 0 = one block per item
+    hipEvent_t events[4];         // optional stage markers: start, setup done, bins done, raster done
+};
+
+// Launch the whole per-frame sequence on a stream.  Returns hipSuccess or the
+// first launch error.
+hipError_t launch_frame(const FrameArgs &a, hipStream_t stream);
+
+// Exposure + tonemap + quantize + pack of one view's linear colour planes.
+hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
+                       hipStream_t stream);
+
+}  // namespace sr

@@ -1,0 +1,1254 @@
+// kernels.hip -- the per-frame rasterization pipeline for gfx950 (MI355X).
+//
+// Stage map (reference paths are relative to the reference tree):
+//   k_transform     SoA vertex fetch, MVP / model / normal transform, once per
+//                   (draw, vertex)          src/renderer.rs:499-521
+//   k_clip_count    frustum clipping of clip-space positions to count fan
+//                   triangles               src/renderer.rs:579-665
+//   k_scan_*        device-wide exclusive scan (post-clip ids, bin offsets)
+//   k_setup         clipping with attributes, snap, cull, bbox, setup records,
+//                   per-tile bin counts     src/renderer.rs:668-846
+//   k_scatter       (triangle, tile) pairs into per-tile lists
+//                                           src/bumpqueue.rs:96-112
+//   k_raster_shade  one workgroup per screen tile: coverage with the
+//                   reference's exact f32 edge-evaluation chains into a 64-bit
+//                   (depth, id) LDS buffer, then visibility-buffer PBR
+//                   shading, skybox and centre luminance
+//                                           src/tilerasterizer.rs:72-543,
+//                                           src/shader.rs:102-329
+//   k_blit          exposure, tonemap, quantize, pack
+//                                           src/renderer.rs:293-355
+//
+// Arithmetic contract: built with -ffp-contract=off, IEEE division and
+// square root, denormals preserved; vector min/max/clamp use the SSE form
+// (a > b ? a : b); Rust `as` casts saturate with NaN -> 0; i32 wraps.
+//
+// Depth resolution: the reference's per-tile sequential depth test
+// (z <= current, later fragments win ties) leaves at each pixel the last
+// fragment, in submission order, among those with the minimal depth; NaN
+// depths never pass and +0 == -0.  The kernel computes exactly that as an
+// order-independent minimum over 64-bit keys (orderable depth << 32 |
+// ~triangle id), so the per-tile lists may be in any order and the result
+// is still deterministic.  The winner's barycentrics and depth bits are then
+// recomputed with the same evaluation chain.
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "swraster.h"
+
+namespace sr {
+
+// ---------------------------------------------------------------------------
+// arithmetic helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float maxps(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float minps(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float clampv(float x, float lo, float hi) { return minps(maxps(x, lo), hi); }
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+__device__ __forceinline__ int32_t wabs(int32_t a) { return a < 0 ? (int32_t)(0u - (uint32_t)a) : a; }
+__device__ __forceinline__ int32_t sat_i32(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+__device__ __forceinline__ uint32_t sat_u32(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
+__device__ __forceinline__ uint32_t sat_u8(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 255.0f) return 255u;
+    return (uint32_t)f;
+}
+// `as usize` then min(limit): saturating, NaN and negatives -> 0
+__device__ __forceinline__ uint32_t sat_index(float f, uint32_t limit) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= (float)limit) return limit;
+    uint32_t v = (uint32_t)f;
+    return v < limit ? v : limit;
+}
+__device__ __forceinline__ float rsqrt_ieee(float x) { return 1.0f / sqrtf(x); }
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 muls(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 adds(f3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 normalize3(f3 a) {
+    float r = rsqrt_ieee(dot3(a, a));
+    return {a.x * r, a.y * r, a.z * r};
+}
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 max3s(f3 a, float m) { return {maxps(a.x, m), maxps(a.y, m), maxps(a.z, m)}; }
+// srgb_to_linear_fast (src/util.rs:111-113)
+__device__ __forceinline__ f3 srgb_fast(f3 x) {
+    f3 r;
+    r.x = x.x * (x.x * (x.x * 0.305306011f + 0.682171111f) + 0.012522878f);
+    r.y = x.y * (x.y * (x.y * 0.305306011f + 0.682171111f) + 0.012522878f);
+    r.z = x.z * (x.z * (x.z * 0.305306011f + 0.682171111f) + 0.012522878f);
+    return r;
+}
+__device__ __forceinline__ f3 unpack_rgb(uint32_t c) {
+    return {(float)((c >> 24) & 0xFFu) / 255.0f, (float)((c >> 16) & 0xFFu) / 255.0f,
+            (float)((c >> 8) & 0xFFu) / 255.0f};
+}
+__device__ __forceinline__ float unpack_a(uint32_t c) { return (float)(c & 0xFFu) / 255.0f; }
+
+template <class T>
+__device__ __forceinline__ int find_last_le(const T *arr, int n, uint32_t key, uint32_t T::*field) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (arr[mid].*field <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Clip-space vertex with attributes (Vertex, src/renderer.rs:30-37)
+// ---------------------------------------------------------------------------
+struct CV {
+    float cx, cy, cz, cw;         // pos_clip
+    float wx, wy, wz;             // pos_world
+    float nx, ny, nz;             // normal
+    float u, v;                   // uv
+    float tx, ty, tz, tw;         // tangent
+};
+
+__device__ __forceinline__ void load_cv(const float4 *xv, const float4 *xt, uint32_t i, CV &o) {
+    float4 a = xv[3 * (size_t)i], b = xv[3 * (size_t)i + 1], c = xv[3 * (size_t)i + 2];
+    o.cx = a.x; o.cy = a.y; o.cz = a.z; o.cw = a.w;
+    o.wx = b.x; o.wy = b.y; o.wz = b.z; o.nx = b.w;
+    o.ny = c.x; o.nz = c.y; o.u = c.z; o.v = c.w;
+    if (xt) {
+        float4 t = xt[i];
+        o.tx = t.x; o.ty = t.y; o.tz = t.z; o.tw = t.w;
+    } else {
+        o.tx = o.ty = o.tz = o.tw = 0.0f;
+    }
+}
+
+// Plane dot products in glam's SSE Vec4::dot order (x*x' + z*z') + (y*y' + w*w')
+// for the six clip planes of src/renderer.rs:581-588.
+__device__ __forceinline__ float plane_dot(int p, float x, float y, float z, float w) {
+    const float PX[6] = {0.0f, 0.0f, 1.0f, -1.0f, 0.0f, 0.0f};
+    const float PY[6] = {0.0f, 0.0f, 0.0f, 0.0f, 1.0f, -1.0f};
+    const float PZ[6] = {1.0f, -1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float xx = PX[p] * x, yy = PY[p] * y, zz = PZ[p] * z, ww = 1.0f * w;
+    return (xx + zz) + (yy + ww);
+}
+
+__device__ __forceinline__ void lerp_cv(const CV &a, const CV &b, float t, CV &o) {
+    o.cx = a.cx + (b.cx - a.cx) * t; o.cy = a.cy + (b.cy - a.cy) * t;
+    o.cz = a.cz + (b.cz - a.cz) * t; o.cw = a.cw + (b.cw - a.cw) * t;
+    o.wx = a.wx + (b.wx - a.wx) * t; o.wy = a.wy + (b.wy - a.wy) * t; o.wz = a.wz + (b.wz - a.wz) * t;
+    o.nx = a.nx + (b.nx - a.nx) * t; o.ny = a.ny + (b.ny - a.ny) * t; o.nz = a.nz + (b.nz - a.nz) * t;
+    o.tx = a.tx + (b.tx - a.tx) * t; o.ty = a.ty + (b.ty - a.ty) * t;
+    o.tz = a.tz + (b.tz - a.tz) * t; o.tw = a.tw + (b.tw - a.tw) * t;
+    o.u = a.u + (b.u - a.u) * t; o.v = a.v + (b.v - a.v) * t;
+}
+
+// Sutherland-Hodgman from plane `first` on (planes before it left the
+// triangle unchanged).  Returns the polygon size (0 or >= 3) in poly.
+__device__ __noinline__ int clip_general(CV *poly, CV *np, int first) {
+    int len = 3;
+    for (int p = first; p < 6; p++) {
+        if (len == 0) return 0;
+        int nl = 0;
+        for (int i = 0; i < len; i++) {
+            const CV &cur = poly[i];
+            const CV &prev = poly[(i + len - 1) % len];
+            float dc = plane_dot(p, cur.cx, cur.cy, cur.cz, cur.cw);
+            float dp = plane_dot(p, prev.cx, prev.cy, prev.cz, prev.cw);
+            bool ci = dc >= 0.0f, pi = dp >= 0.0f;
+            if (ci) {
+                if (!pi) {
+                    float t = dp / (dp - dc);
+                    lerp_cv(prev, cur, t, np[nl++]);
+                }
+                np[nl++] = cur;
+            } else if (pi) {
+                float t = dp / (dp - dc);
+                lerp_cv(prev, cur, t, np[nl++]);
+            }
+        }
+        for (int i = 0; i < nl; i++) poly[i] = np[i];
+        len = nl;
+    }
+    return len < 3 ? 0 : len;
+}
+
+// Position-only pre-pass: planes whose test leaves the triangle untouched are
+// skipped; returns -1 if the triangle passes all planes unchanged, -2 if it is
+// rejected whole by a plane, or the first plane that cuts it.
+__device__ __forceinline__ int clip_classify(const float c[3][4]) {
+    for (int p = 0; p < 6; p++) {
+        int in = 0;
+        for (int k = 0; k < 3; k++) in += plane_dot(p, c[k][0], c[k][1], c[k][2], c[k][3]) >= 0.0f ? 1 : 0;
+        if (in == 3) continue;
+        if (in == 0) return -2;
+        return p;
+    }
+    return -1;
+}
+
+__device__ __noinline__ int clip_count_general(const float c[3][4], int first) {
+    float poly[16][4], np[16][4];
+    for (int k = 0; k < 3; k++)
+        for (int j = 0; j < 4; j++) poly[k][j] = c[k][j];
+    int len = 3;
+    for (int p = first; p < 6; p++) {
+        if (len == 0) return 0;
+        int nl = 0;
+        for (int i = 0; i < len; i++) {
+            const float *cur = poly[i];
+            const float *prev = poly[(i + len - 1) % len];
+            float dc = plane_dot(p, cur[0], cur[1], cur[2], cur[3]);
+            float dp = plane_dot(p, prev[0], prev[1], prev[2], prev[3]);
+            bool ci = dc >= 0.0f, pi = dp >= 0.0f;
+            if (ci) {
+                if (!pi) {
+                    float t = dp / (dp - dc);
+                    for (int j = 0; j < 4; j++) np[nl][j] = prev[j] + (cur[j] - prev[j]) * t;
+                    nl++;
+                }
+                for (int j = 0; j < 4; j++) np[nl][j] = cur[j];
+                nl++;
+            } else if (pi) {
+                float t = dp / (dp - dc);
+                for (int j = 0; j < 4; j++) np[nl][j] = prev[j] + (cur[j] - prev[j]) * t;
+                nl++;
+            }
+        }
+        for (int i = 0; i < nl; i++)
+            for (int j = 0; j < 4; j++) poly[i][j] = np[i][j];
+        len = nl;
+    }
+    return len < 3 ? 0 : len - 2;
+}
+
+// ---------------------------------------------------------------------------
+// k_transform: one thread per (draw, vertex)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_transform(const DevView *views, const DevDraw *draws, int ndraws,
+                                                   uint32_t total, float4 *xv, float4 *xt) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        int d = find_last_le(draws, ndraws, i, &DevDraw::xv_base);
+        const DevDraw &D = draws[d];
+        const DevScene &S = *views[D.view].scene;
+        uint32_t s = D.vtx_base + (i - D.xv_base);
+        float x = S.px[s], y = S.py[s], z = S.pz[s];
+        float nx = S.nx[s], ny = S.ny[s], nz = S.nz[s];
+        float c[4], w[3], n[3];
+        for (int r = 0; r < 4; r++) {
+            float acc = D.mvp[r] * x;
+            acc = acc + D.mvp[4 + r] * y;
+            acc = acc + D.mvp[8 + r] * z;
+            acc = acc + D.mvp[12 + r] * 1.0f;
+            c[r] = acc;
+        }
+        for (int r = 0; r < 3; r++) {
+            float acc = D.model[r] * x;
+            acc = acc + D.model[4 + r] * y;
+            acc = acc + D.model[8 + r] * z;
+            acc = acc + D.model[12 + r] * 1.0f;
+            w[r] = acc;
+            float an = D.model[r] * nx;
+            an = an + D.model[4 + r] * ny;
+            an = an + D.model[8 + r] * nz;
+            n[r] = an;
+        }
+        xv[3 * (size_t)i] = make_float4(c[0], c[1], c[2], c[3]);
+        xv[3 * (size_t)i + 1] = make_float4(w[0], w[1], w[2], n[0]);
+        xv[3 * (size_t)i + 2] = make_float4(n[1], n[2], S.tu[s], S.tv[s]);
+        if (xt) {
+            if ((D.flags & SR_DRAW_TANGENTS) && S.tangent) {
+                float4 t = reinterpret_cast<const float4 *>(S.tangent)[s];
+                float tw[3];
+                for (int r = 0; r < 3; r++) {
+                    float acc = D.model[r] * t.x;
+                    acc = acc + D.model[4 + r] * t.y;
+                    acc = acc + D.model[8 + r] * t.z;
+                    tw[r] = acc;
+                }
+                xt[i] = make_float4(tw[0], tw[1], tw[2], t.w);
+            } else {
+                xt[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_clip_count: fan triangles per input triangle
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const DevDraw *draws, int ndraws,
+                                                    uint32_t total, const float4 *xv, uint32_t *counts) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+        const DevDraw &D = draws[d];
+        if (!(D.flags & SR_DRAW_CLIP)) {
+            counts[t] = 1;
+            continue;
+        }
+        const DevScene &S = *views[D.view].scene;
+        uint32_t lt = t - D.in_tri_base;
+        float c[3][4];
+        for (int k = 0; k < 3; k++) {
+            uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
+            float4 p = xv[3 * (size_t)(D.xv_base + vi)];
+            c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
+        }
+        int cls = clip_classify(c);
+        counts[t] = cls == -1 ? 1u : (cls == -2 ? 0u : (uint32_t)clip_count_general(c, cls));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan (4096 elements per block)
+// ---------------------------------------------------------------------------
+#define SCAN_ITEMS 16
+#define SCAN_BLOCK 256
+#define SCAN_TILE (SCAN_ITEMS * SCAN_BLOCK)
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    int lane = threadIdx.x & 63;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Exclusive block scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &excl, uint32_t *sh) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) sh[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; w++) {
+            uint32_t t = sh[w];
+            sh[w] = acc;
+            acc += t;
+        }
+        sh[16] = acc;
+    }
+    __syncthreads();
+    excl = sh[wave] + inc - v;
+    uint32_t total = sh[16];
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_partials(const uint32_t *in, uint32_t n, uint32_t *partials) {
+    __shared__ uint32_t sh[17];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) s += in[base + k];
+    uint32_t ex;
+    uint32_t total = block_excl_scan(s, ex, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t nparts, uint32_t *total_out) {
+    __shared__ uint32_t sh[17];
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < nparts; c += 1024) {
+        uint32_t i = c + threadIdx.x;
+        uint32_t v = i < nparts ? partials[i] : 0u;
+        uint32_t ex;
+        uint32_t tot = block_excl_scan(v, ex, sh);
+        if (i < nparts) partials[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, uint32_t n, const uint32_t *partials,
+                                                           uint32_t *out, uint32_t *out2) {
+    __shared__ uint32_t sh[17];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t ex;
+    block_excl_scan(s, ex, sh);
+    uint32_t acc = partials[blockIdx.x] + ex;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) {
+            out[base + k] = acc;
+            if (out2) out2[base + k] = acc;
+        }
+        acc += v[k];
+    }
+}
+
+static hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
+                              uint32_t *total, hipStream_t st) {
+    uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Binning rectangle: GPU tiles touched by the quad-aligned pixel bbox
+// ---------------------------------------------------------------------------
+struct Rect { int gx0, gy0, gx1, gy1; };
+
+__device__ __forceinline__ Rect bin_rect(const DevView &V, int mnx, int mny, int mxx, int mxy, int T) {
+    // quads cover [min & ~1, round_up_even(max)) in each axis
+    int qx0 = mnx & ~1, qy0 = mny & ~1;
+    int qx1 = (mxx + 1) & ~1, qy1 = (mxy + 1) & ~1;
+    int gw = V.tiles_x * T, gh = V.tiles_y * T;
+    qx1 = qx1 < gw ? qx1 : gw;
+    qy1 = qy1 < gh ? qy1 : gh;
+    Rect r;
+    r.gx0 = qx0 / T;
+    r.gy0 = qy0 / T;
+    r.gx1 = (qx1 + T - 1) / T;
+    r.gy1 = (qy1 + T - 1) / T;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// k_setup: clip with attributes, snap, cull, bbox, setup records, bin counts
+// ---------------------------------------------------------------------------
+struct SetupOut {
+    uint4 *geo;
+    float4 *raster;
+    float4 *shade;
+    float4 *tan;
+    uint32_t *bin_count;
+    uint32_t cap;
+    DevStatus *status;
+    int T;
+};
+
+__device__ void emit_triangle(const DevView &V, int view, const CV *v0, const CV *v1, const CV *v2, uint32_t id,
+                              int material, uint32_t matflags, const SetupOut &O) {
+    if (id >= O.cap) {
+        atomicOr(&O.status->overflow_records, 1u);
+        return;
+    }
+    const CV *tv[3] = {v0, v1, v2};
+    int32_t px[3], py[3];
+    for (int k = 0; k < 3; k++) {
+        float nxv = tv[k]->cx / tv[k]->cw, nyv = tv[k]->cy / tv[k]->cw;
+        float sx = ((nxv + 1.0f) * V.Wf) / 2.0f;
+        float sy = ((1.0f - nyv) * V.Hf) / 2.0f;
+        px[k] = sat_i32(roundf(sx * 16.0f));
+        py[k] = sat_i32(roundf(sy * 16.0f));
+    }
+    int32_t area = wsub(wmul(wsub(px[1], px[0]), wsub(py[2], py[0])), wmul(wsub(px[2], px[0]), wsub(py[1], py[0])));
+    if (area > 0) {
+        O.geo[id] = make_uint4(0u, 0u, SR_GEO_EMPTY, (uint32_t)view);
+        return;
+    }
+    int32_t mnx = max(min(min(px[0], px[1]), px[2]), 0);
+    int32_t mny = max(min(min(py[0], py[1]), py[2]), 0);
+    int32_t mxx = min(max(max(px[0], px[1]), px[2]), V.W_sub);
+    int32_t mxy = min(max(max(py[0], py[1]), py[2]), V.H_sub);
+    int32_t pmnx = mnx >> 4, pmny = mny >> 4;
+    int32_t pmxx = wadd(mxx, 16) >> 4, pmxy = wadd(mxy, 16) >> 4;
+    if (!(pmnx < pmxx && pmny < pmxy)) {
+        O.geo[id] = make_uint4(0u, 0u, SR_GEO_EMPTY, (uint32_t)view);
+        return;
+    }
+    float ooa = 1.0f / (float)wabs(area);
+    float oow[3], zow[3], uw[3], vw[3];
+    f3 pw[3];
+    for (int k = 0; k < 3; k++) oow[k] = 1.0f / tv[k]->cw;
+    for (int k = 0; k < 3; k++) zow[k] = tv[k]->cz * oow[k];
+    for (int k = 0; k < 3; k++) {
+        uw[k] = tv[k]->u * oow[k];
+        vw[k] = tv[k]->v * oow[k];
+        pw[k] = mk3(tv[k]->wx * oow[k], tv[k]->wy * oow[k], tv[k]->wz * oow[k]);
+    }
+    float dx1 = (float)wsub(px[1], px[0]), dx2 = (float)wsub(px[2], px[0]);
+    float dy1 = (float)wsub(py[1], py[0]), dy2 = (float)wsub(py[2], py[0]);
+    float du1 = uw[1] - uw[0], du2 = uw[2] - uw[0];
+    float dv1 = vw[1] - vw[0], dv2 = vw[2] - vw[0];
+    float du_dx = du1 * dy2 - du2 * dy1;
+    float du_dy = du2 * dx1 - du1 * dx2;
+    float dv_dx = dv1 * dy2 - dv2 * dy1;
+    float dv_dy = dv2 * dx1 - dv1 * dx2;
+    float scale = ooa * 16.0f;
+
+    uint32_t flags = 0;
+    if (matflags & SR_MAT_ALPHA_TESTED) flags |= SR_GEO_ALPHA;
+    if (matflags & SR_MAT_NORMAL_MAP) flags |= SR_GEO_NORMAL_MAP;
+    O.geo[id] = make_uint4((uint32_t)pmnx | ((uint32_t)pmny << 16), (uint32_t)pmxx | ((uint32_t)pmxy << 16), flags,
+                           (uint32_t)view);
+    float4 *R = O.raster + 4 * (size_t)id;
+    R[0] = make_float4(__int_as_float(px[0]), __int_as_float(py[0]), __int_as_float(px[1]), __int_as_float(py[1]));
+    R[1] = make_float4(__int_as_float(px[2]), __int_as_float(py[2]), ooa, oow[0]);
+    R[2] = make_float4(oow[1] - oow[0], oow[2] - oow[0], zow[0], zow[1] - zow[0]);
+    R[3] = make_float4(zow[2] - zow[0], __int_as_float(material), 0.0f, 0.0f);
+    f3 n0 = mk3(v0->nx, v0->ny, v0->nz), n1 = mk3(v1->nx, v1->ny, v1->nz), n2 = mk3(v2->nx, v2->ny, v2->nz);
+    f3 nda = sub3(n1, n0), ndb = sub3(n2, n0);
+    f3 pda = sub3(pw[1], pw[0]), pdb = sub3(pw[2], pw[0]);
+    float4 *Sh = O.shade + 8 * (size_t)id;
+    Sh[0] = make_float4(n0.x, n0.y, n0.z, nda.x);
+    Sh[1] = make_float4(nda.y, nda.z, ndb.x, ndb.y);
+    Sh[2] = make_float4(ndb.z, pw[0].x, pw[0].y, pw[0].z);
+    Sh[3] = make_float4(pda.x, pda.y, pda.z, pdb.x);
+    Sh[4] = make_float4(pdb.y, pdb.z, uw[0], uw[1] - uw[0]);
+    Sh[5] = make_float4(uw[2] - uw[0], vw[0], vw[1] - vw[0], vw[2] - vw[0]);
+    Sh[6] = make_float4(du_dx * scale, du_dy * scale, dv_dx * scale, dv_dy * scale);
+    if ((matflags & SR_MAT_NORMAL_MAP) && O.tan) {
+        f3 t0 = mk3(v0->tx, v0->ty, v0->tz), t1 = mk3(v1->tx, v1->ty, v1->tz), t2 = mk3(v2->tx, v2->ty, v2->tz);
+        f3 tda = sub3(t1, t0), tdb = sub3(t2, t0);
+        float4 *Tn = O.tan + 3 * (size_t)id;
+        Tn[0] = make_float4(t0.x, t0.y, t0.z, tda.x);
+        Tn[1] = make_float4(tda.y, tda.z, tdb.x, tdb.y);
+        Tn[2] = make_float4(tdb.z, v0->tw, v1->tw - v0->tw, v2->tw - v0->tw);
+    }
+    Rect r = bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
+    for (int gy = r.gy0; gy < r.gy1; gy++)
+        for (int gx = r.gx0; gx < r.gx1; gx++)
+            atomicAdd(&O.bin_count[V.item_base + (uint32_t)(gy * V.tiles_x + gx)], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
+                                               const float4 *xv, const float4 *xt, const uint32_t *tri_base,
+                                               SetupOut O) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+        const DevDraw &D = draws[d];
+        const DevView &V = views[D.view];
+        const DevScene &S = *V.scene;
+        uint32_t lt = t - D.in_tri_base;
+        CV v[3];
+        for (int k = 0; k < 3; k++) {
+            uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
+            load_cv(xv, (D.flags & SR_DRAW_TANGENTS) ? xt : nullptr, D.xv_base + vi, v[k]);
+        }
+        uint32_t matflags = S.materials[D.material].flags;
+        uint32_t id = tri_base[t];
+        if (!(D.flags & SR_DRAW_CLIP)) {
+            emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
+            continue;
+        }
+        float c[3][4];
+        for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
+        int cls = clip_classify(c);
+        if (cls == -2) continue;
+        if (cls == -1) {
+            emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
+            continue;
+        }
+        CV poly[16], np[16];
+        poly[0] = v[0]; poly[1] = v[1]; poly[2] = v[2];
+        int len = clip_general(poly, np, cls);
+        for (int i = 1; i + 1 < len; i++)
+            emit_triangle(V, D.view, &poly[0], &poly[i], &poly[i + 1], id + (uint32_t)(i - 1), D.material, matflags, O);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scatter: (triangle, tile) pairs into the per-tile lists
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_scatter(const DevView *views, const uint4 *geo, const DevStatus *status,
+                                                 uint32_t rec_cap, uint32_t *cursor, uint32_t *list, uint32_t pair_cap,
+                                                 DevStatus *status_w, int T) {
+    uint32_t total = status->total_records;
+    if (total > rec_cap) total = rec_cap;
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+        uint4 g = geo[id];
+        if (g.z & SR_GEO_EMPTY) continue;
+        const DevView &V = views[g.w];
+        int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
+        Rect r = bin_rect(V, mnx, mny, mxx, mxy, T);
+        for (int gy = r.gy0; gy < r.gy1; gy++)
+            for (int gx = r.gx0; gx < r.gx1; gx++) {
+                uint32_t pos = atomicAdd(&cursor[V.item_base + (uint32_t)(gy * V.tiles_x + gx)], 1u);
+                if (pos < pair_cap) list[pos] = id;
+                else atomicOr(&status_w->overflow_pairs, 1u);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-(triangle, tile) raster setup (src/tilerasterizer.rs:114-218)
+// ---------------------------------------------------------------------------
+struct TriSetup {
+    float A[3], B[3], C[3], SX[3], SY[3];   // edges: 0 = v1->v2 (w0), 1 = v2->v0 (w1), 2 = v0->v1 (w2)
+    float ooa, oa, oda, odb, za, zda, zdb;
+    int cmnx, cmny, cmxx, cmxy, ox0, oy0;
+    bool coarse;
+};
+
+__device__ __forceinline__ int32_t top_left_bias(int32_t a, int32_t b) { return (a < 0 || (a == 0 && b > 0)) ? 0 : -1; }
+
+__device__ __forceinline__ void tri_setup(const uint4 g, const float4 *R, int tx0, int ty0, TriSetup &S) {
+    float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3];
+    int32_t p0x = __float_as_int(r0.x), p0y = __float_as_int(r0.y), p1x = __float_as_int(r0.z), p1y = __float_as_int(r0.w);
+    int32_t p2x = __float_as_int(r1.x), p2y = __float_as_int(r1.y);
+    S.ooa = r1.z; S.oa = r1.w; S.oda = r2.x; S.odb = r2.y; S.za = r2.z; S.zda = r2.w; S.zdb = r3.x;
+    int32_t a01 = wsub(p1y, p0y), b01 = wsub(p0x, p1x);
+    int32_t c01 = wadd(wsub(wmul(p1x, p0y), wmul(p0x, p1y)), top_left_bias(a01, b01));
+    int32_t a12 = wsub(p2y, p1y), b12 = wsub(p1x, p2x);
+    int32_t c12 = wadd(wsub(wmul(p2x, p1y), wmul(p1x, p2y)), top_left_bias(a12, b12));
+    int32_t a20 = wsub(p0y, p2y), b20 = wsub(p2x, p0x);
+    int32_t c20 = wadd(wsub(wmul(p0x, p2y), wmul(p2x, p0y)), top_left_bias(a20, b20));
+    S.A[0] = (float)a12; S.B[0] = (float)b12; S.C[0] = (float)c12;
+    S.A[1] = (float)a20; S.B[1] = (float)b20; S.C[1] = (float)c20;
+    S.A[2] = (float)a01; S.B[2] = (float)b01; S.C[2] = (float)c01;
+    S.SX[0] = (float)wmul(a12, 32); S.SY[0] = (float)wmul(b12, 32);
+    S.SX[1] = (float)wmul(a20, 32); S.SY[1] = (float)wmul(b20, 32);
+    S.SX[2] = (float)wmul(a01, 32); S.SY[2] = (float)wmul(b01, 32);
+    int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
+    int rx = tx0 & ~(SR_REF_TILE - 1), ry = ty0 & ~(SR_REF_TILE - 1);
+    S.cmnx = max(mnx, rx); S.cmny = max(mny, ry);
+    S.cmxx = min(mxx, rx + SR_REF_TILE); S.cmxy = min(mxy, ry + SR_REF_TILE);
+    S.coarse = (S.cmxx - S.cmnx > 16) || (S.cmxy - S.cmny > 16);
+    S.ox0 = S.cmnx & ~1;
+    S.oy0 = S.cmny & ~1;
+}
+
+// Origin of the fine-raster walk that covers quad (qpx, qpy) and whether the
+// coarse 16px block containing it survives the corner test
+// (src/tilerasterizer.rs:220-291).  Returns false if the quad is not walked.
+__device__ __forceinline__ bool quad_origin(const TriSetup &S, int qpx, int qpy, int &bx, int &by) {
+    if (!(qpx >= S.ox0 && qpx < S.cmxx && qpy >= S.oy0 && qpy < S.cmxy)) return false;
+    bx = S.ox0;
+    by = S.oy0;
+    if (S.coarse) {
+        bx = S.ox0 + ((qpx - S.ox0) & ~15);
+        by = S.oy0 + ((qpy - S.oy0) & ~15);
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool block_rejected(const TriSetup &S, int bx, int by) {
+    float cx0 = (float)(bx * 16 + 8), cx1 = (float)(bx * 16 + 256 - 8);
+    float cy0 = (float)(by * 16 + 8), cy1 = (float)(by * 16 + 256 - 8);
+    bool out = false;
+    for (int e = 0; e < 3; e++) {
+        float w00 = (S.A[e] * cx0 + S.B[e] * cy0) + S.C[e];
+        float w10 = (S.A[e] * cx1 + S.B[e] * cy0) + S.C[e];
+        float w01 = (S.A[e] * cx0 + S.B[e] * cy1) + S.C[e];
+        float w11 = (S.A[e] * cx1 + S.B[e] * cy1) + S.C[e];
+        float m = maxps(maxps(w00, w10), maxps(w01, w11));
+        out = out || (m < 0.0f);
+    }
+    return out;
+}
+
+// Edge value of edge e at lane (sx, sy) of the quad i steps right and j steps
+// down from the walk origin (bx, by): origin evaluation, then j serial
+// +step_y, then i serial +step_x (src/tilerasterizer.rs:307-381).
+__device__ __forceinline__ float edge_chain(const TriSetup &S, int e, int bx, int by, int sx, int sy, int i, int j) {
+    float X = (float)(bx * 16 + 8 + 16 * sx);
+    float Y = (float)(by * 16 + 8 + 16 * sy);
+    float v = (S.A[e] * X + S.B[e] * Y) + S.C[e];
+    for (int k = 0; k < j; k++) v = v + S.SY[e];
+    for (int k = 0; k < i; k++) v = v + S.SX[e];
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Texture sampling (src/texture.rs:578-863)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float apply_wrap(float texel, float dim, int mode) {
+    float t2;
+    if (mode == SR_WRAP_CLAMP_TO_EDGE) {
+        t2 = texel;
+    } else if (mode == SR_WRAP_REPEAT) {
+        t2 = texel - floorf(texel / dim) * dim;
+    } else {
+        float two = dim * 2.0f;
+        float t = texel - floorf(texel / two) * two;
+        t2 = minps(t, two - t);
+    }
+    return minps(t2, dim - 1.0f);
+}
+
+__device__ __forceinline__ uint32_t compute_mip(const DevTexture &T, const float dd[4]) {
+    float d0 = dd[0] * T.whv[0], d1 = dd[1] * T.whv[1], d2 = dd[2] * T.whv[2], d3 = dd[3] * T.whv[3];
+    float dx2 = d0 * d0 + d2 * d2;
+    float dy2 = d1 * d1 + d3 * d3;
+    float fp = (dx2 + dy2) * 0.5f;
+    fp = fmaxf(fp, 1.0f);   // f32::max: NaN -> 1.0
+    uint32_t u = sat_u32(fp);
+    uint32_t mip = (31u - (uint32_t)__builtin_clz(u)) >> 1;
+    return mip < T.max_mip ? mip : T.max_mip;
+}
+
+__device__ __forceinline__ uint32_t texel_nearest(const DevScene &S, const DevTexRef &ref, float u, float v,
+                                                  const float dd[4]) {
+    const DevTexture &T = S.textures[ref.tex];
+    uint32_t mip = compute_mip(T, dd);
+    float wf = (float)T.mip_w[mip], hf = (float)T.mip_h[mip];
+    uint32_t x = sat_u32(apply_wrap(floorf(u * wf), wf, ref.wrap_s));
+    uint32_t y = sat_u32(apply_wrap(floorf(v * hf), hf, ref.wrap_t));
+    uint32_t idx = T.mip_off[mip] + y * T.mip_w[mip] + x;
+    return S.texels[T.base + idx];
+}
+
+__device__ __forceinline__ void cube_uv(f3 n, float &uf, float &vf, uint32_t &slice) {
+    float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+    bool mx = (ax >= ay) && (ax >= az);
+    bool my = (ay > ax) && (ay >= az);
+    float sx = n.x >= 0.0f ? 1.0f : -1.0f;
+    float sy = n.y >= 0.0f ? 1.0f : -1.0f;
+    float sz = n.z >= 0.0f ? 1.0f : -1.0f;
+    float u_x = -n.z * sx, v_x = -n.y, d_x = ax;
+    float u_y = n.x, v_y = n.z * sy, d_y = ay;
+    float u_z = n.x * sz, v_z = -n.y, d_z = az;
+    float u = mx ? u_x : (my ? u_y : u_z);
+    float v = mx ? v_x : (my ? v_y : v_z);
+    float den = mx ? d_x : (my ? d_y : d_z);
+    den = maxps(den, 1.0e-19f);
+    uf = 0.5f * (u / den + 1.0f);
+    vf = 0.5f * (v / den + 1.0f);
+    float ix = n.x >= 0.0f ? 0.0f : 1.0f;
+    float iy = n.y >= 0.0f ? 2.0f : 3.0f;
+    float iz = n.z >= 0.0f ? 4.0f : 5.0f;
+    slice = sat_u32(mx ? ix : (my ? iy : iz));
+}
+
+// sample_cubemap_rgb (nearest, full-resolution faces at every mip)
+__device__ __forceinline__ f3 cube_nearest(const uint32_t *data, uint32_t size, uint32_t mip, f3 n) {
+    float u, v;
+    uint32_t slice;
+    cube_uv(n, u, v, slice);
+    float wf = (float)size;
+    float x = roundf(u * (wf - 1.0f)), y = roundf(v * (wf - 1.0f));
+    uint32_t xi = sat_u32(clampv(x, 0.0f, wf - 1.0f));
+    uint32_t yi = sat_u32(clampv(y, 0.0f, wf - 1.0f));
+    uint32_t off = mip * 6u * size * size + slice * (size * size);
+    return unpack_rgb(data[off + yi * size + xi]);
+}
+
+// ---------------------------------------------------------------------------
+// Voxel grid lookup (src/voxelgrid.rs:264-368)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float tri_const(float k, float fx, float fy, float fz) {
+    float a = k * (1.0f - fx) + k * fx;
+    float b = a * (1.0f - fy) + a * fy;
+    return b * (1.0f - fz) + b * fz;
+}
+
+__device__ void voxel_lookup(const DevScene &S, f3 p, f3 rgb[4], float &w0, float &w1) {
+    float vx = (p.x - S.vox_min[0]) / S.vox_size[0];
+    float vy = (p.y - S.vox_min[1]) / S.vox_size[1];
+    float vz = (p.z - S.vox_min[2]) / S.vox_size[2];
+    float x0f = floorf(vx), y0f = floorf(vy), z0f = floorf(vz);
+    uint32_t W = (uint32_t)S.vox_dims[0], H = (uint32_t)S.vox_dims[1], D = (uint32_t)S.vox_dims[2];
+    uint32_t x0 = sat_index(x0f, W - 1), y0 = sat_index(y0f, H - 1), z0 = sat_index(z0f, D - 1);
+    uint32_t x1 = min(x0 + 1, W - 1), y1 = min(y0 + 1, H - 1), z1 = min(z0 + 1, D - 1);
+    float fx = vx - x0f, fy = vy - y0f, fz = vz - z0f;
+    float ox = 1.0f - fx, oy = 1.0f - fy, oz = 1.0f - fz;
+    size_t sy = W, sz = (size_t)W * H;
+    auto idx = [&](uint32_t x, uint32_t y, uint32_t z) { return (size_t)z * sz + (size_t)y * sy + x; };
+    if (S.vox_sh4) {
+        const float4 *g = reinterpret_cast<const float4 *>(S.vox_sh4);
+        for (int c = 0; c < 4; c++) {
+            float4 v000 = g[4 * idx(x0, y0, z0) + c], v001 = g[4 * idx(x0, y0, z1) + c];
+            float4 v010 = g[4 * idx(x0, y1, z0) + c], v011 = g[4 * idx(x0, y1, z1) + c];
+            float4 v100 = g[4 * idx(x1, y0, z0) + c], v101 = g[4 * idx(x1, y0, z1) + c];
+            float4 v110 = g[4 * idx(x1, y1, z0) + c], v111 = g[4 * idx(x1, y1, z1) + c];
+            float o[4];
+            const float *a000 = &v000.x, *a001 = &v001.x, *a010 = &v010.x, *a011 = &v011.x;
+            const float *a100 = &v100.x, *a101 = &v101.x, *a110 = &v110.x, *a111 = &v111.x;
+            for (int ch = 0; ch < 4; ch++) {
+                float v00 = a000[ch] * ox + a100[ch] * fx;
+                float v01 = a001[ch] * ox + a101[ch] * fx;
+                float v10 = a010[ch] * ox + a110[ch] * fx;
+                float v11 = a011[ch] * ox + a111[ch] * fx;
+                float v0 = v00 * oy + v10 * fy;
+                float v1 = v01 * oy + v11 * fy;
+                o[ch] = v0 * oz + v1 * fz;
+            }
+            rgb[c] = mk3(o[0], o[1], o[2]);
+            if (c == 0) w0 = o[3];
+            if (c == 1) w1 = o[3];
+        }
+        return;
+    }
+    for (int c = 0; c < 4; c++)
+        rgb[c] = mk3(tri_const(S.sh_const[c][0], fx, fy, fz), tri_const(S.sh_const[c][1], fx, fy, fz),
+                     tri_const(S.sh_const[c][2], fx, fy, fz));
+    const float *L = S.vox_light;
+    float v000 = L[idx(x0, y0, z0)], v001 = L[idx(x0, y0, z1)], v010 = L[idx(x0, y1, z0)], v011 = L[idx(x0, y1, z1)];
+    float v100 = L[idx(x1, y0, z0)], v101 = L[idx(x1, y0, z1)], v110 = L[idx(x1, y1, z0)], v111 = L[idx(x1, y1, z1)];
+    float v00 = v000 * ox + v100 * fx;
+    float v01 = v001 * ox + v101 * fx;
+    float v10 = v010 * ox + v110 * fx;
+    float v11 = v011 * ox + v111 * fx;
+    float v0 = v00 * oy + v10 * fy;
+    float v1 = v01 * oy + v11 * fy;
+    w0 = v0 * oz + v1 * fz;
+    w1 = tri_const(S.sh_const[1][3], fx, fy, fz);
+}
+
+// ---------------------------------------------------------------------------
+// PBR shading of one pixel (pbr_shader, src/shader.rs:110-309, opaque path)
+// b1[4], b2[4]: the quad's four barycentric pairs; lane: this pixel's lane.
+// ---------------------------------------------------------------------------
+__device__ f3 shade_pixel(const DevScene &S, const DevView &V, const float4 *R, const float4 *Sh, const float4 *Tn,
+                          const float b1[4], const float b2[4], int lane) {
+    float4 r1 = R[1], r2 = R[2], r3 = R[3];
+    float oa = r1.w, oda = r2.x, odb = r2.y;
+    int material = __float_as_int(r3.y);
+    const DevMaterial &M = S.materials[material];
+    float4 s0 = Sh[0], s1 = Sh[1], s2 = Sh[2], s3 = Sh[3], s4 = Sh[4], s5 = Sh[5], s6 = Sh[6];
+    float wv[4];
+    for (int j = 0; j < 4; j++) wv[j] = 1.0f / ((oa + b1[j] * oda) + b2[j] * odb);
+    float B1 = b1[lane], B2 = b2[lane];
+    float w = wv[lane];
+    f3 na = mk3(s0.x, s0.y, s0.z), nda = mk3(s0.w, s1.x, s1.y), ndb = mk3(s1.z, s1.w, s2.x);
+    f3 pa = mk3(s2.y, s2.z, s2.w), pda = mk3(s3.x, s3.y, s3.z), pdb = mk3(s3.w, s4.x, s4.y);
+    f3 in_n = mk3((na.x + B1 * nda.x) + B2 * ndb.x, (na.y + B1 * nda.y) + B2 * ndb.y, (na.z + B1 * nda.z) + B2 * ndb.z);
+    in_n = normalize3(in_n);
+    f3 pos = mk3(((pa.x + B1 * pda.x) + B2 * pdb.x) * w, ((pa.y + B1 * pda.y) + B2 * pdb.y) * w,
+                 ((pa.z + B1 * pda.z) + B2 * pdb.z) * w);
+    float uvx = ((s4.z + B1 * s4.w) + B2 * s5.x) * w;
+    float uvy = ((s5.y + B1 * s5.z) + B2 * s5.w) * w;
+    float dd[4] = {s6.x * wv[0], s6.y * wv[1], s6.z * wv[2], s6.w * wv[3]};
+    f3 nw = normalize3(in_n);
+    if ((M.flags & SR_MAT_NORMAL_MAP) && Tn) {
+        float4 t0 = Tn[0], t1 = Tn[1], t2 = Tn[2];
+        f3 ta = mk3(t0.x, t0.y, t0.z), tda = mk3(t0.w, t1.x, t1.y), tdb = mk3(t1.z, t1.w, t2.x);
+        f3 in_t = mk3((ta.x + B1 * tda.x) + B2 * tdb.x, (ta.y + B1 * tda.y) + B2 * tdb.y, (ta.z + B1 * tda.z) + B2 * tdb.z);
+        in_t = normalize3(in_t);
+        float tsign = (t2.y + B1 * t2.z) + B2 * t2.w;
+        f3 tw = normalize3(sub3(in_t, muls(in_n, dot3(in_n, in_t))));
+        float hand = tsign >= 0.0f ? 1.0f : -1.0f;
+        f3 bw = muls(cross3(in_n, tw), hand);
+        f3 tsn = unpack_rgb(texel_nearest(S, M.normal_tex, uvx, uvy, dd));
+        tsn = mk3(tsn.x * 2.0f - 1.0f, tsn.y * 2.0f - 1.0f, tsn.z * 2.0f - 1.0f);
+        nw = add3(add3(muls(tw, tsn.x), muls(bw, tsn.y)), muls(in_n, tsn.z));
+        nw = normalize3(nw);
+    }
+    f3 L = mk3(S.light_dir[0], S.light_dir[1], S.light_dir[2]);
+    f3 LC = mk3(S.light_color[0], S.light_color[1], S.light_color[2]);
+    f3 vdir = sub3(mk3(V.cam_pos[0], V.cam_pos[1], V.cam_pos[2]), pos);
+    f3 vn = normalize3(vdir);
+    float ndl = maxps(dot3(nw, L), 0.0f);
+    f3 hv = normalize3(add3(L, vn));
+    float ndh = maxps(dot3(nw, hv), 0.0f);
+    float ndv = maxps(dot3(nw, vn), 1.0e-4f);
+    float vdh = maxps(dot3(vn, hv), 0.0f);
+    f3 gi[4];
+    float gw0, gw1;
+    voxel_lookup(S, pos, gi, gw0, gw1);
+    float vli = clampv(gw0, 0.0f, 1.0f);
+    float skyv = clampv(gw1, 0.0f, 1.0f);
+    f3 base = mk3(M.base[0], M.base[1], M.base[2]);
+    if (M.base_tex.tex >= 0) base = mul3(base, srgb_fast(unpack_rgb(texel_nearest(S, M.base_tex, uvx, uvy, dd))));
+    float rough = M.roughness, metal = M.metallic;
+    if (M.mr_tex.tex >= 0) {
+        f3 mr = unpack_rgb(texel_nearest(S, M.mr_tex, uvx, uvy, dd));
+        rough = rough * mr.y;
+        metal = metal * mr.z;
+    }
+    rough = clampv(rough, 0.045f, 1.0f);
+    metal = clampv(metal, 0.0f, 1.0f);
+    float ao = 1.0f;
+    if (M.occlusion_tex.tex >= 0) {
+        ao = unpack_rgb(texel_nearest(S, M.occlusion_tex, uvx, uvy, dd)).x;
+        ao = 1.0f + (ao - 1.0f) * M.occlusion_strength;
+    }
+    f3 f0 = mk3(0.04f + (base.x - 0.04f) * metal, 0.04f + (base.y - 0.04f) * metal, 0.04f + (base.z - 0.04f) * metal);
+    float omvh = 1.0f - vdh;
+    float omvh2 = omvh * omvh;
+    float omvh4 = omvh2 * omvh2;
+    float omvh5 = omvh4 * omvh;
+    f3 bf = mk3(f0.x + (1.0f - f0.x) * omvh5, f0.y + (1.0f - f0.y) * omvh5, f0.z + (1.0f - f0.z) * omvh5);
+    const float PI = 3.14159265358979323846f;
+    const float EPS = 1e-6f;
+    float alpha = rough * rough;
+    float alpha2 = alpha * alpha;
+    float ndh2 = ndh * ndh;
+    float dden = ndh2 * (alpha2 - 1.0f) + 1.0f;
+    float brdf_d = alpha2 / (PI * (dden * dden) + EPS);
+    float k = rough + 1.0f;
+    k = (k * k) * 0.125f;
+    float gv = ndv / ((ndv * (1.0f - k) + k) + EPS);
+    float gl = ndl / ((ndl * (1.0f - k) + k) + EPS);
+    float brdf_g = gv * gl;
+    float sdg = (brdf_d * brdf_g) / ((4.0f * ndl) * ndv + EPS);
+    float om = 1.0f - metal;
+    f3 kdd = mk3((1.0f - bf.x) * om, (1.0f - bf.y) * om, (1.0f - bf.z) * om);
+    const float INV_PI = 1.0f / 3.14159265358979323846f;
+    f3 lam = muls(base, INV_PI);
+    f3 cdd = muls(muls(mul3(mul3(LC, kdd), lam), ndl), vli);
+    f3 cds = muls(muls(muls(mul3(LC, bf), sdg), ndl), vli);
+    f3 kdi = mk3((1.0f - f0.x) * om, (1.0f - f0.y) * om, (1.0f - f0.z) * om);
+    f3 irr;
+    irr.x = ((gi[0].x + gi[1].x * nw.y) + gi[2].x * nw.z) + gi[3].x * nw.x;
+    irr.y = ((gi[0].y + gi[1].y * nw.y) + gi[2].y * nw.z) + gi[3].y * nw.x;
+    irr.z = ((gi[0].z + gi[1].z * nw.y) + gi[2].z * nw.z) + gi[3].z * nw.x;
+    irr = max3s(irr, 0.0f);
+    f3 cid = muls(mul3(mul3(irr, base), kdi), INV_PI);
+    f3 iv = muls(vn, -1.0f);
+    f3 refl = sub3(iv, muls(muls(nw, dot3(iv, nw)), 2.0f));
+    float smip = rough * S.spec_max_mip_f;
+    // sample_cubemap_trilinear_rgb
+    float mip = clampv(smip, 0.0f, S.spec_max_mip_f);
+    float m0 = floorf(mip);
+    float m1 = minps(m0 + 1.0f, S.spec_max_mip_f);
+    float tt = mip - m0;
+    uint32_t mi0 = min(sat_u32(m0), S.spec_mips - 1), mi1 = min(sat_u32(m1), S.spec_mips - 1);
+    f3 c0 = cube_nearest(S.specular, S.spec_size, mi0, refl);
+    f3 c1 = cube_nearest(S.specular, S.spec_size, mi1, refl);
+    f3 env = mk3(c0.x + (c1.x - c0.x) * tt, c0.y + (c1.y - c0.y) * tt, c0.z + (c1.z - c0.z) * tt);
+    // sample_bilinear_rgb on the BRDF LUT, mip 0, clamp-to-edge
+    float lu = clampv(ndv, 0.0f, 1.0f), lv = clampv(rough, 0.0f, 1.0f);
+    float lw = (float)S.lut_size;
+    float xf = lu * lw - 0.5f, yf = lv * lw - 0.5f;
+    float x0 = floorf(xf), y0 = floorf(yf);
+    float x1 = x0 + 1.0f, y1 = y0 + 1.0f;
+    float fx = xf - x0, fy = yf - y0;
+    float omfx = 1.0f - fx, omfy = 1.0f - fy;
+    uint32_t x0i = sat_u32(minps(x0, lw - 1.0f)), y0i = sat_u32(minps(y0, lw - 1.0f));
+    uint32_t x1i = sat_u32(minps(x1, lw - 1.0f)), y1i = sat_u32(minps(y1, lw - 1.0f));
+    f3 p00 = unpack_rgb(S.lut[y0i * S.lut_size + x0i]);
+    f3 p10 = unpack_rgb(S.lut[y0i * S.lut_size + x1i]);
+    f3 p01 = unpack_rgb(S.lut[y1i * S.lut_size + x0i]);
+    f3 p11 = unpack_rgb(S.lut[y1i * S.lut_size + x1i]);
+    float w00 = omfx * omfy, w10 = fx * omfy, w01 = omfx * fy, w11 = fx * fy;
+    float lutx = ((p00.x * w00 + p10.x * w10) + p01.x * w01) + p11.x * w11;
+    float luty = ((p00.y * w00 + p10.y * w10) + p01.y * w01) + p11.y * w11;
+    f3 bsf = adds(muls(f0, lutx), luty);
+    f3 cis = mul3(env, bsf);
+    float ao_spec = 1.0f + (ao - 1.0f) * 0.5f;
+    cis = muls(cis, ao_spec * skyv);
+    f3 color = add3(add3(add3(cdd, cds), cid), cis);
+    f3 em = mk3(1.0f, 1.0f, 1.0f);
+    if (M.emissive_tex.tex >= 0) em = srgb_fast(unpack_rgb(texel_nearest(S, M.emissive_tex, uvx, uvy, dd)));
+    color = add3(color, mul3(em, mk3(M.emissive[0], M.emissive[1], M.emissive[2])));
+    return color;
+}
+
+// compute_skybox (src/tilerasterizer.rs:478-508) for one lane of quad (qx, qy)
+__device__ f3 skybox_pixel(const DevScene &S, const DevView &V, int qx, int qy, int sx, int sy) {
+    float pxf = (float)qx + (sx ? 1.5f : 0.5f);
+    float pyf = (float)qy + (sy ? 1.5f : 0.5f);
+    float ndx = (pxf * V.one_over_w) * 2.0f - 1.0f;
+    float ndy = (1.0f - pyf * V.one_over_h) * 2.0f - 1.0f;
+    const float *m = V.sky;
+    f3 d;
+    d.x = ((ndx * m[0] + ndy * m[1]) + 1.0f * m[2]) + m[3];
+    d.y = ((ndx * m[4] + ndy * m[5]) + 1.0f * m[6]) + m[7];
+    d.z = ((ndx * m[8] + ndy * m[9]) + 1.0f * m[10]) + m[11];
+    f3 n = normalize3(d);
+    return srgb_fast(cube_nearest(S.cubemap, S.cube_size, 0, n));
+}
+
+// ---------------------------------------------------------------------------
+// k_raster_shade: one workgroup per (view, tile)
+// ---------------------------------------------------------------------------
+struct RasterArgs {
+    const DevView *views;
+    int32_t num_views;
+    uint32_t total_items;
+    const uint32_t *bin_start;
+    const uint32_t *bin_end;
+    const uint32_t *list;
+    uint32_t pair_cap;
+    const uint4 *geo;
+    const float4 *raster;
+    const float4 *shade;
+    const float4 *tan;
+    const uint32_t *tri_base;
+    uint32_t *out_ids;
+    float *out_depth;
+    float *out_color;
+    uint64_t color_plane;
+    uint32_t *out_rgba;
+    float *out_lum;
+    DevStatus *status;
+    int32_t persistent;
+};
+
+__device__ __forceinline__ uint32_t depth_key(float z) {
+    if (z == 0.0f) z = 0.0f;  // -0 == +0
+    uint32_t b = __float_as_uint(z);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ bool alpha_pass(const DevScene &S, const float4 *R, const float4 *Sh, float b1, float b2,
+                                           float w) {
+    int material = __float_as_int(R[3].y);
+    const DevMaterial &M = S.materials[material];
+    float4 s4 = Sh[4], s5 = Sh[5], s6 = Sh[6];
+    float u = ((s4.z + b1 * s4.w) + b2 * s5.x) * w;
+    float v = ((s5.y + b1 * s5.z) + b2 * s5.w) * w;
+    float dd[4] = {s6.x, s6.y, s6.z, s6.w};
+    float a = unpack_a(texel_nearest(S, M.base_tex, u, v, dd));
+    return a >= M.alpha_cutoff;
+}
+
+template <int T, int NT>
+__device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned long long *keys) {
+    constexpr int QW = T / 2;                 // quads per row
+    constexpr int NQ = QW * QW;               // quads per tile
+    constexpr int LPT = NQ < 64 ? NQ : 64;    // lanes per triangle
+    constexpr int TPW = 64 / LPT;             // triangles per wave step
+    constexpr int QPL = NQ / LPT;             // quads per lane
+    constexpr int NW = NT / 64;               // waves
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    int v = 0;
+    {
+        int lo = 0, hi = P.num_views - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (P.views[mid].item_base <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        v = lo;
+    }
+    const DevView &V = P.views[v];
+    const DevScene &S = *V.scene;
+    uint32_t tile = item - V.item_base;
+    int tx0 = (int)(tile % (uint32_t)V.tiles_x) * T, ty0 = (int)(tile / (uint32_t)V.tiles_x) * T;
+
+    for (int i = tid; i < T * T; i += NT) keys[i] = SR_EMPTY_KEY;
+    __syncthreads();
+
+    // ---- phase 1: coverage and depth ----
+    uint32_t start = P.bin_start[item];
+    uint32_t end = P.bin_end[item];
+    if (end > P.pair_cap) end = P.pair_cap;
+    const int slot = wave * TPW + lane / LPT;
+    const int ql = lane % LPT;
+    for (uint32_t e = start + (uint32_t)slot; e < end; e += (uint32_t)(NW * TPW)) {
+        uint32_t id = P.list[e];
+        if (TPW == 1) id = __builtin_amdgcn_readfirstlane(id);
+        uint4 g = P.geo[id];
+        const float4 *R = P.raster + 4 * (size_t)id;
+        TriSetup ts;
+        tri_setup(g, R, tx0, ty0, ts);
+        uint32_t inv_id = ~id;
+        for (int k = 0; k < QPL; k++) {
+            int q = ql + k * LPT;
+            int qx = q % QW, qy = q / QW;
+            int qpx = tx0 + 2 * qx, qpy = ty0 + 2 * qy;
+            int bx, by;
+            if (!quad_origin(ts, qpx, qpy, bx, by)) continue;
+            if (ts.coarse && block_rejected(ts, bx, by)) continue;
+            int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
+            for (int s = 0; s < 4; s++) {
+                int sx = s & 1, sy = s >> 1;
+                float w0 = edge_chain(ts, 0, bx, by, sx, sy, i, j);
+                float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
+                float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
+                if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)) continue;
+                float b1 = w1 * ts.ooa, b2 = w2 * ts.ooa;
+                float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
+                float z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
+                if (z != z) continue;
+                if (g.z & SR_GEO_ALPHA) {
+                    if (!alpha_pass(S, R, P.shade + 8 * (size_t)id, b1, b2, w)) continue;
+                }
+                unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)inv_id;
+                atomicMin(&keys[(2 * qy + sy) * T + 2 * qx + sx], key);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: resolve, visibility buffer, shading ----
+    uint32_t rec0 = P.tri_base[V.in_tri_base];
+    bool fused = (V.flags & 1u) != 0;
+    for (int pb = 0; pb < T * T; pb += NT) {
+        int p = pb + tid;
+        int q = p >> 2, s = p & 3;
+        int qx = q % QW, qy = q / QW;
+        int sx = s & 1, sy = s >> 1;
+        int px = tx0 + 2 * qx + sx, py = ty0 + 2 * qy + sy;
+        unsigned long long key = keys[(2 * qy + sy) * T + 2 * qx + sx];
+        bool has = key != SR_EMPTY_KEY;
+        uint32_t id = has ? ~(uint32_t)key : 0u;
+        float b1 = 0.0f, b2 = 0.0f, z = INFINITY;
+        if (has) {
+            uint4 g = P.geo[id];
+            TriSetup ts;
+            tri_setup(g, P.raster + 4 * (size_t)id, tx0, ty0, ts);
+            int qpx = tx0 + 2 * qx, qpy = ty0 + 2 * qy;
+            int bx = ts.ox0, by = ts.oy0;
+            quad_origin(ts, qpx, qpy, bx, by);
+            int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
+            float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
+            float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
+            b1 = w1 * ts.ooa;
+            b2 = w2 * ts.ooa;
+            float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
+            z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
+        }
+        float qb1[4], qb2[4];
+        int qbase = lane & ~3;
+        for (int k = 0; k < 4; k++) {
+            qb1[k] = __shfl(b1, qbase + k, 64);
+            qb2[k] = __shfl(b2, qbase + k, 64);
+        }
+        bool on_screen = px < V.W && py < V.H;
+        bool center = ((tx0 & 63) == 0) && ((ty0 & 63) == 32) && q == 0;
+        f3 color = mk3(0.0f, 0.0f, 0.0f);
+        if (on_screen || center) {
+            bool covered = has && z != INFINITY;
+            if (covered) {
+                color = shade_pixel(S, V, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id,
+                                    (P.tan ? P.tan + 3 * (size_t)id : nullptr), qb1, qb2, s);
+            } else {
+                color = skybox_pixel(S, V, tx0 + 2 * qx, ty0 + 2 * qy, sx, sy);
+            }
+        }
+        if (on_screen) {
+            size_t o = V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px;
+            P.out_ids[o] = has ? id - rec0 : 0xFFFFFFFFu;
+            P.out_depth[o] = z;
+            if (fused) {
+                float k = 0.2f;
+                float cr = color.x * V.exposure, cg = color.y * V.exposure, cb = color.z * V.exposure;
+                cr = (cr / (cr + k)) * (1.0f + k);
+                cg = (cg / (cg + k)) * (1.0f + k);
+                cb = (cb / (cb + k)) * (1.0f + k);
+                P.out_rgba[o] = (sat_u8(cr * 255.0f) << 24) | (sat_u8(cg * 255.0f) << 16) |
+                                (sat_u8(cb * 255.0f) << 8) | 0xFFu;
+            } else {
+                P.out_color[o] = color.x;
+                P.out_color[P.color_plane + o] = color.y;
+                P.out_color[2 * P.color_plane + o] = color.z;
+            }
+        }
+        // centre-quad luminance of the 64px tile (src/tilerasterizer.rs:103-106)
+        float lum = (color.x * 0.2126f + color.y * 0.7152f) + color.z * 0.0722f;
+        float l0 = __shfl(lum, qbase, 64), l1 = __shfl(lum, qbase + 1, 64);
+        float l2 = __shfl(lum, qbase + 2, 64), l3 = __shfl(lum, qbase + 3, 64);
+        if (center && s == 0) {
+            int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
+            P.out_lum[V.lum_offset + (uint32_t)ri] = (((l0 + l1) + l2) + l3) * 0.25f;
+        }
+    }
+}
+
+template <int T, int NT>
+__global__ void __launch_bounds__(NT) k_raster_shade(RasterArgs P) {
+    __shared__ unsigned long long keys[T * T];
+    if (!P.persistent) {
+        raster_shade_item<T, NT>(P, blockIdx.x, keys);
+        return;
+    }
+    __shared__ uint32_t next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(&P.status->work_counter, 1u);
+        __syncthreads();
+        uint32_t item = next;
+        __syncthreads();
+        if (item >= P.total_items) break;
+        raster_shade_item<T, NT>(P, item, keys);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_blit: exposure, tonemap, quantize, pack (src/renderer.rs:293-355)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_blit(const float *color, uint64_t plane, int W, int H, float exposure,
+                                              uint32_t *out) {
+    uint64_t n = (uint64_t)W * (uint64_t)(H & ~1);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float k = 0.2f;
+        float c[3];
+        for (int ch = 0; ch < 3; ch++) {
+            float x = color[ch * plane + i] * exposure;
+            c[ch] = (x / (x + k)) * (1.0f + k);
+        }
+        out[i] = (sat_u8(c[0] * 255.0f) << 24) | (sat_u8(c[1] * 255.0f) << 16) | (sat_u8(c[2] * 255.0f) << 8) | 0xFFu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
+    const uint32_t GRID_CAP = 256 * 16;
+    if (a.events[0]) hipEventRecord(a.events[0], st);
+    if (a.total_xv > 0)
+        hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
+                           a.num_draws, a.total_xv, a.xv, a.any_tangents ? a.xt : nullptr);
+    if (a.total_in > 0)
+        hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
+                           a.num_draws, a.total_in, a.xv, a.tri_counts);
+    // tri_counts[total_in] is a zero sentinel so tri_base[total_in] = total
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, a.tri_counts + a.total_in, 1u, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.bin_count,
+                       a.total_items, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(a.status),
+                       (uint32_t)(sizeof(DevStatus) / 4), 0u);
+    hipError_t e = launch_scan(a.tri_counts, a.total_in + 1, a.tri_base, nullptr, a.scan_partials,
+                               &a.status->total_records, st);
+    if (e != hipSuccess) return e;
+    SetupOut so;
+    so.geo = a.rec_geo; so.raster = a.rec_raster; so.shade = a.rec_shade; so.tan = a.rec_tan;
+    so.bin_count = a.bin_count; so.cap = a.rec_cap; so.status = a.status; so.T = a.tile_size;
+    if (a.total_in > 0)
+        hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
+                           a.num_draws, a.total_in, a.xv, a.any_tangents ? a.xt : nullptr, a.tri_base, so);
+    if (a.events[1]) hipEventRecord(a.events[1], st);
+    e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scatter, dim3(GRID_CAP), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
+                       a.bin_end, a.list, a.pair_cap, a.status, a.tile_size);
+    if (a.events[2]) hipEventRecord(a.events[2], st);
+    RasterArgs P;
+    P.views = a.views; P.num_views = a.num_views; P.total_items = a.total_items;
+    P.bin_start = a.bin_start; P.bin_end = a.bin_end; P.list = a.list; P.pair_cap = a.pair_cap;
+    P.geo = a.rec_geo; P.raster = a.rec_raster; P.shade = a.rec_shade; P.tan = a.any_tangents ? a.rec_tan : nullptr;
+    P.tri_base = a.tri_base; P.out_ids = a.out_ids; P.out_depth = a.out_depth; P.out_color = a.out_color;
+    P.color_plane = a.color_plane; P.out_rgba = a.out_rgba; P.out_lum = a.out_lum; P.status = a.status;
+    P.persistent = a.persistent_blocks > 0;
+    uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : a.total_items;
+    switch (a.tile_size) {
+        case 8: hipLaunchKernelGGL((k_raster_shade<8, 64>), dim3(grid), dim3(64), 0, st, P); break;
+        case 32: hipLaunchKernelGGL((k_raster_shade<32, 256>), dim3(grid), dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_raster_shade<16, 256>), dim3(grid), dim3(256), 0, st, P); break;
+    }
+    if (a.events[3]) hipEventRecord(a.events[3], st);
+    return hipGetLastError();
+}
+
+hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
+                       hipStream_t st) {
+    uint64_t n = (uint64_t)W * (uint64_t)(H & ~1);
+    hipLaunchKernelGGL(k_blit, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, st, color, plane, W, H, exposure, out);
+    return hipGetLastError();
+}
+
+}  // namespace sr

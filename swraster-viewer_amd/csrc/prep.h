@@ -1,0 +1,24 @@
+// prep.h -- host-side load-time input producers and camera math.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "swraster.h"
+
+namespace sr {
+
+void build_mip_chain(uint32_t width, uint32_t height, int type, const uint32_t *base, std::vector<uint32_t> &data,
+                     std::vector<uint32_t> &offsets, std::vector<uint32_t> &widths, std::vector<uint32_t> &heights);
+void compute_brdf_lut(uint32_t size, std::vector<uint32_t> &out, int threads);
+void compute_irradiance_sh4(const uint32_t *faces, uint32_t size, float sh[4][3]);
+void compute_specular_cubemap(const uint32_t *faces, uint32_t size, uint32_t sample_count, std::vector<uint32_t> &out,
+                              uint32_t &num_mips, int threads);
+void scene_bounds(const sr_scene_desc *d, float mn[3], float mx[3], float center[3], float *diag);
+int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<float> &light, float wmin[3],
+                     float wmax[3]);
+void camera_new(const float pos[3], const float look[3], float fov, float w, float h, float far_plane, sr_camera *c);
+void camera_update(sr_camera *c);
+void compute_tangents(const sr_primitive_desc &p, std::vector<float> &out);
+
+}  // namespace sr

@@ -1,0 +1,365 @@
+// renderer.cpp -- the drop-in per-frame renderer (Renderer, src/renderer.rs:145-355)
+// and the batched multi-view launch.
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "swraster.h"
+
+using namespace sr;
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) return fail(SR_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+static const float DEFAULT_EXPOSURE = 2.0f;
+
+struct sr_renderer {
+    int W = 0, H = 0;
+    int device = 0;
+    Pipeline pipe;
+    DevBuf ids, depth, color, rgba, lum;
+    int ref_tiles = 0;
+    float auto_exposure = DEFAULT_EXPOSURE, auto_exposure_target = DEFAULT_EXPOSURE, auto_exposure_ev = 1.0f;
+    sr_frame_stats stats;
+    bool has_frame = false;
+};
+
+struct sr_batch {
+    int device = 0;
+    Pipeline pipe;
+    DevBuf ids, depth, rgba, lum;
+    std::vector<uint64_t> fb_offset;
+    std::vector<int> width, height;
+    sr_batch_stats stats;
+    int persistent = 0;
+    bool use_graph = false;
+};
+
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    return ms;
+}
+
+extern "C" {
+
+// ---- renderer ---------------------------------------------------------------
+SR_API int sr_renderer_create(int32_t width, int32_t height, const sr_renderer_opts *opts, sr_renderer **out) {
+    if (!out || width <= 0 || height <= 0 || width > 32000 || height > 32000)
+        return fail(SR_ERR_INVALID_ARGUMENT, "renderer size must be in 1..32000");
+    int32_t ndev = 0;
+    sr_device_count(&ndev);
+    int dev = opts ? opts->device : 0;
+    if (dev < 0 || dev >= ndev) return fail(SR_ERR_INVALID_ARGUMENT, "no HIP device %d (have %d)", dev, ndev);
+    sr_renderer *r = new sr_renderer();
+    r->W = width;
+    r->H = height;
+    r->device = dev;
+    memset(&r->stats, 0, sizeof(r->stats));
+    r->auto_exposure_ev = std::log2(DEFAULT_EXPOSURE);
+    int rc = r->pipe.init(dev, opts ? opts->tile_size : 0);
+    size_t np = (size_t)width * height;
+    int rtx = (width + SR_REF_TILE - 1) / SR_REF_TILE, rty = (height + SR_REF_TILE - 1) / SR_REF_TILE;
+    r->ref_tiles = rtx * rty;
+    if (!rc) rc = r->ids.ensure(np * 4);
+    if (!rc) rc = r->depth.ensure(np * 4);
+    if (!rc) rc = r->color.ensure(np * 12);
+    if (!rc) rc = r->rgba.ensure(np * 4);
+    if (!rc) rc = r->lum.ensure((size_t)r->ref_tiles * 4);
+    if (rc) {
+        r->pipe.destroy();
+        delete r;
+        return rc;
+    }
+    // the first frame's tile luminance is 1.0 as in the reference (src/renderer.rs:119)
+    std::vector<float> ones(r->ref_tiles, 1.0f);
+    hipMemcpy(r->lum.ptr, ones.data(), ones.size() * 4, hipMemcpyHostToDevice);
+    *out = r;
+    return SR_OK;
+}
+
+SR_API int sr_renderer_destroy(sr_renderer *r) {
+    if (!r) return SR_OK;
+    hipSetDevice(r->device);
+    r->pipe.destroy();
+    r->ids.release();
+    r->depth.release();
+    r->color.release();
+    r->rgba.release();
+    r->lum.release();
+    delete r;
+    return SR_OK;
+}
+
+SR_API int sr_renderer_render_scene(sr_renderer *r, const sr_scene *scene, const sr_camera *camera) {
+    if (!r || !scene || !camera) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->device != r->device) return fail(SR_ERR_INVALID_ARGUMENT, "scene and renderer are on different devices");
+    std::vector<DevDraw> draws;
+    uint32_t in_tri = 0, xv = 0;
+    int rc = build_draws(scene, *camera, 0, in_tri, xv, draws);
+    if (rc) return rc;
+    std::vector<DevView> views(1);
+    int T = r->pipe.tile_size;
+    fill_view(views[0], scene, *camera, r->W, r->H, T, 0, 0, 0, 0, r->auto_exposure, 0);
+    uint32_t items = (uint32_t)(views[0].tiles_x * views[0].tiles_y);
+    rc = r->pipe.run(views, draws, xv, in_tri, items, scene->any_tangents, (uint32_t *)r->ids.ptr,
+                     (float *)r->depth.ptr, (float *)r->color.ptr, (uint64_t)r->W * r->H, nullptr,
+                     (float *)r->lum.ptr, 0, true);
+    if (rc) return rc;
+    r->has_frame = true;
+    r->stats.input_triangles = in_tri;
+    r->stats.setup_triangles = r->pipe.last_records;
+    r->stats.binned_pairs = r->pipe.last_pairs;
+    r->stats.tiles = items;
+    r->stats.draws = (uint32_t)draws.size();
+    r->stats.ms_setup = ev_ms(r->pipe.ev[0], r->pipe.ev[1]);
+    r->stats.ms_bin = ev_ms(r->pipe.ev[1], r->pipe.ev[2]);
+    r->stats.ms_raster_shade = ev_ms(r->pipe.ev[2], r->pipe.ev[3]);
+    r->stats.ms_total = ev_ms(r->pipe.ev[0], r->pipe.ev[3]);
+    return SR_OK;
+}
+
+// Renderer::update_auto_exposure (src/renderer.rs:258-290)
+SR_API int sr_renderer_update_auto_exposure(sr_renderer *r, float dt) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    size_t n = (size_t)r->ref_tiles;
+    if (n == 0) return SR_OK;
+    float dtc = std::fmax(dt, 0.0f);
+    float tau = std::fmax(1.0f, 1e-4f);
+    float alpha = 1.0f - std::exp(-(dtc / tau));
+    if (alpha == 0.0f) {
+        // the adaptation step is zero: exposure is 2^ev whatever the target
+        r->auto_exposure = std::pow(2.0f, r->auto_exposure_ev);
+        return SR_OK;
+    }
+    hipSetDevice(r->device);
+    std::vector<float> s(n);
+    HIP_TRY(hipMemcpy(s.data(), r->lum.ptr, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) s[i] = std::log2(std::fmax(s[i], 1e-4f));
+    auto key = [](float f) {
+        int32_t b;
+        memcpy(&b, &f, 4);
+        return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
+    };
+    std::sort(s.begin(), s.end(), [&](float a, float b) { return key(a) < key(b); });
+    size_t trim = (size_t)std::floor((float)n * 0.10f);
+    trim = std::min(trim, (n - 1) / 2);
+    float sum = 0.0f;
+    for (size_t i = trim; i < n - trim; i++) sum += s[i];
+    float mean = sum / (float)(n - 2 * trim);
+    float y = std::fmin(std::fmax(0.45f, 0.0f), 1.0f);
+    float k = 0.2f;
+    float denom = std::fmax(1.0f + k - y, 1e-6f);
+    float meter = std::fmax((y * k) / denom, 1e-4f);
+    float target_ev = std::log2(meter) - mean;
+    float target = std::fmin(std::fmax(std::pow(2.0f, target_ev), 0.05f), 32.0f);
+    r->auto_exposure_target = target;
+    float tev = std::log2(r->auto_exposure_target);
+    r->auto_exposure_ev += (tev - r->auto_exposure_ev) * alpha;
+    r->auto_exposure = std::pow(2.0f, r->auto_exposure_ev);
+    return SR_OK;
+}
+
+SR_API float sr_renderer_exposure(const sr_renderer *r) { return r ? r->auto_exposure : 0.0f; }
+
+SR_API int sr_renderer_set_exposure(sr_renderer *r, float exposure) {
+    if (!r || !(exposure > 0.0f)) return fail(SR_ERR_INVALID_ARGUMENT, "exposure must be positive");
+    r->auto_exposure = exposure;
+    r->auto_exposure_ev = std::log2(exposure);
+    return SR_OK;
+}
+
+SR_API int sr_renderer_blit_to_device(sr_renderer *r, void *device_pixels) {
+    if (!r || !device_pixels) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure,
+                        (uint32_t *)device_pixels, r->pipe.stream));
+    HIP_TRY(hipStreamSynchronize(r->pipe.stream));
+    return SR_OK;
+}
+
+SR_API int sr_renderer_blit_to_buffer(sr_renderer *r, uint32_t *pixels, size_t width, size_t height) {
+    if (!r || !pixels) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if ((int)width != r->W || (int)height != r->H)
+        return fail(SR_ERR_INVALID_ARGUMENT, "buffer is %zux%zu, renderer is %dx%d", width, height, r->W, r->H);
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure,
+                        (uint32_t *)r->rgba.ptr, r->pipe.stream));
+    size_t rows = (size_t)(r->H & ~1);
+    HIP_TRY(hipMemcpyAsync(pixels, r->rgba.ptr, rows * (size_t)r->W * 4, hipMemcpyDeviceToHost, r->pipe.stream));
+    HIP_TRY(hipStreamSynchronize(r->pipe.stream));
+    return SR_OK;
+}
+
+SR_API int sr_renderer_read_vbuffer(sr_renderer *r, uint32_t *tri_id, float *depth) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    size_t n = (size_t)r->W * r->H;
+    if (tri_id) HIP_TRY(hipMemcpy(tri_id, r->ids.ptr, n * 4, hipMemcpyDeviceToHost));
+    if (depth) HIP_TRY(hipMemcpy(depth, r->depth.ptr, n * 4, hipMemcpyDeviceToHost));
+    return SR_OK;
+}
+
+SR_API int sr_renderer_read_color(sr_renderer *r, float *rgb) {
+    if (!r || !rgb) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    size_t n = (size_t)r->W * r->H;
+    std::vector<float> planes(n * 3);
+    HIP_TRY(hipMemcpy(planes.data(), r->color.ptr, n * 12, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        rgb[3 * i] = planes[i];
+        rgb[3 * i + 1] = planes[n + i];
+        rgb[3 * i + 2] = planes[2 * n + i];
+    }
+    return SR_OK;
+}
+
+SR_API int sr_renderer_read_tile_luminance(sr_renderer *r, float *lum, int32_t *count) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (count) *count = r->ref_tiles;
+    if (lum) {
+        hipSetDevice(r->device);
+        HIP_TRY(hipMemcpy(lum, r->lum.ptr, (size_t)r->ref_tiles * 4, hipMemcpyDeviceToHost));
+    }
+    return SR_OK;
+}
+
+SR_API int sr_renderer_get_stats(const sr_renderer *r, sr_frame_stats *stats) {
+    if (!r || !stats) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    *stats = r->stats;
+    return SR_OK;
+}
+
+SR_API int sr_renderer_device_buffers(sr_renderer *r, void **tri_id, void **depth, void **color_rgb) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (tri_id) *tri_id = r->ids.ptr;
+    if (depth) *depth = r->depth.ptr;
+    if (color_rgb) *color_rgb = r->color.ptr;
+    return SR_OK;
+}
+
+// ---- batch -------------------------------------------------------------------
+SR_API int sr_batch_create(const sr_batch_opts *opts, sr_batch **out) {
+    if (!out) return fail(SR_ERR_INVALID_ARGUMENT, "null output");
+    int32_t ndev = 0;
+    sr_device_count(&ndev);
+    int dev = opts ? opts->device : 0;
+    if (dev < 0 || dev >= ndev) return fail(SR_ERR_INVALID_ARGUMENT, "no HIP device %d (have %d)", dev, ndev);
+    sr_batch *b = new sr_batch();
+    b->device = dev;
+    memset(&b->stats, 0, sizeof(b->stats));
+    int rc = b->pipe.init(dev, opts ? opts->tile_size : 0);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    int cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+    b->persistent = opts && opts->persistent_blocks > 0 ? opts->persistent_blocks : cus * 8;
+    b->use_graph = opts && opts->use_graph;
+    *out = b;
+    return SR_OK;
+}
+
+SR_API int sr_batch_destroy(sr_batch *b) {
+    if (!b) return SR_OK;
+    hipSetDevice(b->device);
+    b->pipe.destroy();
+    b->ids.release();
+    b->depth.release();
+    b->rgba.release();
+    b->lum.release();
+    delete b;
+    return SR_OK;
+}
+
+SR_API int sr_batch_render(sr_batch *b, const sr_view *views, int32_t n) {
+    if (!b || !views || n <= 0) return fail(SR_ERR_INVALID_ARGUMENT, "empty batch");
+    std::vector<DevView> hv(n);
+    std::vector<DevDraw> draws;
+    uint32_t in_tri = 0, xv = 0, items = 0, lum = 0;
+    uint64_t fb = 0;
+    bool tangents = false;
+    int T = b->pipe.tile_size;
+    b->fb_offset.assign(n, 0);
+    b->width.assign(n, 0);
+    b->height.assign(n, 0);
+    for (int i = 0; i < n; i++) {
+        const sr_view &v = views[i];
+        if (!v.scene || v.width <= 0 || v.height <= 0 || v.width > 32000 || v.height > 32000)
+            return fail(SR_ERR_INVALID_ARGUMENT, "view %d is invalid", i);
+        if (v.scene->device != b->device) return fail(SR_ERR_INVALID_ARGUMENT, "view %d scene on another device", i);
+        uint32_t tri0 = in_tri;
+        int rc = build_draws(v.scene, v.camera, i, in_tri, xv, draws);
+        if (rc) return rc;
+        fill_view(hv[i], v.scene, v.camera, v.width, v.height, T, items, tri0, fb, lum,
+                  v.exposure > 0.0f ? v.exposure : DEFAULT_EXPOSURE, 1u);
+        b->fb_offset[i] = fb;
+        b->width[i] = v.width;
+        b->height[i] = v.height;
+        items += (uint32_t)(hv[i].tiles_x * hv[i].tiles_y);
+        lum += (uint32_t)(hv[i].ref_tiles_x * hv[i].ref_tiles_y);
+        fb += (uint64_t)v.width * v.height;
+        tangents = tangents || v.scene->any_tangents;
+    }
+    hipSetDevice(b->device);
+    int rc;
+    if ((rc = b->ids.ensure(fb * 4))) return rc;
+    if ((rc = b->depth.ensure(fb * 4))) return rc;
+    if ((rc = b->rgba.ensure(fb * 4))) return rc;
+    if ((rc = b->lum.ensure((size_t)lum * 4))) return rc;
+    rc = b->pipe.run(hv, draws, xv, in_tri, items, tangents, (uint32_t *)b->ids.ptr, (float *)b->depth.ptr, nullptr, 0,
+                     (uint32_t *)b->rgba.ptr, (float *)b->lum.ptr, b->persistent, true);
+    if (rc) return rc;
+    b->stats.input_triangles = in_tri;
+    b->stats.setup_triangles = b->pipe.last_records;
+    b->stats.binned_pairs = b->pipe.last_pairs;
+    b->stats.pixels = fb;
+    b->stats.views = (uint32_t)n;
+    b->stats.work_items = items;
+    b->stats.ms_total = ev_ms(b->pipe.ev[0], b->pipe.ev[3]);
+    return SR_OK;
+}
+
+SR_API int sr_batch_read_vbuffer(sr_batch *b, int32_t view, uint32_t *tri_id, float *depth) {
+    if (!b || view < 0 || view >= (int)b->fb_offset.size()) return fail(SR_ERR_INVALID_ARGUMENT, "bad view index");
+    hipSetDevice(b->device);
+    size_t n = (size_t)b->width[view] * b->height[view];
+    if (tri_id)
+        HIP_TRY(hipMemcpy(tri_id, (uint32_t *)b->ids.ptr + b->fb_offset[view], n * 4, hipMemcpyDeviceToHost));
+    if (depth) HIP_TRY(hipMemcpy(depth, (float *)b->depth.ptr + b->fb_offset[view], n * 4, hipMemcpyDeviceToHost));
+    return SR_OK;
+}
+
+SR_API int sr_batch_read_pixels(sr_batch *b, int32_t view, uint32_t *pixels) {
+    if (!b || !pixels || view < 0 || view >= (int)b->fb_offset.size())
+        return fail(SR_ERR_INVALID_ARGUMENT, "bad view index");
+    hipSetDevice(b->device);
+    size_t rows = (size_t)(b->height[view] & ~1);
+    HIP_TRY(hipMemcpy(pixels, (uint32_t *)b->rgba.ptr + b->fb_offset[view], rows * b->width[view] * 4,
+                      hipMemcpyDeviceToHost));
+    return SR_OK;
+}
+
+SR_API int sr_batch_get_stats(const sr_batch *b, sr_batch_stats *stats) {
+    if (!b || !stats) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    *stats = b->stats;
+    return SR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// runtime.h -- host-side objects behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "swraster.h"
+
+namespace sr {
+
+extern thread_local std::string g_last_error;
+int fail(int code, const char *fmt, ...);
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+};
+
+struct PinBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+};
+
+struct Sphere {
+    float c[3];
+    float r;
+};
+
+struct HostPrim {
+    uint32_t vtx_base, num_vtx, idx_base, num_tri;
+    int32_t material;
+    bool tangents;
+    Sphere bs;
+};
+struct HostMesh {
+    std::vector<int> prims, opaque, translucent;
+};
+struct HostNode {
+    float transform[16];
+    int mesh;
+    Sphere bs_world;
+};
+
+// Per-frame pipeline state (transient buffers, stream, events).
+struct Pipeline {
+    int device = 0;
+    int tile_size = 16;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    DevBuf views, draws, xv, xt, tri_counts, tri_base, bin_count, bin_start, bin_end, partials, list;
+    DevBuf rec_geo, rec_raster, rec_shade, rec_tan, status;
+    PinBuf stage, status_host;
+    uint32_t rec_cap = 0, pair_cap = 0;
+    uint32_t last_records = 0, last_pairs = 0;
+    int init(int device, int tile);
+    void destroy();
+    int run(const std::vector<DevView> &views, const std::vector<DevDraw> &draws, uint32_t total_xv,
+            uint32_t total_in, uint32_t total_items, bool any_tangents, uint32_t *out_ids, float *out_depth,
+            float *out_color, uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks,
+            bool sync);
+};
+
+}  // namespace sr
+
+struct sr_scene {
+    int device = 0;
+    std::vector<sr::DevBuf> buffers;
+    sr::DevBuf dscene;            // DevScene struct in device memory
+    std::vector<sr::HostNode> nodes;
+    std::vector<sr::HostMesh> meshes;
+    std::vector<sr::HostPrim> prims;
+    bool any_tangents = false;
+    uint64_t triangles = 0, vertices = 0, device_bytes = 0;
+};
+
+namespace sr {
+int build_draws(const sr_scene *scene, const sr_camera &cam, int view, uint32_t &in_tri, uint32_t &xv,
+                std::vector<DevDraw> &out);
+void fill_view(DevView &v, const sr_scene *scene, const sr_camera &cam, int W, int H, int T, uint32_t item_base,
+               uint32_t in_tri_base, uint64_t fb_offset, uint32_t lum_offset, float exposure, uint32_t flags);
+}  // namespace sr

@@ -1,0 +1,162 @@
+"""GPU parity: the HIP pipeline against the CPU oracle on the same scenes.
+
+Bar: visibility buffer (canonical triangle id + depth bits) bit-identical;
+RGBA8 within +-1 LSB per channel; linear colour and 64px-tile centre
+luminance reported (and expected identical, since both sides evaluate the
+same f32 expressions with IEEE division and square root and no FMA).
+"""
+import numpy as np
+import pytest
+
+from conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def render_gpu(sr, gscene, cam, w, h, tile=0, dt=0.0):
+    r = sr.Renderer(w, h, tile_size=tile)
+    r.render_scene(gscene, cam)
+    ids, depth = r.read_vbuffer()
+    rgb = r.read_color()
+    lum = r.read_tile_luminance()
+    r.update_auto_exposure(dt)
+    px = r.blit_to_buffer()
+    st = r.stats()
+    exposure = r.exposure
+    r.close()
+    return {"ids": ids, "depth": depth, "rgb": rgb, "lum": lum, "px": px, "stats": st, "exposure": exposure}
+
+
+def render_oracle(orc, scene, cam, w, h, dt=0.0, threads=8):
+    o = orc.OracleRenderer(w, h)
+    o.render_scene(orc.OracleScene(scene), cam, threads=threads)
+    f = o.read_frame()
+    f["lum"] = o.tile_luminance()
+    o.update_auto_exposure(dt)
+    f["px"] = o.blit()
+    f["exposure"] = o.exposure
+    f["setup"] = o.setup_count()
+    return f
+
+
+def compare(sr, g, o, name):
+    ids_eq = np.array_equal(g["ids"], o["ids"])
+    d_eq = np.array_equal(bits(g["depth"]), bits(o["depth"]))
+    nbad = int((g["ids"] != o["ids"]).sum() + (bits(g["depth"]) != bits(o["depth"])).sum())
+    if not (ids_eq and d_eq):
+        bad = np.argwhere((g["ids"] != o["ids"]) | (bits(g["depth"]) != bits(o["depth"])))
+        y, x = bad[0]
+        print(f"[{name}] first mismatch at ({x},{y}): gpu id {g['ids'][y, x]} depth {g['depth'][y, x]!r}, "
+              f"oracle id {o['ids'][y, x]} depth {o['depth'][y, x]!r}; {nbad} mismatching values")
+    assert ids_eq and d_eq, f"{name}: visibility buffer differs in {nbad} values"
+    rgba_g, rgba_o = sr.unpack_rgba(g["px"]).astype(int), sr.unpack_rgba(o["px"]).astype(int)
+    lsb = int(np.abs(rgba_g - rgba_o).max())
+    rgb_err = float(np.nanmax(np.abs(g["rgb"].astype(np.float64) - o["rgb"])))
+    print(f"[{name}] vbuffer identical; covered {(g['ids'] != sr.NO_ID).sum()} px; max RGBA8 diff {lsb} LSB; "
+          f"max linear diff {rgb_err:.3g}; colour bit-identical {np.array_equal(bits(g['rgb']), bits(o['rgb']))}")
+    assert lsb <= 1, f"{name}: RGBA8 differs by {lsb} LSB"
+    assert np.array_equal(bits(g["lum"]), bits(o["lum"])) or np.allclose(g["lum"], o["lum"], rtol=1e-6)
+    assert g["exposure"] == o["exposure"]
+
+
+@pytest.fixture(scope="module")
+def c1(sr, scenes, env):
+    scene, spec = scenes.make_c1()
+    scene.prepare(env, cache_key=scene.name)
+    return scene, spec, sr.Scene(scene)
+
+
+def test_c1_parity(sr, orc, scenes, c1):
+    scene, spec, gscene = c1
+    cam = scenes.make_camera(spec)
+    w, h = spec["width"], spec["height"]
+    g = render_gpu(sr, gscene, cam, w, h)
+    o = render_oracle(orc, scene, cam, w, h)
+    assert g["stats"]["setup_triangles"] == o["setup"]
+    compare(sr, g, o, "c1")
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_c1_tile_sizes_identical(sr, orc, scenes, c1, tile):
+    scene, spec, gscene = c1
+    cam = scenes.make_camera(spec, 640, 360)
+    g = render_gpu(sr, gscene, cam, 640, 360, tile=tile)
+    o = render_oracle(orc, scene, cam, 640, 360)
+    compare(sr, g, o, f"c1 tile {tile}")
+
+
+@pytest.mark.parametrize("size", [(97, 61), (333, 202), (64, 64), (1, 1), (130, 2)])
+def test_c1_odd_viewports(sr, orc, scenes, c1, size):
+    scene, spec, gscene = c1
+    w, h = size
+    cam = scenes.make_camera(spec, w, h)
+    compare(sr, render_gpu(sr, gscene, cam, w, h), render_oracle(orc, scene, cam, w, h), f"c1 {w}x{h}")
+
+
+def test_c1_inside_cluster_clipping(sr, orc, scenes, c1):
+    """Camera inside the sphere cluster: near/side-plane clipping and
+    negative depths (view-space cull near plane sits behind the eye)."""
+    scene, spec, gscene = c1
+    s = dict(spec)
+    s["position"] = [0.3, 0.2, 1.0]
+    s["yaw"] = 0.4
+    cam = scenes.make_camera(s, 800, 600)
+    g = render_gpu(sr, gscene, cam, 800, 600)
+    o = render_oracle(orc, scene, cam, 800, 600)
+    assert o["setup"] > 0
+    compare(sr, g, o, "c1 inside")
+
+
+def test_c1_exposure_adaptation(sr, orc, scenes, c1):
+    scene, spec, gscene = c1
+    cam = scenes.make_camera(spec, 640, 360)
+    compare(sr, render_gpu(sr, gscene, cam, 640, 360, dt=0.25), render_oracle(orc, scene, cam, 640, 360, dt=0.25),
+            "c1 dt=0.25")
+
+
+def test_c2_parity(sr, orc, scenes, env):
+    scene, spec = scenes.make_c2()
+    scene.prepare(env, cache_key=scene.name)
+    gscene = sr.Scene(scene)
+    cam = scenes.make_camera(spec)
+    w, h = spec["width"], spec["height"]
+    g = render_gpu(sr, gscene, cam, w, h)
+    o = render_oracle(orc, scene, cam, w, h, threads=16)
+    compare(sr, g, o, "c2")
+
+
+def test_c3_parity(sr, orc, scenes, env):
+    scene, spec = scenes.make_c3()
+    scene.prepare(env, cache_key=scene.name)
+    gscene = sr.Scene(scene)
+    cam = scenes.make_camera(spec)
+    w, h = spec["width"], spec["height"]
+    g = render_gpu(sr, gscene, cam, w, h)
+    o = render_oracle(orc, scene, cam, w, h, threads=16)
+    compare(sr, g, o, "c3")
+
+
+def test_determinism(sr, scenes, c1):
+    scene, spec, gscene = c1
+    cam = scenes.make_camera(spec)
+    a = render_gpu(sr, gscene, cam, spec["width"], spec["height"])
+    b = render_gpu(sr, gscene, cam, spec["width"], spec["height"])
+    assert np.array_equal(a["ids"], b["ids"]) and np.array_equal(bits(a["depth"]), bits(b["depth"]))
+    assert np.array_equal(a["px"], b["px"])
+
+
+def test_batch_matches_plain(sr, scenes, c1):
+    scene, spec, gscene = c1
+    views, plain = [], []
+    for i, (w, h) in enumerate([(320, 240), (640, 480), (1280, 720), (200, 150)]):
+        s = dict(spec)
+        s["yaw"] = 0.1 * i
+        cam = scenes.make_camera(s, w, h)
+        views.append((gscene, cam, w, h, 2.0))
+        plain.append(render_gpu(sr, gscene, cam, w, h))
+    b = sr.Batch()
+    b.render(views)
+    for i, p in enumerate(plain):
+        ids, depth = b.read_vbuffer(i)
+        assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"]))
+        assert np.array_equal(b.read_pixels(i), p["px"])
