@@ -1,0 +1,176 @@
+"""Benchmark of the MI355X rasterization pipeline.
+
+One step = one plain frame of the drop-in path on one GPU:
+render_scene (transform, clip, bin, raster, shade) + update_auto_exposure(0)
++ blit into device memory, with scene geometry, textures and the voxel grid
+already resident in HBM.  The default workload is the small-triangle stress
+scene at 1920x1080 (724x724 heightfield, 1,045,458 triangles, one 2048^2
+texture), the configuration the Gpix/s headline is quoted on.
+
+Prints one JSON line:
+  value          Gpix/s (W*H / frame time), whole job (one GPU)
+  mtri_per_s     submitted triangles / frame time
+  p50/p99 ms     per-frame latency over the timed steps
+  roofline       dominant kernel (raster+shade): algorithmic bytes per launch
+                 (vbuffer write 8 + read 8 + RGBA8 write 4 per pixel, plus one
+                 4-byte base-colour texel per covered textured pixel) over its
+                 HIP-event time, against 8 TB/s
+  frame_roofline the same for the whole frame with the full per-frame
+                 algorithmic bytes (vertices, indices, pixels, texels)
+  cpu_baseline   the CPU oracle in reference-like threaded mode (128-triangle
+                 jobs into shared 64px-tile queues, one job per tile) on this
+                 host's cores, on a bounded sample of the same frames
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mtri/s & Gpix/s @1080p; batched frames/s vs plain; p50/p99 frame latency"
+
+
+def algorithmic_bytes(scene, w, h, n_cov):
+    textured = any(m.get("base_color_tex") is not None for m in scene.materials)
+    nv = sum(len(p["positions"]) for p in scene.primitives)
+    nt = scene.triangles
+    per_vertex = 24 + (8 if textured else 0)
+    frame = per_vertex * nv + 12 * nt + 20 * w * h + (4 * n_cov if textured else 0)
+    kernel = 20 * w * h + (4 * n_cov if textured else 0)
+    return frame, kernel
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import __graft_entry__ as entry
+    sr = entry.load_package()
+    import swraster_viewer_amd.scenes as sc
+    import ctypes as C
+
+    scene, spec = sc.CONFIGS[args.config]()
+    scene.prepare(sr.Environment.load(), cache_key=scene.name)
+    cam = sc.make_camera(spec)
+    w, h = spec["width"], spec["height"]
+    gscene = sr.Scene(scene)
+    r = sr.Renderer(w, h, tile_size=args.tile)
+    L = sr.lib()
+    dev_px = C.c_void_p()
+    import ctypes.util  # noqa: F401
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipDeviceSynchronize.restype = C.c_int
+    assert hip.hipMalloc(C.byref(dev_px), w * h * 4) == 0
+
+    def frame():
+        sr.check(L.sr_renderer_render_scene(r.handle, gscene.handle, C.byref(cam.c)))
+        sr.check(L.sr_renderer_update_auto_exposure(r.handle, 0.0))
+        sr.check(L.sr_renderer_blit_to_device(r.handle, dev_px))
+
+    for _ in range(args.warmup):
+        frame()
+    hip.hipDeviceSynchronize()
+    lat = []
+    kernel_ms = []
+    stage = {"ms_setup": 0.0, "ms_bin": 0.0, "ms_raster_shade": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        frame()
+        lat.append((time.perf_counter() - t) * 1e3)
+        st = r.stats()
+        kernel_ms.append(st["ms_raster_shade"])
+        for k in stage:
+            stage[k] += st[k]
+    hip.hipDeviceSynchronize()
+    total = time.perf_counter() - t0
+    ms = total * 1e3 / args.steps
+    ids, _ = r.read_vbuffer()
+    n_cov = int((ids != sr.NO_ID).sum())
+    frame_b, kernel_b = algorithmic_bytes(scene, w, h, n_cov)
+    k_ms = float(np.mean(kernel_ms))
+    achieved = kernel_b / (k_ms * 1e-3) / 1e9
+    frame_achieved = frame_b / (ms * 1e-3) / 1e9
+    st = r.stats()
+    out = {
+        "metric": METRIC,
+        "value": round(w * h / (ms * 1e-3) / 1e9, 4),
+        "unit": "Gpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, swraster-viewer_amd/scenes.py)",
+        "config": {"workload": {"c1": "c1_icospheres_720p", "c2": "c2_heightfield_1080p",
+                                "c3": "c3_overdraw_2160p"}[args.config],
+                   "triangles": scene.triangles, "width": w, "height": h,
+                   "tile_size": r.stats()["tiles"] and (args.tile or 16),
+                   "step": "render_scene + update_auto_exposure(0) + blit to device"},
+        "mtri_per_s": round(scene.triangles / (ms * 1e-3) / 1e6, 2),
+        "frames_per_s": round(1e3 / ms, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                       "p99": round(float(np.percentile(lat, 99)), 4)},
+        "stages_ms": {k: round(v / args.steps, 4) for k, v in stage.items()},
+        "setup_triangles": st["setup_triangles"],
+        "binned_pairs": st["binned_pairs"],
+        "covered_pixels": n_cov,
+        "roofline": {"kernel": "k_raster_shade", "bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "algorithmic_bytes_per_launch": kernel_b,
+                     "avg_launch_ms": round(k_ms, 5)},
+        "frame_roofline": {"achieved": round(frame_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(frame_achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": frame_b},
+    }
+    traffic_path = os.path.join(REPO, "profiles", f"traffic_{args.config}.json")
+    if os.path.exists(traffic_path):
+        with open(traffic_path) as fh:
+            out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch")
+
+    if not args.no_cpu_baseline:
+        orc = entry.load_oracle()
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        o = orc.OracleRenderer(w, h)
+        osc = orc.OracleScene(scene)
+        o.render_scene(osc, cam, threads=cores, canonical=False)
+        times = []
+        t_end = time.perf_counter() + args.cpu_seconds
+        while time.perf_counter() < t_end and len(times) < 50:
+            t = time.perf_counter()
+            o.render_scene(osc, cam, threads=cores, canonical=False)
+            o.update_auto_exposure(0.0)
+            o.blit()
+            times.append(time.perf_counter() - t)
+        med = float(np.median(times))
+        out["cpu_baseline"] = {"value": round(w * h / med / 1e9, 5), "unit": "Gpix/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{len(times)} frames of the same workload after 1 warm-up "
+                                         f"(median {med * 1e3:.1f} ms/frame, "
+                                         f"{scene.triangles / med / 1e6:.2f} Mtri/s)"}
+        out["gpu_vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

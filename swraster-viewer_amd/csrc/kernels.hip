@@ -204,7 +204,7 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
 }
 
 __device__ __noinline__ int clip_count_general(const float c[3][4], int first) {
-    float poly[16][4], np[16][4];
+    float poly[10][4], np[10][4];
     for (int k = 0; k < 3; k++)
         for (int j = 0; j < 4; j++) poly[k][j] = c[k][j];
     int len = 3;
@@ -448,11 +448,50 @@ struct SetupOut {
     int T;
 };
 
-__device__ void emit_triangle(const DevView &V, int view, const CV *v0, const CV *v1, const CV *v2, uint32_t id,
+// Wave-aggregated counting: every lane of the wave calls this together; lanes
+// with bin == ~0u take no part.  One atomic per distinct bin per wave.
+__device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t bin) {
+    bool pending = bin != 0xFFFFFFFFu;
+    while (__ballot(pending)) {
+        uint64_t pm = __ballot(pending);
+        int first = __ffsll((unsigned long long)pm) - 1;
+        uint32_t leader = __shfl(bin, first, 64);
+        bool mine = pending && bin == leader;
+        uint64_t m = __ballot(mine);
+        if (mine && (int)(threadIdx.x & 63) == first) atomicAdd(&counts[leader], (uint32_t)__popcll(m));
+        pending = pending && !mine;
+    }
+}
+
+// Wave-aggregated slot claim: returns this lane's slot (bin != ~0u).
+__device__ __forceinline__ uint32_t wave_claim(uint32_t *cursor, uint32_t bin) {
+    bool pending = bin != 0xFFFFFFFFu;
+    uint32_t slot = 0;
+    int lane = threadIdx.x & 63;
+    while (__ballot(pending)) {
+        uint64_t pm = __ballot(pending);
+        int first = __ffsll((unsigned long long)pm) - 1;
+        uint32_t leader = __shfl(bin, first, 64);
+        bool mine = pending && bin == leader;
+        uint64_t m = __ballot(mine);
+        uint32_t base = 0;
+        if (lane == first) base = atomicAdd(&cursor[leader], (uint32_t)__popcll(m));
+        base = __shfl(base, first, 64);
+        if (mine) slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        pending = pending && !mine;
+    }
+    return slot;
+}
+
+// Snap, cull, bbox and setup of one (post-clip) triangle (bin_triangle,
+// src/renderer.rs:668-831).  Writes the records and returns the bin rectangle
+// (empty when culled or when the box is empty).
+__device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV *v1, const CV *v2, uint32_t id,
                               int material, uint32_t matflags, const SetupOut &O) {
+    Rect none = {0, 0, 0, 0};
     if (id >= O.cap) {
         atomicOr(&O.status->overflow_records, 1u);
-        return;
+        return none;
     }
     const CV *tv[3] = {v0, v1, v2};
     int32_t px[3], py[3];
@@ -466,7 +505,7 @@ __device__ void emit_triangle(const DevView &V, int view, const CV *v0, const CV
     int32_t area = wsub(wmul(wsub(px[1], px[0]), wsub(py[2], py[0])), wmul(wsub(px[2], px[0]), wsub(py[1], py[0])));
     if (area > 0) {
         O.geo[id] = make_uint4(0u, 0u, SR_GEO_EMPTY, (uint32_t)view);
-        return;
+        return none;
     }
     int32_t mnx = max(min(min(px[0], px[1]), px[2]), 0);
     int32_t mny = max(min(min(py[0], py[1]), py[2]), 0);
@@ -476,7 +515,7 @@ __device__ void emit_triangle(const DevView &V, int view, const CV *v0, const CV
     int32_t pmxx = wadd(mxx, 16) >> 4, pmxy = wadd(mxy, 16) >> 4;
     if (!(pmnx < pmxx && pmny < pmxy)) {
         O.geo[id] = make_uint4(0u, 0u, SR_GEO_EMPTY, (uint32_t)view);
-        return;
+        return none;
     }
     float ooa = 1.0f / (float)wabs(area);
     float oow[3], zow[3], uw[3], vw[3];
@@ -527,45 +566,128 @@ __device__ void emit_triangle(const DevView &V, int view, const CV *v0, const CV
         Tn[1] = make_float4(tda.y, tda.z, tdb.x, tdb.y);
         Tn[2] = make_float4(tdb.z, v0->tw, v1->tw - v0->tw, v2->tw - v0->tw);
     }
-    Rect r = bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
-    for (int gy = r.gy0; gy < r.gy1; gy++)
-        for (int gx = r.gx0; gx < r.gx1; gx++)
-            atomicAdd(&O.bin_count[V.item_base + (uint32_t)(gy * V.tiles_x + gx)], 1u);
+    return bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
+}
+
+// Clipped triangles (rare): Sutherland-Hodgman with attributes, fan, emit,
+// plain atomics for the bins.
+__device__ __noinline__ void clip_emit(const DevView &V, int view, const CV *v, int first, uint32_t id, int material,
+                                       uint32_t matflags, const SetupOut &O) {
+    CV poly[10], np[10];
+    poly[0] = v[0]; poly[1] = v[1]; poly[2] = v[2];
+    int len = clip_general(poly, np, first);
+    for (int i = 1; i + 1 < len; i++) {
+        Rect r = emit_triangle(V, view, &poly[0], &poly[i], &poly[i + 1], id + (uint32_t)(i - 1), material, matflags, O);
+        for (int gy = r.gy0; gy < r.gy1; gy++)
+            for (int gx = r.gx0; gx < r.gx1; gx++)
+                atomicAdd(&O.bin_count[V.item_base + (uint32_t)(gy * V.tiles_x + gx)], 1u);
+    }
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+
+// Bin counting / scattering for one rectangle per lane, called by the whole
+// wave.  Rectangles of at most SMALL_RECT bins (the common small-triangle
+// case, where neighbouring lanes hit the same bins) use wave-aggregated
+// atomics; larger ones are taken one at a time by the whole wave, one bin per
+// lane, so each triangle costs ceil(bins / 64) wave steps of distinct atomics.
+#define SMALL_RECT 4
+
+__device__ __forceinline__ void bin_rect_count(const DevView *views, int view, Rect r, uint32_t *counts) {
+    int rw = r.gx1 - r.gx0, rh = r.gy1 - r.gy0;
+    int nb = (rw > 0 && rh > 0) ? rw * rh : 0;
+    const DevView &V = views[view];
+    int ns = nb <= SMALL_RECT ? nb : 0;
+    int maxs = wave_max(ns);
+    for (int k = 0; k < maxs; k++) {
+        uint32_t bin = 0xFFFFFFFFu;
+        if (k < ns) bin = V.item_base + (uint32_t)((r.gy0 + k / rw) * V.tiles_x + r.gx0 + k % rw);
+        wave_count(counts, bin);
+    }
+    uint64_t big = __ballot(nb > SMALL_RECT);
+    int lane = threadIdx.x & 63;
+    while (big) {
+        int L = __ffsll((unsigned long long)big) - 1;
+        big &= big - 1;
+        int bx0 = __builtin_amdgcn_readlane(r.gx0, L), by0 = __builtin_amdgcn_readlane(r.gy0, L);
+        int bw = __builtin_amdgcn_readlane(rw, L), bn = __builtin_amdgcn_readlane(nb, L);
+        int bv = __builtin_amdgcn_readlane(view, L);
+        const DevView &B = views[bv];
+        for (int k = lane; k < bn; k += 64)
+            atomicAdd(&counts[B.item_base + (uint32_t)((by0 + k / bw) * B.tiles_x + bx0 + k % bw)], 1u);
+    }
+}
+
+__device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view, Rect r, uint32_t id, uint32_t *cursor,
+                                                 uint32_t *list, uint32_t pair_cap, DevStatus *status) {
+    int rw = r.gx1 - r.gx0, rh = r.gy1 - r.gy0;
+    int nb = (rw > 0 && rh > 0) ? rw * rh : 0;
+    const DevView &V = views[view];
+    int ns = nb <= SMALL_RECT ? nb : 0;
+    int maxs = wave_max(ns);
+    for (int k = 0; k < maxs; k++) {
+        uint32_t bin = 0xFFFFFFFFu;
+        if (k < ns) bin = V.item_base + (uint32_t)((r.gy0 + k / rw) * V.tiles_x + r.gx0 + k % rw);
+        uint32_t pos = wave_claim(cursor, bin);
+        if (k < ns) {
+            if (pos < pair_cap) list[pos] = id;
+            else atomicOr(&status->overflow_pairs, 1u);
+        }
+    }
+    uint64_t big = __ballot(nb > SMALL_RECT);
+    int lane = threadIdx.x & 63;
+    while (big) {
+        int L = __ffsll((unsigned long long)big) - 1;
+        big &= big - 1;
+        int bx0 = __builtin_amdgcn_readlane(r.gx0, L), by0 = __builtin_amdgcn_readlane(r.gy0, L);
+        int bw = __builtin_amdgcn_readlane(rw, L), bn = __builtin_amdgcn_readlane(nb, L);
+        int bv = __builtin_amdgcn_readlane(view, L);
+        uint32_t bid = (uint32_t)__builtin_amdgcn_readlane((int)id, L);
+        const DevView &B = views[bv];
+        for (int k = lane; k < bn; k += 64) {
+            uint32_t pos = atomicAdd(&cursor[B.item_base + (uint32_t)((by0 + k / bw) * B.tiles_x + bx0 + k % bw)], 1u);
+            if (pos < pair_cap) list[pos] = bid;
+            else atomicOr(&status->overflow_pairs, 1u);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
                                                const float4 *xv, const float4 *xt, const uint32_t *tri_base,
                                                SetupOut O) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
-        const DevDraw &D = draws[d];
-        const DevView &V = views[D.view];
-        const DevScene &S = *V.scene;
-        uint32_t lt = t - D.in_tri_base;
-        CV v[3];
-        for (int k = 0; k < 3; k++) {
-            uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-            load_cv(xv, (D.flags & SR_DRAW_TANGENTS) ? xt : nullptr, D.xv_base + vi, v[k]);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t t = base + threadIdx.x;
+        Rect r = {0, 0, 0, 0};
+        int view = 0;
+        if (t < total) {
+            int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+            const DevDraw &D = draws[d];
+            view = D.view;
+            const DevView &V = views[D.view];
+            const DevScene &S = *V.scene;
+            uint32_t lt = t - D.in_tri_base;
+            CV v[3];
+            for (int k = 0; k < 3; k++) {
+                uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
+                load_cv(xv, (D.flags & SR_DRAW_TANGENTS) ? xt : nullptr, D.xv_base + vi, v[k]);
+            }
+            uint32_t matflags = S.materials[D.material].flags;
+            uint32_t id = tri_base[t];
+            int cls = -1;
+            if (D.flags & SR_DRAW_CLIP) {
+                float c[3][4];
+                for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
+                cls = clip_classify(c);
+            }
+            if (cls == -1) r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
+            else if (cls >= 0) clip_emit(V, D.view, v, cls, id, D.material, matflags, O);
         }
-        uint32_t matflags = S.materials[D.material].flags;
-        uint32_t id = tri_base[t];
-        if (!(D.flags & SR_DRAW_CLIP)) {
-            emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
-            continue;
-        }
-        float c[3][4];
-        for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
-        int cls = clip_classify(c);
-        if (cls == -2) continue;
-        if (cls == -1) {
-            emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
-            continue;
-        }
-        CV poly[16], np[16];
-        poly[0] = v[0]; poly[1] = v[1]; poly[2] = v[2];
-        int len = clip_general(poly, np, cls);
-        for (int i = 1; i + 1 < len; i++)
-            emit_triangle(V, D.view, &poly[0], &poly[i], &poly[i + 1], id + (uint32_t)(i - 1), D.material, matflags, O);
+        bin_rect_count(views, view, r, O.bin_count);
     }
 }
 
@@ -577,18 +699,20 @@ __global__ void __launch_bounds__(256) k_scatter(const DevView *views, const uin
                                                  DevStatus *status_w, int T) {
     uint32_t total = status->total_records;
     if (total > rec_cap) total = rec_cap;
-    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
-        uint4 g = geo[id];
-        if (g.z & SR_GEO_EMPTY) continue;
-        const DevView &V = views[g.w];
-        int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
-        Rect r = bin_rect(V, mnx, mny, mxx, mxy, T);
-        for (int gy = r.gy0; gy < r.gy1; gy++)
-            for (int gx = r.gx0; gx < r.gx1; gx++) {
-                uint32_t pos = atomicAdd(&cursor[V.item_base + (uint32_t)(gy * V.tiles_x + gx)], 1u);
-                if (pos < pair_cap) list[pos] = id;
-                else atomicOr(&status_w->overflow_pairs, 1u);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t id = base + threadIdx.x;
+        Rect r = {0, 0, 0, 0};
+        uint32_t view = 0;
+        if (id < total) {
+            uint4 g = geo[id];
+            view = g.w;
+            if (!(g.z & SR_GEO_EMPTY)) {
+                int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
+                r = bin_rect(views[view], mnx, mny, mxx, mxy, T);
             }
+        }
+        bin_rect_scatter(views, (int)view, r, id, cursor, list, pair_cap, status_w);
     }
 }
 
@@ -1006,13 +1130,57 @@ __device__ __forceinline__ bool alpha_pass(const DevScene &S, const float4 *R, c
     return a >= M.alpha_cutoff;
 }
 
+#define SMALL_QUADS 4
+
+__device__ __forceinline__ float rlf(float v, int L) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), L)); }
+__device__ __forceinline__ int rli(int v, int L) { return __builtin_amdgcn_readlane(v, L); }
+
+__device__ __forceinline__ void bcast_setup(const TriSetup &s, TriSetup &o, int L) {
+    for (int e = 0; e < 3; e++) {
+        o.A[e] = rlf(s.A[e], L); o.B[e] = rlf(s.B[e], L); o.C[e] = rlf(s.C[e], L);
+        o.SX[e] = rlf(s.SX[e], L); o.SY[e] = rlf(s.SY[e], L);
+    }
+    o.ooa = rlf(s.ooa, L); o.oa = rlf(s.oa, L); o.oda = rlf(s.oda, L); o.odb = rlf(s.odb, L);
+    o.za = rlf(s.za, L); o.zda = rlf(s.zda, L); o.zdb = rlf(s.zdb, L);
+    o.cmnx = rli(s.cmnx, L); o.cmny = rli(s.cmny, L); o.cmxx = rli(s.cmxx, L); o.cmxy = rli(s.cmxy, L);
+    o.ox0 = rli(s.ox0, L); o.oy0 = rli(s.oy0, L);
+    o.coarse = rli((int)s.coarse, L) != 0;
+}
+
+// Coverage and depth of one quad of one triangle: the reference's walk origin
+// and block test, then per-lane edge chains, barycentrics and z; covered
+// lanes with a non-NaN depth enter the 64-bit minimum.
+template <int T>
+__device__ __forceinline__ void raster_quad(const DevScene &S, const RasterArgs &P, const TriSetup &ts, uint32_t gflags,
+                                            uint32_t id, int qpx, int qpy, int tx0, int ty0,
+                                            unsigned long long *keys) {
+    int bx, by;
+    if (!quad_origin(ts, qpx, qpy, bx, by)) return;
+    if (ts.coarse && block_rejected(ts, bx, by)) return;
+    int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
+    uint32_t inv_id = ~id;
+    for (int s = 0; s < 4; s++) {
+        int sx = s & 1, sy = s >> 1;
+        float w0 = edge_chain(ts, 0, bx, by, sx, sy, i, j);
+        float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
+        float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
+        if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)) continue;
+        float b1 = w1 * ts.ooa, b2 = w2 * ts.ooa;
+        float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
+        float z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
+        if (z != z) continue;
+        if (gflags & SR_GEO_ALPHA) {
+            if (!alpha_pass(S, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id, b1, b2, w)) continue;
+        }
+        unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)inv_id;
+        atomicMin(&keys[(qpy - ty0 + sy) * T + (qpx - tx0) + sx], key);
+    }
+}
+
 template <int T, int NT>
 __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned long long *keys) {
     constexpr int QW = T / 2;                 // quads per row
     constexpr int NQ = QW * QW;               // quads per tile
-    constexpr int LPT = NQ < 64 ? NQ : 64;    // lanes per triangle
-    constexpr int TPW = 64 / LPT;             // triangles per wave step
-    constexpr int QPL = NQ / LPT;             // quads per lane
     constexpr int NW = NT / 64;               // waves
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -1035,42 +1203,50 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
     __syncthreads();
 
     // ---- phase 1: coverage and depth ----
+    // Each wave takes 64 bin entries at a time (one record load per lane, all
+    // in flight together).  Triangles touching at most SMALL_QUADS quads of
+    // this tile are walked by their own lane; larger ones are broadcast and
+    // walked by the whole wave with one quad per lane.
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
     if (end > P.pair_cap) end = P.pair_cap;
-    const int slot = wave * TPW + lane / LPT;
-    const int ql = lane % LPT;
-    for (uint32_t e = start + (uint32_t)slot; e < end; e += (uint32_t)(NW * TPW)) {
-        uint32_t id = P.list[e];
-        if (TPW == 1) id = __builtin_amdgcn_readfirstlane(id);
-        uint4 g = P.geo[id];
-        const float4 *R = P.raster + 4 * (size_t)id;
+    for (uint32_t c0 = start + (uint32_t)wave * 64u; c0 < end; c0 += (uint32_t)NW * 64u) {
+        uint32_t e = c0 + (uint32_t)lane;
+        bool valid = e < end;
+        uint32_t id = valid ? P.list[e] : 0u;
         TriSetup ts;
-        tri_setup(g, R, tx0, ty0, ts);
-        uint32_t inv_id = ~id;
-        for (int k = 0; k < QPL; k++) {
-            int q = ql + k * LPT;
-            int qx = q % QW, qy = q / QW;
-            int qpx = tx0 + 2 * qx, qpy = ty0 + 2 * qy;
-            int bx, by;
-            if (!quad_origin(ts, qpx, qpy, bx, by)) continue;
-            if (ts.coarse && block_rejected(ts, bx, by)) continue;
-            int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
-            for (int s = 0; s < 4; s++) {
-                int sx = s & 1, sy = s >> 1;
-                float w0 = edge_chain(ts, 0, bx, by, sx, sy, i, j);
-                float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
-                float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
-                if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)) continue;
-                float b1 = w1 * ts.ooa, b2 = w2 * ts.ooa;
-                float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
-                float z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
-                if (z != z) continue;
-                if (g.z & SR_GEO_ALPHA) {
-                    if (!alpha_pass(S, R, P.shade + 8 * (size_t)id, b1, b2, w)) continue;
-                }
-                unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)inv_id;
-                atomicMin(&keys[(2 * qy + sy) * T + 2 * qx + sx], key);
+        uint4 g = make_uint4(0u, 0u, 0u, 0u);
+        int nqx = 0, nqy = 0, qx0 = 0, qy0 = 0;
+        if (valid) {
+            g = P.geo[id];
+            tri_setup(g, P.raster + 4 * (size_t)id, tx0, ty0, ts);
+            qx0 = max(ts.ox0, tx0);
+            qy0 = max(ts.oy0, ty0);
+            int qx1 = min(ts.cmxx, tx0 + T), qy1 = min(ts.cmxy, ty0 + T);
+            nqx = qx1 > qx0 ? (qx1 - qx0 + 1) >> 1 : 0;
+            nqy = qy1 > qy0 ? (qy1 - qy0 + 1) >> 1 : 0;
+        } else {
+            memset(&ts, 0, sizeof(ts));
+        }
+        int nq = nqx * nqy;
+        bool small = nq <= SMALL_QUADS;
+        if (valid && small) {
+            for (int k = 0; k < nq; k++) {
+                int qpx = qx0 + 2 * (k % nqx), qpy = qy0 + 2 * (k / nqx);
+                raster_quad<T>(S, P, ts, g.z, id, qpx, qpy, tx0, ty0, keys);
+            }
+        }
+        uint64_t big = __ballot(valid && !small);
+        while (big) {
+            int L = __ffsll((unsigned long long)big) - 1;
+            big &= big - 1;
+            TriSetup bs;
+            bcast_setup(ts, bs, L);
+            uint32_t bflags = (uint32_t)__builtin_amdgcn_readlane((int)g.z, L);
+            uint32_t bid = (uint32_t)__builtin_amdgcn_readlane((int)id, L);
+            for (int q = lane; q < NQ; q += 64) {
+                int qx = q % QW, qy = q / QW;
+                raster_quad<T>(S, P, bs, bflags, bid, tx0 + 2 * qx, ty0 + 2 * qy, tx0, ty0, keys);
             }
         }
     }
