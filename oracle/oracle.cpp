@@ -1834,4 +1834,58 @@ __attribute__((visibility("default"))) int orc_mip_chain(const sr_texture_desc *
     return 0;
 }
 
+// ---- unit entry points for closed-form tests ---------------------------------
+// sample4_rgb with its mip selection on a texture built from `t` (mips
+// generated); returns the chosen mip level.
+__attribute__((visibility("default"))) int orc_sample4(const sr_texture_desc *t, int wrap_s, int wrap_t, const float u[4],
+                                                       const float v[4], const float du_dv[4], float out_rgb[12]) {
+    Texture tex;
+    texture_from_base(tex, t->width, t->height, t->type, t->texels, true);
+    Sampled s;
+    s.tex = &tex;
+    s.wrap_s = wrap_s;
+    s.wrap_t = wrap_t;
+    V4<float> U, Vv, D;
+    for (int i = 0; i < 4; i++) { U[i] = u[i]; Vv[i] = v[i]; D[i] = du_dv[i]; }
+    V3x4<float> c = sample4_rgb(s, U, Vv, D);
+    for (int i = 0; i < 4; i++) { out_rgb[3 * i] = c.x[i]; out_rgb[3 * i + 1] = c.y[i]; out_rgb[3 * i + 2] = c.z[i]; }
+    return (int)compute_mip_level(tex, D);
+}
+
+// Trilinear voxel lookup on a full SH4 grid (16 floats per voxel) at four
+// world positions; writes rgb[4 coeff][3 ch][4 lane] and w[4 coeff][4 lane].
+__attribute__((visibility("default"))) int orc_voxel_lookup(int w, int h, int d, const float wmin[3], const float wmax[3],
+                                                            const float *sh4, const float pos[12], float *rgb_out,
+                                                            float *w_out) {
+    VoxelGrid g;
+    g.w = w; g.h = h; g.d = d;
+    g.wmin = v3(wmin[0], wmin[1], wmin[2]);
+    g.wmax = v3(wmax[0], wmax[1], wmax[2]);
+    g.sh4.assign(sh4, sh4 + (size_t)w * h * d * 16);
+    V3x4<float> p;
+    for (int i = 0; i < 4; i++) { p.x[i] = pos[3 * i]; p.y[i] = pos[3 * i + 1]; p.z[i] = pos[3 * i + 2]; }
+    V3x4<float> rgb[4];
+    V4<float> wo[4];
+    voxel_lookup(g, p, rgb, wo);
+    for (int c = 0; c < 4; c++)
+        for (int l = 0; l < 4; l++) {
+            rgb_out[(c * 3 + 0) * 4 + l] = rgb[c].x[l];
+            rgb_out[(c * 3 + 1) * 4 + l] = rgb[c].y[l];
+            rgb_out[(c * 3 + 2) * 4 + l] = rgb[c].z[l];
+            w_out[c * 4 + l] = wo[c][l];
+        }
+    return 0;
+}
+
+// cubemap_uv_from_normal for four directions.
+__attribute__((visibility("default"))) int orc_cubemap_uv(const float n[12], float uv[8], uint32_t face[4]) {
+    V3x4<float> v;
+    for (int i = 0; i < 4; i++) { v.x[i] = n[3 * i]; v.y[i] = n[3 * i + 1]; v.z[i] = n[3 * i + 2]; }
+    V4<float> u, vv;
+    U4 s;
+    cubemap_uv_from_normal(v, u, vv, s);
+    for (int i = 0; i < 4; i++) { uv[2 * i] = u[i]; uv[2 * i + 1] = vv[i]; face[i] = s[i]; }
+    return 0;
+}
+
 }  // extern "C"

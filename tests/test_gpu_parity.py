@@ -160,3 +160,35 @@ def test_batch_matches_plain(sr, scenes, c1):
         ids, depth = b.read_vbuffer(i)
         assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"]))
         assert np.array_equal(b.read_pixels(i), p["px"])
+
+
+def test_gpu_matches_committed_c1_frame(sr, scenes, env):
+    """The GPU reproduces the committed oracle frame (tests/golden/) without
+    running the oracle."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_160x90.npz"))
+    scene, spec = scenes.make_c1(width=160, height=90)
+    scene.prepare(env, cache_key=scene.name)
+    out = render_gpu(sr, sr.Scene(scene), scenes.make_camera(spec), 160, 90)
+    assert np.array_equal(out["ids"], g["ids"])
+    assert np.array_equal(bits(out["depth"]), bits(g["depth"]))
+    assert np.array_equal(bits(out["lum"]), bits(g["tile_luminance"]))
+    assert np.array_equal(out["px"], g["pixels"])
+
+
+def test_fp32_shading_error_vs_fp64(sr, orc, scenes, c1):
+    """FP32 shading (GPU) against the FP64 restatement of the shader on the
+    same visibility buffer; the error is reported (nearest-texel and mip
+    choices can flip at boundaries, so the bound is loose)."""
+    scene, spec, gscene = c1
+    cam = scenes.make_camera(spec, 640, 360)
+    g = render_gpu(sr, gscene, cam, 640, 360)
+    o = orc.OracleRenderer(640, 360)
+    o.render_scene(orc.OracleScene(scene), cam, threads=8, fp64=True)
+    f = o.read_frame(fp64=True)
+    assert np.array_equal(g["ids"], f["ids"])
+    err = np.abs(g["rgb"].astype(np.float64) - f["rgb64"])
+    covered = g["ids"] != sr.NO_ID
+    print(f"[fp64] max abs error {err.max():.3e}, covered-pixel p99.9 {np.percentile(err[covered], 99.9):.3e}, "
+          f"mean {err[covered].mean():.3e}")
+    assert np.percentile(err[covered], 99.0) < 1e-3
