@@ -786,13 +786,45 @@ __device__ __forceinline__ bool block_rejected(const TriSetup &S, int bx, int by
 // Edge value of edge e at lane (sx, sy) of the quad i steps right and j steps
 // down from the walk origin (bx, by): origin evaluation, then j serial
 // +step_y, then i serial +step_x (src/tilerasterizer.rs:307-381).
+__device__ __forceinline__ float chain_serial(float v, float sy, float sx, int j, int i) {
+    for (int k = 0; k < j; k++) v = v + sy;
+    for (int k = 0; k < i; k++) v = v + sx;
+    return v;
+}
+
+// Trailing zero bits of an integer-valued float (64 for zero).
+__device__ __forceinline__ int ctz_intf(float x) {
+    uint32_t b = __float_as_uint(x) & 0x7FFFFFFFu;
+    if (b == 0u) return 64;
+    return (int)(b >> 23) - 150 + __builtin_ctz((b & 0x7FFFFFu) | 0x800000u);
+}
+
+// The serial chain v + sy (j times) + sx (i times) in O(1).  v, sy and sx are
+// integers.  If 2^t divides all three and every partial sum stays below
+// 2^(24+t) in magnitude, every partial sum is representable, each addition is
+// exact, and the chain equals v + j*sy + i*sx -- which two fused multiply-adds
+// return exactly (one rounding of a representable value).  The partial sums
+// lie on the segments v -> v+j*sy -> end, so bounding those three points
+// bounds them all; rounding is monotone, so testing the rounded points is
+// equivalent.  Clears ok when the shortcut does not apply.
+__device__ __forceinline__ float chain_fast(float v, float sy, float sx, float fj, float fi, bool &ok) {
+    float r1 = fmaf(fj, sy, v);
+    float r2 = fmaf(fi, sx, r1);
+    float m = fmaxf(fmaxf(fabsf(v), fabsf(r1)), fabsf(r2));
+    if (!(m < 16777216.0f)) {
+        int t = min(min(ctz_intf(v), min(ctz_intf(sy), ctz_intf(sx))), 100);
+        ok = ok && (m < __uint_as_float((uint32_t)(151 + t) << 23));
+    }
+    return r2;
+}
+
 __device__ __forceinline__ float edge_chain(const TriSetup &S, int e, int bx, int by, int sx, int sy, int i, int j) {
     float X = (float)(bx * 16 + 8 + 16 * sx);
     float Y = (float)(by * 16 + 8 + 16 * sy);
     float v = (S.A[e] * X + S.B[e] * Y) + S.C[e];
-    for (int k = 0; k < j; k++) v = v + S.SY[e];
-    for (int k = 0; k < i; k++) v = v + S.SX[e];
-    return v;
+    bool ok = true;
+    float r = chain_fast(v, S.SY[e], S.SX[e], (float)j, (float)i, ok);
+    return ok ? r : chain_serial(v, S.SY[e], S.SX[e], j, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,57 +1162,100 @@ __device__ __forceinline__ bool alpha_pass(const DevScene &S, const float4 *R, c
     return a >= M.alpha_cutoff;
 }
 
-#define SMALL_QUADS 4
+// Per-wave staging of one batch of 64 bin entries (structure of arrays, one
+// 64-wide row per field) from which the wave's lanes draw quad tasks.
+enum {
+    F_A = 0, F_B = 3, F_C = 6, F_SX = 9, F_SY = 12,
+    F_OOA = 15, F_OA, F_ODA, F_ODB, F_ZA, F_ZDA, F_ZDB,
+    F_ID, F_ORG, F_CMX, F_Q0, F_NQX, F_DIV, F_INCL, F_COUNT
+};
 
-__device__ __forceinline__ float rlf(float v, int L) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), L)); }
-__device__ __forceinline__ int rli(int v, int L) { return __builtin_amdgcn_readlane(v, L); }
-
-__device__ __forceinline__ void bcast_setup(const TriSetup &s, TriSetup &o, int L) {
-    for (int e = 0; e < 3; e++) {
-        o.A[e] = rlf(s.A[e], L); o.B[e] = rlf(s.B[e], L); o.C[e] = rlf(s.C[e], L);
-        o.SX[e] = rlf(s.SX[e], L); o.SY[e] = rlf(s.SY[e], L);
-    }
-    o.ooa = rlf(s.ooa, L); o.oa = rlf(s.oa, L); o.oda = rlf(s.oda, L); o.odb = rlf(s.odb, L);
-    o.za = rlf(s.za, L); o.zda = rlf(s.zda, L); o.zdb = rlf(s.zdb, L);
-    o.cmnx = rli(s.cmnx, L); o.cmny = rli(s.cmny, L); o.cmxx = rli(s.cmxx, L); o.cmxy = rli(s.cmxy, L);
-    o.ox0 = rli(s.ox0, L); o.oy0 = rli(s.oy0, L);
-    o.coarse = rli((int)s.coarse, L) != 0;
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Coverage and depth of one quad of one triangle: the reference's walk origin
-// and block test, then per-lane edge chains, barycentrics and z; covered
-// lanes with a non-NaN depth enter the 64-bit minimum.
+// Coverage and depth of the quad at (qpx, qpy) for the staged entry o: the
+// reference's 16px block choice and corner rejection, then the four samples'
+// edge chains (src/tilerasterizer.rs:220-381); covered samples with a non-NaN
+// depth enter the 64-bit minimum.
 template <int T>
-__device__ __forceinline__ void raster_quad(const DevScene &S, const RasterArgs &P, const TriSetup &ts, uint32_t gflags,
-                                            uint32_t id, int qpx, int qpy, int tx0, int ty0,
-                                            unsigned long long *keys) {
-    int bx, by;
-    if (!quad_origin(ts, qpx, qpy, bx, by)) return;
-    if (ts.coarse && block_rejected(ts, bx, by)) return;
-    int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
-    uint32_t inv_id = ~id;
-    for (int s = 0; s < 4; s++) {
-        int sx = s & 1, sy = s >> 1;
-        float w0 = edge_chain(ts, 0, bx, by, sx, sy, i, j);
-        float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
-        float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
-        if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)) continue;
-        float b1 = w1 * ts.ooa, b2 = w2 * ts.ooa;
-        float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
-        float z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
-        if (z != z) continue;
-        if (gflags & SR_GEO_ALPHA) {
-            if (!alpha_pass(S, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id, b1, b2, w)) continue;
-        }
-        unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)inv_id;
-        atomicMin(&keys[(qpy - ty0 + sy) * T + (qpx - tx0) + sx], key);
+__device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P, const uint32_t *st, int o, int qpx,
+                                          int qpy, int tx0, int ty0, unsigned long long *keys) {
+#define SF(f) __uint_as_float(st[(f) * 64 + o])
+    uint32_t org = st[F_ORG * 64 + o], nf = st[F_NQX * 64 + o];
+    int ox0 = (int)(org & 0xFFFFu), oy0 = (int)(org >> 16);
+    bool coarse = (nf & 0x10000u) != 0;
+    int bx = ox0, by = oy0;
+    if (coarse) {
+        bx = ox0 + ((qpx - ox0) & ~15);
+        by = oy0 + ((qpy - oy0) & ~15);
     }
+    float A[3], B[3], Cc[3];
+#pragma unroll
+    for (int e = 0; e < 3; e++) { A[e] = SF(F_A + e); B[e] = SF(F_B + e); Cc[e] = SF(F_C + e); }
+    if (coarse) {
+        float cx0 = (float)(bx * 16 + 8), cx1 = (float)(bx * 16 + 256 - 8);
+        float cy0 = (float)(by * 16 + 8), cy1 = (float)(by * 16 + 256 - 8);
+        bool out = false;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            float w00 = (A[e] * cx0 + B[e] * cy0) + Cc[e];
+            float w10 = (A[e] * cx1 + B[e] * cy0) + Cc[e];
+            float w01 = (A[e] * cx0 + B[e] * cy1) + Cc[e];
+            float w11 = (A[e] * cx1 + B[e] * cy1) + Cc[e];
+            out = out || (maxps(maxps(w00, w10), maxps(w01, w11)) < 0.0f);
+        }
+        if (out) return;
+    }
+    int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
+    float fi = (float)i, fj = (float)j;
+    float X0 = (float)(bx * 16 + 8), X1 = (float)(bx * 16 + 24);
+    float Y0 = (float)(by * 16 + 8), Y1 = (float)(by * 16 + 24);
+    float v[3][4], w[3][4];
+    bool ok = true;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        float sx = SF(F_SX + e), sy = SF(F_SY + e);
+        float ax0 = A[e] * X0, ax1 = A[e] * X1, by0 = B[e] * Y0, by1 = B[e] * Y1;
+        v[e][0] = (ax0 + by0) + Cc[e];
+        v[e][1] = (ax1 + by0) + Cc[e];
+        v[e][2] = (ax0 + by1) + Cc[e];
+        v[e][3] = (ax1 + by1) + Cc[e];
+#pragma unroll
+        for (int s = 0; s < 4; s++) w[e][s] = chain_fast(v[e][s], sy, sx, fj, fi, ok);
+    }
+    if (!ok) {
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            float sx = SF(F_SX + e), sy = SF(F_SY + e);
+#pragma unroll
+            for (int s = 0; s < 4; s++) w[e][s] = chain_serial(v[e][s], sy, sx, j, i);
+        }
+    }
+    uint32_t id = st[F_ID * 64 + o];
+    bool alpha = (nf & 0x20000u) != 0;
+    float ooa = SF(F_OOA), oa = SF(F_OA), oda = SF(F_ODA), odb = SF(F_ODB);
+    float za = SF(F_ZA), zda = SF(F_ZDA), zdb = SF(F_ZDB);
+    unsigned long long *row = keys + (qpy - ty0) * T + (qpx - tx0);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        if (!(w[0][s] >= 0.0f && w[1][s] >= 0.0f && w[2][s] >= 0.0f)) continue;
+        float b1 = w[1][s] * ooa, b2 = w[2][s] * ooa;
+        float wv = 1.0f / ((oa + b1 * oda) + b2 * odb);
+        float z = ((za + b1 * zda) + b2 * zdb) * wv;
+        if (z != z) continue;
+        if (alpha && !alpha_pass(S, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id, b1, b2, wv)) continue;
+        unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)(~id);
+        atomicMin(&row[(s >> 1) * T + (s & 1)], key);
+    }
+#undef SF
 }
 
 template <int T, int NT>
-__device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned long long *keys) {
+__device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned long long *keys, uint32_t *stage) {
     constexpr int QW = T / 2;                 // quads per row
-    constexpr int NQ = QW * QW;               // quads per tile
     constexpr int NW = NT / 64;               // waves
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
@@ -1203,52 +1278,77 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
     __syncthreads();
 
     // ---- phase 1: coverage and depth ----
-    // Each wave takes 64 bin entries at a time (one record load per lane, all
-    // in flight together).  Triangles touching at most SMALL_QUADS quads of
-    // this tile are walked by their own lane; larger ones are broadcast and
-    // walked by the whole wave with one quad per lane.
+    // Each wave takes 64 bin entries at a time: every lane loads and sets up
+    // one triangle, stages it in LDS, and counts the quads of its bounding
+    // box inside this tile.  A wave prefix sum turns the counts into a dense
+    // list of (entry, quad) tasks that the wave then walks 64 at a time, so
+    // small and large triangles alike keep every lane busy.
+    uint32_t *st = stage + wave * (F_COUNT * 64);
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
     if (end > P.pair_cap) end = P.pair_cap;
     for (uint32_t c0 = start + (uint32_t)wave * 64u; c0 < end; c0 += (uint32_t)NW * 64u) {
         uint32_t e = c0 + (uint32_t)lane;
-        bool valid = e < end;
-        uint32_t id = valid ? P.list[e] : 0u;
-        TriSetup ts;
-        uint4 g = make_uint4(0u, 0u, 0u, 0u);
-        int nqx = 0, nqy = 0, qx0 = 0, qy0 = 0;
-        if (valid) {
-            g = P.geo[id];
+        int nq = 0;
+        if (e < end) {
+            uint32_t id = P.list[e];
+            uint4 g = P.geo[id];
+            TriSetup ts;
             tri_setup(g, P.raster + 4 * (size_t)id, tx0, ty0, ts);
-            qx0 = max(ts.ox0, tx0);
-            qy0 = max(ts.oy0, ty0);
+            int qx0 = max(ts.ox0, tx0), qy0 = max(ts.oy0, ty0);
             int qx1 = min(ts.cmxx, tx0 + T), qy1 = min(ts.cmxy, ty0 + T);
-            nqx = qx1 > qx0 ? (qx1 - qx0 + 1) >> 1 : 0;
-            nqy = qy1 > qy0 ? (qy1 - qy0 + 1) >> 1 : 0;
-        } else {
-            memset(&ts, 0, sizeof(ts));
+            int nqx = qx1 > qx0 ? (qx1 - qx0 + 1) >> 1 : 0;
+            int nqy = qy1 > qy0 ? (qy1 - qy0 + 1) >> 1 : 0;
+            nq = nqx * nqy;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                st[(F_A + k) * 64 + lane] = __float_as_uint(ts.A[k]);
+                st[(F_B + k) * 64 + lane] = __float_as_uint(ts.B[k]);
+                st[(F_C + k) * 64 + lane] = __float_as_uint(ts.C[k]);
+                st[(F_SX + k) * 64 + lane] = __float_as_uint(ts.SX[k]);
+                st[(F_SY + k) * 64 + lane] = __float_as_uint(ts.SY[k]);
+            }
+            st[F_OOA * 64 + lane] = __float_as_uint(ts.ooa);
+            st[F_OA * 64 + lane] = __float_as_uint(ts.oa);
+            st[F_ODA * 64 + lane] = __float_as_uint(ts.oda);
+            st[F_ODB * 64 + lane] = __float_as_uint(ts.odb);
+            st[F_ZA * 64 + lane] = __float_as_uint(ts.za);
+            st[F_ZDA * 64 + lane] = __float_as_uint(ts.zda);
+            st[F_ZDB * 64 + lane] = __float_as_uint(ts.zdb);
+            st[F_ID * 64 + lane] = id;
+            st[F_ORG * 64 + lane] = (uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16);
+            st[F_Q0 * 64 + lane] = (uint32_t)qx0 | ((uint32_t)qy0 << 16);
+            st[F_NQX * 64 + lane] = (uint32_t)nqx | (ts.coarse ? 0x10000u : 0u) | ((g.z & SR_GEO_ALPHA) ? 0x20000u : 0u);
+            // k / nqx == (k * DIV) >> 16 for k < 256, nqx <= 16
+            st[F_DIV * 64 + lane] = nqx ? (65535u + (uint32_t)nqx) / (uint32_t)nqx : 0u;
         }
-        int nq = nqx * nqy;
-        bool small = nq <= SMALL_QUADS;
-        if (valid && small) {
-            for (int k = 0; k < nq; k++) {
-                int qpx = qx0 + 2 * (k % nqx), qpy = qy0 + 2 * (k / nqx);
-                raster_quad<T>(S, P, ts, g.z, id, qpx, qpy, tx0, ty0, keys);
+        int incl = nq;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        st[F_INCL * 64 + lane] = (uint32_t)incl;
+        int total = __builtin_amdgcn_readlane(incl, 63);
+        wave_lds_sync();
+        for (int k0 = 0; k0 < total; k0 += 64) {
+            int k = k0 + lane;
+            if (k < total) {
+                // owner = number of entries whose inclusive count is <= k
+                int o = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1)
+                    if ((int)st[F_INCL * 64 + o + step - 1] <= k) o += step;
+                int before = o ? (int)st[F_INCL * 64 + o - 1] : 0;
+                uint32_t local = (uint32_t)(k - before);
+                uint32_t q0 = st[F_Q0 * 64 + o];
+                uint32_t row = (local * st[F_DIV * 64 + o]) >> 16;
+                uint32_t col = local - row * (st[F_NQX * 64 + o] & 0xFFFFu);
+                int qpx = (int)(q0 & 0xFFFFu) + 2 * (int)col, qpy = (int)(q0 >> 16) + 2 * (int)row;
+                walk_quad<T>(S, P, st, o, qpx, qpy, tx0, ty0, keys);
             }
         }
-        uint64_t big = __ballot(valid && !small);
-        while (big) {
-            int L = __ffsll((unsigned long long)big) - 1;
-            big &= big - 1;
-            TriSetup bs;
-            bcast_setup(ts, bs, L);
-            uint32_t bflags = (uint32_t)__builtin_amdgcn_readlane((int)g.z, L);
-            uint32_t bid = (uint32_t)__builtin_amdgcn_readlane((int)id, L);
-            for (int q = lane; q < NQ; q += 64) {
-                int qx = q % QW, qy = q / QW;
-                raster_quad<T>(S, P, bs, bflags, bid, tx0 + 2 * qx, ty0 + 2 * qy, tx0, ty0, keys);
-            }
-        }
+        wave_lds_sync();
     }
     __syncthreads();
 
@@ -1330,8 +1430,9 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
 template <int T, int NT>
 __global__ void __launch_bounds__(NT) k_raster_shade(RasterArgs P) {
     __shared__ unsigned long long keys[T * T];
+    __shared__ uint32_t stage[(NT / 64) * F_COUNT * 64];
     if (!P.persistent) {
-        raster_shade_item<T, NT>(P, blockIdx.x, keys);
+        raster_shade_item<T, NT>(P, blockIdx.x, keys, stage);
         return;
     }
     __shared__ uint32_t next;
@@ -1341,7 +1442,7 @@ __global__ void __launch_bounds__(NT) k_raster_shade(RasterArgs P) {
         uint32_t item = next;
         __syncthreads();
         if (item >= P.total_items) break;
-        raster_shade_item<T, NT>(P, item, keys);
+        raster_shade_item<T, NT>(P, item, keys, stage);
         __syncthreads();
     }
 }
