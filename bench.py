@@ -90,7 +90,7 @@ def main():
     hip.hipDeviceSynchronize()
     lat = []
     kernel_ms = []
-    stage = {"ms_setup": 0.0, "ms_bin": 0.0, "ms_raster_shade": 0.0}
+    stage = {"ms_setup": 0.0, "ms_bin": 0.0, "ms_raster": 0.0, "ms_shade": 0.0, "ms_raster_shade": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         t = time.perf_counter()

@@ -204,6 +204,8 @@ typedef struct sr_frame_stats {
     float ms_raster_shade;         /* per-tile raster + vbuffer shading */
     float ms_blit;
     float ms_total;
+    float ms_raster;               /* coverage, depth and visibility buffer */
+    float ms_shade;                /* visibility-buffer shading */
 } sr_frame_stats;
 
 /* ---- batched views and render server ------------------------------------ */

@@ -118,7 +118,8 @@ class sr_frame_stats(C.Structure):
     _fields_ = [("input_triangles", C.c_uint64), ("setup_triangles", C.c_uint64),
                 ("binned_pairs", C.c_uint64), ("tiles", C.c_uint32), ("draws", C.c_uint32),
                 ("ms_setup", C.c_float), ("ms_bin", C.c_float), ("ms_raster_shade", C.c_float),
-                ("ms_blit", C.c_float), ("ms_total", C.c_float)]
+                ("ms_blit", C.c_float), ("ms_total", C.c_float), ("ms_raster", C.c_float),
+                ("ms_shade", C.c_float)]
 
 
 class sr_view(C.Structure):

@@ -41,8 +41,9 @@ struct FrameArgs {
     uint64_t color_plane;
     uint32_t *out_rgba;           // fused blit target (views with flags & 1)
     float *out_lum;
+    uint32_t *center_ids;         // 4 per 64px tile (sum over views)
     int32_t persistent_blocks;    // 0 = one block per item
-    hipEvent_t events[4];         // optional stage markers: start, setup done, bins done, raster done
+    hipEvent_t events[5];         // optional stage markers: start, setup done, bins done, shade done, raster done
 };
 
 // Launch the whole per-frame sequence on a stream.  Returns hipSuccess or the

@@ -1140,6 +1140,7 @@ struct RasterArgs {
     uint64_t color_plane;
     uint32_t *out_rgba;
     float *out_lum;
+    uint32_t *center_ids;         // 4 winners per 64px tile: its centre quad, on screen or not
     DevStatus *status;
     int32_t persistent;
 };
@@ -1167,7 +1168,15 @@ __device__ __forceinline__ bool alpha_pass(const DevScene &S, const float4 *R, c
 enum {
     F_A = 0, F_B = 3, F_C = 6, F_SX = 9, F_SY = 12,
     F_OOA = 15, F_OA, F_ODA, F_ODB, F_ZA, F_ZDA, F_ZDB,
-    F_ID, F_ORG, F_CMX, F_Q0, F_NQX, F_DIV, F_INCL, F_COUNT
+    F_ID, F_ORG, F_Q0, F_FLAGS, F_INCL, F_ROWS
+};
+// After F_ROWS: inclusive quad counts per quad row (u16 pairs), then each
+// row's first quad column (4-bit nibbles).
+template <int T> struct Stage {
+    static constexpr int NR = T / 2;
+    static constexpr int CUM = F_ROWS;
+    static constexpr int LO = F_ROWS + NR / 2;
+    static constexpr int COUNT = LO + (NR + 7) / 8;
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1176,38 +1185,177 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Coverage and depth of the quad at (qpx, qpy) for the staged entry o: the
-// reference's 16px block choice and corner rejection, then the four samples'
-// edge chains (src/tilerasterizer.rs:220-381); covered samples with a non-NaN
-// depth enter the 64-bit minimum.
-template <int T>
-__device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P, const uint32_t *st, int o, int qpx,
-                                          int qpy, int tx0, int ty0, unsigned long long *keys) {
-#define SF(f) __uint_as_float(st[(f) * 64 + o])
-    uint32_t org = st[F_ORG * 64 + o], nf = st[F_NQX * 64 + o];
-    int ox0 = (int)(org & 0xFFFFu), oy0 = (int)(org >> 16);
-    bool coarse = (nf & 0x10000u) != 0;
-    int bx = ox0, by = oy0;
-    if (coarse) {
-        bx = ox0 + ((qpx - ox0) & ~15);
-        by = oy0 + ((qpy - oy0) & ~15);
-    }
-    float A[3], B[3], Cc[3];
+// f32 corner test of the 16px block at (bx, by) (src/tilerasterizer.rs:258-291)
+__device__ __forceinline__ bool block_out(const TriSetup &ts, int bx, int by) {
+    float cx0 = (float)(bx * 16 + 8), cx1 = (float)(bx * 16 + 256 - 8);
+    float cy0 = (float)(by * 16 + 8), cy1 = (float)(by * 16 + 256 - 8);
+    bool out = false;
 #pragma unroll
-    for (int e = 0; e < 3; e++) { A[e] = SF(F_A + e); B[e] = SF(F_B + e); Cc[e] = SF(F_C + e); }
-    if (coarse) {
-        float cx0 = (float)(bx * 16 + 8), cx1 = (float)(bx * 16 + 256 - 8);
-        float cy0 = (float)(by * 16 + 8), cy1 = (float)(by * 16 + 256 - 8);
-        bool out = false;
+    for (int e = 0; e < 3; e++) {
+        float w00 = (ts.A[e] * cx0 + ts.B[e] * cy0) + ts.C[e];
+        float w10 = (ts.A[e] * cx1 + ts.B[e] * cy0) + ts.C[e];
+        float w01 = (ts.A[e] * cx0 + ts.B[e] * cy1) + ts.C[e];
+        float w11 = (ts.A[e] * cx1 + ts.B[e] * cy1) + ts.C[e];
+        out = out || (maxps(maxps(w00, w10), maxps(w01, w11)) < 0.0f);
+    }
+    return out;
+}
+
+// Stages one bin entry and returns how many of its quads inside the tile
+// can hold a covered sample.
+//
+// Walked quads are the reference's: quad-aligned from (ox0, oy0) up to the
+// clipped bounding box, in 16px blocks whose corner test rejects them.  Rows
+// are then narrowed to conservative column spans.  Let E = A X + B Y + C be
+// the exact edge function of the f32 coefficients.  The reference's value
+// at a sample is E after at most 20 roundings (products and sums of the
+// origin evaluation, at most 16 chain additions), each off by at most 2^-24
+// of a magnitude bounded by P = |A| Xm + |B| Ym + |C| + 512 (|A| + |B|), so
+// it differs from E by less than M = 2^-19 P; a sample with E < -M is never
+// covered.  Per row, each edge with A != 0 bounds X on one side, taken at
+// the looser of the row's two sample lines; the bounds are evaluated in
+// double with a slack far above its rounding error.  The argument needs
+// SX = 32 A and SY = 32 B (no i32 wrap), else the full rows are kept.
+template <int T>
+__device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint32_t gflags, int tx0, int ty0,
+                                           uint32_t *st, int lane) {
+    using SG = Stage<T>;
+    constexpr int NR = SG::NR;
+    int qx0 = max(ts.ox0, tx0), qy0 = max(ts.oy0, ty0);
+    int qx1 = min(ts.cmxx, tx0 + T), qy1 = min(ts.cmxy, ty0 + T);
+    int nqx = qx1 > qx0 ? (qx1 - qx0 + 1) >> 1 : 0;
+    int nqy = qy1 > qy0 ? (qy1 - qy0 + 1) >> 1 : 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        st[(F_A + k) * 64 + lane] = __float_as_uint(ts.A[k]);
+        st[(F_B + k) * 64 + lane] = __float_as_uint(ts.B[k]);
+        st[(F_C + k) * 64 + lane] = __float_as_uint(ts.C[k]);
+        st[(F_SX + k) * 64 + lane] = __float_as_uint(ts.SX[k]);
+        st[(F_SY + k) * 64 + lane] = __float_as_uint(ts.SY[k]);
+    }
+    st[F_OOA * 64 + lane] = __float_as_uint(ts.ooa);
+    st[F_OA * 64 + lane] = __float_as_uint(ts.oa);
+    st[F_ODA * 64 + lane] = __float_as_uint(ts.oda);
+    st[F_ODB * 64 + lane] = __float_as_uint(ts.odb);
+    st[F_ZA * 64 + lane] = __float_as_uint(ts.za);
+    st[F_ZDA * 64 + lane] = __float_as_uint(ts.zda);
+    st[F_ZDB * 64 + lane] = __float_as_uint(ts.zdb);
+    st[F_ID * 64 + lane] = id;
+    st[F_ORG * 64 + lane] = (uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16);
+    st[F_Q0 * 64 + lane] = (uint32_t)qx0 | ((uint32_t)qy0 << 16);
+
+    // rejected 16px blocks among the (at most 3x3) that meet the tile
+    uint32_t flags = (ts.coarse ? 1u : 0u) | ((gflags & SR_GEO_ALPHA) ? 2u : 0u);
+    if (ts.coarse && nqx > 0 && nqy > 0) {
+        int kx0 = (qx0 - ts.ox0) >> 4, kx1 = (qx0 + 2 * (nqx - 1) - ts.ox0) >> 4;
+        int ky0 = (qy0 - ts.oy0) >> 4, ky1 = (qy0 + 2 * (nqy - 1) - ts.oy0) >> 4;
+        uint32_t mask = 0u;
+        for (int ky = ky0; ky <= ky1; ky++)
+            for (int kx = kx0; kx <= kx1; kx++)
+                if (block_out(ts, ts.ox0 + 16 * kx, ts.oy0 + 16 * ky)) mask |= 1u << ((ky - ky0) * 3 + (kx - kx0));
+        flags |= (mask << 8) | ((uint32_t)kx0 << 20) | ((uint32_t)ky0 << 24);
+    }
+    st[F_FLAGS * 64 + lane] = flags;
+
+    bool spans = nqx * nqy > 4;
+#pragma unroll
+    for (int e = 0; e < 3; e++) spans = spans && fabsf(ts.A[e]) < 67108864.0f && fabsf(ts.B[e]) < 67108864.0f;
+    double base[3], slope[3], slack[3];
+    int kind[3];
+    if (spans) {
+        double Xm = (double)((tx0 + T) * 16 + 24), Ym = (double)((ty0 + T) * 16 + 24);
 #pragma unroll
         for (int e = 0; e < 3; e++) {
-            float w00 = (A[e] * cx0 + B[e] * cy0) + Cc[e];
-            float w10 = (A[e] * cx1 + B[e] * cy0) + Cc[e];
-            float w01 = (A[e] * cx0 + B[e] * cy1) + Cc[e];
-            float w11 = (A[e] * cx1 + B[e] * cy1) + Cc[e];
-            out = out || (maxps(maxps(w00, w10), maxps(w01, w11)) < 0.0f);
+            double A = ts.A[e], B = ts.B[e], C = ts.C[e];
+            double M = 0x1p-19 * (fabs(A) * Xm + fabs(B) * Ym + fabs(C) + 512.0 * (fabs(A) + fabs(B)));
+            kind[e] = A > 0.0 ? 1 : (A < 0.0 ? -1 : 0);
+            if (kind[e] != 0) {
+                // E >= -M  <=>  A X >= -(M + C + B Y)
+                double inv = 1.0 / A;
+                base[e] = -(M + C) * inv;
+                slope[e] = -B * inv;
+            } else {
+                base[e] = M + C;      // feasible where B Y + C + M >= 0
+                slope[e] = B;
+            }
+            slack[e] = 16.0 + 0x1p-40 * (fabs(base[e]) + fabs(slope[e]) * Ym);
         }
-        if (out) return;
+    }
+    int cum = 0;
+    uint32_t cw = 0u, lw = 0u;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        int l = 0, n = 0;
+        if (r < nqy) {
+            if (!spans) {
+                n = nqx;
+            } else {
+                double Y0 = (double)((qy0 + 2 * r) * 16 + 8), Y1 = Y0 + 16.0;
+                double xlo = -1.0e300, xhi = 1.0e300;
+                bool feas = true;
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    double b0 = fma(slope[e], Y0, base[e]), b1 = fma(slope[e], Y1, base[e]);
+                    if (kind[e] > 0) xlo = fmax(xlo, fmin(b0, b1) - slack[e]);
+                    else if (kind[e] < 0) xhi = fmin(xhi, fmax(b0, b1) + slack[e]);
+                    else feas = feas && fmax(b0, b1) >= -slack[e];
+                }
+                // quad column c has samples at X = (qx0 + 2c) * 16 + 8 and + 24
+                double clo = fmax(ceil((xlo - 24.0 - 16.0 * qx0) * (1.0 / 32.0)), 0.0);
+                double chi = fmin(floor((xhi - 8.0 - 16.0 * qx0) * (1.0 / 32.0)), (double)(nqx - 1));
+                if (feas && clo <= chi) {
+                    l = (int)clo;
+                    n = (int)chi - l + 1;
+                }
+            }
+        }
+        cum += n;
+        cw |= (uint32_t)cum << (16 * (r & 1));
+        lw |= (uint32_t)l << (4 * (r & 7));
+        if (r & 1) {
+            st[(SG::CUM + r / 2) * 64 + lane] = cw;
+            cw = 0u;
+        }
+        if ((r & 7) == 7 || r == NR - 1) {
+            st[(SG::LO + r / 8) * 64 + lane] = lw;
+            lw = 0u;
+        }
+    }
+    return cum;
+}
+
+// Coverage and depth of the local-th candidate quad of staged entry o: the
+// reference's block choice and rejection, then the four samples' edge
+// chains (src/tilerasterizer.rs:220-381); covered samples with a non-NaN
+// depth enter the 64-bit minimum.
+template <int T>
+__device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P, const uint32_t *st, int o, int local,
+                                          int tx0, int ty0, unsigned long long *keys) {
+    using SG = Stage<T>;
+    constexpr int NR = SG::NR;
+#define SF(f) __uint_as_float(st[(f) * 64 + o])
+    // row = number of rows whose inclusive count is <= local
+    int row = 0;
+#pragma unroll
+    for (int r = 0; r < NR; r += 2) {
+        uint32_t cw = st[(SG::CUM + r / 2) * 64 + o];
+        row += ((int)(cw & 0xFFFFu) <= local) + ((int)(cw >> 16) <= local);
+    }
+    int before = 0;
+    if (row > 0) before = (int)((st[(SG::CUM + (row - 1) / 2) * 64 + o] >> (16 * ((row - 1) & 1))) & 0xFFFFu);
+    int col = (int)((st[(SG::LO + row / 8) * 64 + o] >> (4 * (row & 7))) & 15u) + (local - before);
+    uint32_t q0 = st[F_Q0 * 64 + o];
+    int qpx = (int)(q0 & 0xFFFFu) + 2 * col, qpy = (int)(q0 >> 16) + 2 * row;
+
+    uint32_t org = st[F_ORG * 64 + o], flags = st[F_FLAGS * 64 + o];
+    int ox0 = (int)(org & 0xFFFFu), oy0 = (int)(org >> 16);
+    int bx = ox0, by = oy0;
+    if (flags & 1u) {
+        int kx = (qpx - ox0) >> 4, ky = (qpy - oy0) >> 4;
+        int bit = (ky - (int)((flags >> 24) & 15u)) * 3 + (kx - (int)((flags >> 20) & 15u));
+        if ((flags >> (8 + bit)) & 1u) return;
+        bx = ox0 + 16 * kx;
+        by = oy0 + 16 * ky;
     }
     int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
     float fi = (float)i, fj = (float)j;
@@ -1217,12 +1365,13 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
     bool ok = true;
 #pragma unroll
     for (int e = 0; e < 3; e++) {
+        float A = SF(F_A + e), B = SF(F_B + e), Cc = SF(F_C + e);
         float sx = SF(F_SX + e), sy = SF(F_SY + e);
-        float ax0 = A[e] * X0, ax1 = A[e] * X1, by0 = B[e] * Y0, by1 = B[e] * Y1;
-        v[e][0] = (ax0 + by0) + Cc[e];
-        v[e][1] = (ax1 + by0) + Cc[e];
-        v[e][2] = (ax0 + by1) + Cc[e];
-        v[e][3] = (ax1 + by1) + Cc[e];
+        float ax0 = A * X0, ax1 = A * X1, by0 = B * Y0, by1 = B * Y1;
+        v[e][0] = (ax0 + by0) + Cc;
+        v[e][1] = (ax1 + by0) + Cc;
+        v[e][2] = (ax0 + by1) + Cc;
+        v[e][3] = (ax1 + by1) + Cc;
 #pragma unroll
         for (int s = 0; s < 4; s++) w[e][s] = chain_fast(v[e][s], sy, sx, fj, fi, ok);
     }
@@ -1235,10 +1384,10 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
         }
     }
     uint32_t id = st[F_ID * 64 + o];
-    bool alpha = (nf & 0x20000u) != 0;
+    bool alpha = (flags & 2u) != 0;
     float ooa = SF(F_OOA), oa = SF(F_OA), oda = SF(F_ODA), odb = SF(F_ODB);
     float za = SF(F_ZA), zda = SF(F_ZDA), zdb = SF(F_ZDB);
-    unsigned long long *row = keys + (qpy - ty0) * T + (qpx - tx0);
+    unsigned long long *kq = keys + (qpy - ty0) * T + (qpx - tx0);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         if (!(w[0][s] >= 0.0f && w[1][s] >= 0.0f && w[2][s] >= 0.0f)) continue;
@@ -1248,28 +1397,35 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
         if (z != z) continue;
         if (alpha && !alpha_pass(S, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id, b1, b2, wv)) continue;
         unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)(~id);
-        atomicMin(&row[(s >> 1) * T + (s & 1)], key);
+        atomicMin(&kq[(s >> 1) * T + (s & 1)], key);
     }
 #undef SF
 }
 
+__device__ __forceinline__ int find_view(const RasterArgs &P, uint32_t item) {
+    int lo = 0, hi = P.num_views - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (P.views[mid].item_base <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// The 2x2 quad at the origin of a tile whose origin is (64k, 64m + 32) is the
+// centre quad of its 64px reference tile (src/tilerasterizer.rs:103-106).
+__device__ __forceinline__ bool centre_tile(int tx0, int ty0) { return ((tx0 & 63) == 0) && ((ty0 & 63) == 32); }
+
+// ---------------------------------------------------------------------------
+// k_raster: one workgroup per (view, tile): coverage and depth of every bin
+// entry into an LDS depth buffer, then the visibility buffer (triangle id per
+// pixel, or none).
+// ---------------------------------------------------------------------------
 template <int T, int NT>
-__device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned long long *keys, uint32_t *stage) {
-    constexpr int QW = T / 2;                 // quads per row
+__device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long long *keys, uint32_t *stage) {
     constexpr int NW = NT / 64;               // waves
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    int v = 0;
-    {
-        int lo = 0, hi = P.num_views - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (P.views[mid].item_base <= item) lo = mid;
-            else hi = mid - 1;
-        }
-        v = lo;
-    }
-    const DevView &V = P.views[v];
+    const DevView &V = P.views[find_view(P, item)];
     const DevScene &S = *V.scene;
     uint32_t tile = item - V.item_base;
     int tx0 = (int)(tile % (uint32_t)V.tiles_x) * T, ty0 = (int)(tile / (uint32_t)V.tiles_x) * T;
@@ -1283,7 +1439,7 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
     // box inside this tile.  A wave prefix sum turns the counts into a dense
     // list of (entry, quad) tasks that the wave then walks 64 at a time, so
     // small and large triangles alike keep every lane busy.
-    uint32_t *st = stage + wave * (F_COUNT * 64);
+    uint32_t *st = stage + wave * (Stage<T>::COUNT * 64);
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
     if (end > P.pair_cap) end = P.pair_cap;
@@ -1295,32 +1451,7 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
             uint4 g = P.geo[id];
             TriSetup ts;
             tri_setup(g, P.raster + 4 * (size_t)id, tx0, ty0, ts);
-            int qx0 = max(ts.ox0, tx0), qy0 = max(ts.oy0, ty0);
-            int qx1 = min(ts.cmxx, tx0 + T), qy1 = min(ts.cmxy, ty0 + T);
-            int nqx = qx1 > qx0 ? (qx1 - qx0 + 1) >> 1 : 0;
-            int nqy = qy1 > qy0 ? (qy1 - qy0 + 1) >> 1 : 0;
-            nq = nqx * nqy;
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                st[(F_A + k) * 64 + lane] = __float_as_uint(ts.A[k]);
-                st[(F_B + k) * 64 + lane] = __float_as_uint(ts.B[k]);
-                st[(F_C + k) * 64 + lane] = __float_as_uint(ts.C[k]);
-                st[(F_SX + k) * 64 + lane] = __float_as_uint(ts.SX[k]);
-                st[(F_SY + k) * 64 + lane] = __float_as_uint(ts.SY[k]);
-            }
-            st[F_OOA * 64 + lane] = __float_as_uint(ts.ooa);
-            st[F_OA * 64 + lane] = __float_as_uint(ts.oa);
-            st[F_ODA * 64 + lane] = __float_as_uint(ts.oda);
-            st[F_ODB * 64 + lane] = __float_as_uint(ts.odb);
-            st[F_ZA * 64 + lane] = __float_as_uint(ts.za);
-            st[F_ZDA * 64 + lane] = __float_as_uint(ts.zda);
-            st[F_ZDB * 64 + lane] = __float_as_uint(ts.zdb);
-            st[F_ID * 64 + lane] = id;
-            st[F_ORG * 64 + lane] = (uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16);
-            st[F_Q0 * 64 + lane] = (uint32_t)qx0 | ((uint32_t)qy0 << 16);
-            st[F_NQX * 64 + lane] = (uint32_t)nqx | (ts.coarse ? 0x10000u : 0u) | ((g.z & SR_GEO_ALPHA) ? 0x20000u : 0u);
-            // k / nqx == (k * DIV) >> 16 for k < 256, nqx <= 16
-            st[F_DIV * 64 + lane] = nqx ? (65535u + (uint32_t)nqx) / (uint32_t)nqx : 0u;
+            nq = stage_entry<T>(ts, id, g.z, tx0, ty0, st, lane);
         }
         int incl = nq;
 #pragma unroll
@@ -1340,19 +1471,62 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
                 for (int step = 32; step > 0; step >>= 1)
                     if ((int)st[F_INCL * 64 + o + step - 1] <= k) o += step;
                 int before = o ? (int)st[F_INCL * 64 + o - 1] : 0;
-                uint32_t local = (uint32_t)(k - before);
-                uint32_t q0 = st[F_Q0 * 64 + o];
-                uint32_t row = (local * st[F_DIV * 64 + o]) >> 16;
-                uint32_t col = local - row * (st[F_NQX * 64 + o] & 0xFFFFu);
-                int qpx = (int)(q0 & 0xFFFFu) + 2 * (int)col, qpy = (int)(q0 >> 16) + 2 * (int)row;
-                walk_quad<T>(S, P, st, o, qpx, qpy, tx0, ty0, keys);
+                walk_quad<T>(S, P, st, o, k - before, tx0, ty0, keys);
             }
         }
         wave_lds_sync();
     }
     __syncthreads();
 
-    // ---- phase 2: resolve, visibility buffer, shading ----
+    // ---- resolve: canonical triangle id per pixel ----
+    uint32_t rec0 = P.tri_base[V.in_tri_base];
+    bool centre = centre_tile(tx0, ty0);
+    for (int p = tid; p < T * T; p += NT) {
+        int px = tx0 + (p % T), py = ty0 + (p / T);
+        unsigned long long key = keys[p];
+        uint32_t out = key != SR_EMPTY_KEY ? ~(uint32_t)key - rec0 : 0xFFFFFFFFu;
+        if (px < V.W && py < V.H) P.out_ids[V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px] = out;
+        if (centre && px < tx0 + 2 && py < ty0 + 2) {
+            int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
+            P.center_ids[4 * (V.lum_offset + (uint32_t)ri) + (uint32_t)((py - ty0) * 2 + (px - tx0))] = out;
+        }
+    }
+}
+
+template <int T, int NT>
+__global__ void __launch_bounds__(NT) k_raster(RasterArgs P) {
+    __shared__ unsigned long long keys[T * T];
+    __shared__ uint32_t stage[(NT / 64) * Stage<T>::COUNT * 64];
+    if (!P.persistent) {
+        raster_item<T, NT>(P, blockIdx.x, keys, stage);
+        return;
+    }
+    __shared__ uint32_t next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(&P.status->work_counter, 1u);
+        __syncthreads();
+        uint32_t item = next;
+        __syncthreads();
+        if (item >= P.total_items) break;
+        raster_item<T, NT>(P, item, keys, stage);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_shade: one workgroup per (view, tile), one thread per pixel in 2x2-quad
+// order: barycentrics and depth of the visible triangle, PBR shading or
+// skybox, colour planes or the fused tonemapped RGBA8, and the centre-quad
+// luminance of each 64px tile (src/tilerasterizer.rs:384-476).
+// ---------------------------------------------------------------------------
+template <int T, int NT>
+__device__ void shade_item(const RasterArgs &P, uint32_t item) {
+    constexpr int QW = T / 2;                 // quads per row
+    const int tid = threadIdx.x, lane = tid & 63;
+    const DevView &V = P.views[find_view(P, item)];
+    const DevScene &S = *V.scene;
+    uint32_t tile = item - V.item_base;
+    int tx0 = (int)(tile % (uint32_t)V.tiles_x) * T, ty0 = (int)(tile / (uint32_t)V.tiles_x) * T;
     uint32_t rec0 = P.tri_base[V.in_tri_base];
     bool fused = (V.flags & 1u) != 0;
     for (int pb = 0; pb < T * T; pb += NT) {
@@ -1361,9 +1535,14 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
         int qx = q % QW, qy = q / QW;
         int sx = s & 1, sy = s >> 1;
         int px = tx0 + 2 * qx + sx, py = ty0 + 2 * qy + sy;
-        unsigned long long key = keys[(2 * qy + sy) * T + 2 * qx + sx];
-        bool has = key != SR_EMPTY_KEY;
-        uint32_t id = has ? ~(uint32_t)key : 0u;
+        bool on_screen = px < V.W && py < V.H;
+        bool center = centre_tile(tx0, ty0) && q == 0;
+        size_t o = V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px;
+        uint32_t vid = 0xFFFFFFFFu;
+        if (center) vid = P.center_ids[4 * (V.lum_offset + (uint32_t)((ty0 / 64) * V.ref_tiles_x + (tx0 / 64))) + s];
+        else if (on_screen) vid = P.out_ids[o];
+        bool has = vid != 0xFFFFFFFFu;
+        uint32_t id = has ? vid + rec0 : 0u;
         float b1 = 0.0f, b2 = 0.0f, z = INFINITY;
         if (has) {
             uint4 g = P.geo[id];
@@ -1386,8 +1565,6 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
             qb1[k] = __shfl(b1, qbase + k, 64);
             qb2[k] = __shfl(b2, qbase + k, 64);
         }
-        bool on_screen = px < V.W && py < V.H;
-        bool center = ((tx0 & 63) == 0) && ((ty0 & 63) == 32) && q == 0;
         f3 color = mk3(0.0f, 0.0f, 0.0f);
         if (on_screen || center) {
             bool covered = has && z != INFINITY;
@@ -1399,8 +1576,6 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
             }
         }
         if (on_screen) {
-            size_t o = V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px;
-            P.out_ids[o] = has ? id - rec0 : 0xFFFFFFFFu;
             P.out_depth[o] = z;
             if (fused) {
                 float k = 0.2f;
@@ -1428,23 +1603,8 @@ __device__ void raster_shade_item(const RasterArgs &P, uint32_t item, unsigned l
 }
 
 template <int T, int NT>
-__global__ void __launch_bounds__(NT) k_raster_shade(RasterArgs P) {
-    __shared__ unsigned long long keys[T * T];
-    __shared__ uint32_t stage[(NT / 64) * F_COUNT * 64];
-    if (!P.persistent) {
-        raster_shade_item<T, NT>(P, blockIdx.x, keys, stage);
-        return;
-    }
-    __shared__ uint32_t next;
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(&P.status->work_counter, 1u);
-        __syncthreads();
-        uint32_t item = next;
-        __syncthreads();
-        if (item >= P.total_items) break;
-        raster_shade_item<T, NT>(P, item, keys, stage);
-        __syncthreads();
-    }
+__global__ void __launch_bounds__(NT) k_shade(RasterArgs P) {
+    shade_item<T, NT>(P, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1510,12 +1670,19 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     P.geo = a.rec_geo; P.raster = a.rec_raster; P.shade = a.rec_shade; P.tan = a.any_tangents ? a.rec_tan : nullptr;
     P.tri_base = a.tri_base; P.out_ids = a.out_ids; P.out_depth = a.out_depth; P.out_color = a.out_color;
     P.color_plane = a.color_plane; P.out_rgba = a.out_rgba; P.out_lum = a.out_lum; P.status = a.status;
+    P.center_ids = a.center_ids;
     P.persistent = a.persistent_blocks > 0;
     uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : a.total_items;
     switch (a.tile_size) {
-        case 8: hipLaunchKernelGGL((k_raster_shade<8, 64>), dim3(grid), dim3(64), 0, st, P); break;
-        case 32: hipLaunchKernelGGL((k_raster_shade<32, 256>), dim3(grid), dim3(256), 0, st, P); break;
-        default: hipLaunchKernelGGL((k_raster_shade<16, 256>), dim3(grid), dim3(256), 0, st, P); break;
+        case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, st, P); break;
+        case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, st, P); break;
+    }
+    if (a.events[4]) hipEventRecord(a.events[4], st);
+    switch (a.tile_size) {
+        case 8: hipLaunchKernelGGL((k_shade<8, 64>), dim3(a.total_items), dim3(64), 0, st, P); break;
+        case 32: hipLaunchKernelGGL((k_shade<32, 256>), dim3(a.total_items), dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_shade<16, 256>), dim3(a.total_items), dim3(256), 0, st, P); break;
     }
     if (a.events[3]) hipEventRecord(a.events[3], st);
     return hipGetLastError();

@@ -125,6 +125,8 @@ SR_API int sr_renderer_render_scene(sr_renderer *r, const sr_scene *scene, const
     r->stats.ms_setup = ev_ms(r->pipe.ev[0], r->pipe.ev[1]);
     r->stats.ms_bin = ev_ms(r->pipe.ev[1], r->pipe.ev[2]);
     r->stats.ms_raster_shade = ev_ms(r->pipe.ev[2], r->pipe.ev[3]);
+    r->stats.ms_raster = ev_ms(r->pipe.ev[2], r->pipe.ev[4]);
+    r->stats.ms_shade = ev_ms(r->pipe.ev[4], r->pipe.ev[3]);
     r->stats.ms_total = ev_ms(r->pipe.ev[0], r->pipe.ev[3]);
     return SR_OK;
 }

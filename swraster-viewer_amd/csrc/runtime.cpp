@@ -204,7 +204,7 @@ void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     DevBuf *bufs[] = {&views, &draws, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
-                      &list, &rec_geo, &rec_raster, &rec_shade, &rec_tan, &status};
+                      &list, &rec_geo, &rec_raster, &rec_shade, &rec_tan, &status, &center};
     for (DevBuf *b : bufs) b->release();
     stage.release();
     status_host.release();
@@ -232,6 +232,10 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     if ((r = bin_count.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_start.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_end.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
+    size_t lum_slots = 1;
+    for (const DevView &v : hv)
+        lum_slots = std::max<size_t>(lum_slots, (size_t)v.lum_offset + (size_t)v.ref_tiles_x * v.ref_tiles_y);
+    if ((r = center.ensure(lum_slots * 16))) return r;
     size_t maxn = std::max<size_t>((size_t)total_in + 1, total_items);
     if ((r = partials.ensure(((maxn + 4095) / 4096 + 1) * 4))) return r;
     if (rec_cap < total_in + total_in / 8 + 1024) rec_cap = total_in + total_in / 8 + 1024;
@@ -279,8 +283,9 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.color_plane = color_plane;
         a.out_rgba = out_rgba;
         a.out_lum = out_lum;
+        a.center_ids = (uint32_t *)center.ptr;
         a.persistent_blocks = persistent_blocks;
-        for (int i = 0; i < 4; i++) a.events[i] = ev[i];
+        for (int i = 0; i < 5; i++) a.events[i] = ev[i];
         HIP_TRY(launch_frame(a, stream));
         HIP_TRY(hipMemcpyAsync(status_host.ptr, status.ptr, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
         if (!sync) return SR_OK;
