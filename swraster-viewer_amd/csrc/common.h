@@ -127,5 +127,7 @@ struct DevStatus {
     uint32_t overflow_records;
     uint32_t overflow_pairs;
     uint32_t work_counter;        // persistent-kernel queue head
-    uint32_t pad[3];
+    uint32_t clip_queued;         // triangles queued for clipping
+    uint32_t overflow_clip;
+    uint32_t clip_overflow;       // a clipped polygon outgrew its room (frame fails)
 };

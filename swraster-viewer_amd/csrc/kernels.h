@@ -33,6 +33,9 @@ struct FrameArgs {
     float4 *rec_shade;            // rec_cap * 8
     float4 *rec_tan;              // rec_cap * 3 (or null)
     uint32_t rec_cap, pair_cap;
+    uint32_t *clip_queue;         // clip_cap input triangles that cross a frustum plane
+    float *clip_buf;              // clip_cap * 16 vertices * 16 channels
+    uint32_t clip_cap;
     DevStatus *status;
     // outputs (per view at DevView.fb_offset)
     uint32_t *out_ids;

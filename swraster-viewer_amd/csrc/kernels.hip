@@ -159,36 +159,6 @@ __device__ __forceinline__ void lerp_cv(const CV &a, const CV &b, float t, CV &o
     o.u = a.u + (b.u - a.u) * t; o.v = a.v + (b.v - a.v) * t;
 }
 
-// Sutherland-Hodgman from plane `first` on (planes before it left the
-// triangle unchanged).  Returns the polygon size (0 or >= 3) in poly.
-__device__ __noinline__ int clip_general(CV *poly, CV *np, int first) {
-    int len = 3;
-    for (int p = first; p < 6; p++) {
-        if (len == 0) return 0;
-        int nl = 0;
-        for (int i = 0; i < len; i++) {
-            const CV &cur = poly[i];
-            const CV &prev = poly[(i + len - 1) % len];
-            float dc = plane_dot(p, cur.cx, cur.cy, cur.cz, cur.cw);
-            float dp = plane_dot(p, prev.cx, prev.cy, prev.cz, prev.cw);
-            bool ci = dc >= 0.0f, pi = dp >= 0.0f;
-            if (ci) {
-                if (!pi) {
-                    float t = dp / (dp - dc);
-                    lerp_cv(prev, cur, t, np[nl++]);
-                }
-                np[nl++] = cur;
-            } else if (pi) {
-                float t = dp / (dp - dc);
-                lerp_cv(prev, cur, t, np[nl++]);
-            }
-        }
-        for (int i = 0; i < nl; i++) poly[i] = np[i];
-        len = nl;
-    }
-    return len < 3 ? 0 : len;
-}
-
 // Position-only pre-pass: planes whose test leaves the triangle untouched are
 // skipped; returns -1 if the triangle passes all planes unchanged, -2 if it is
 // rejected whole by a plane, or the first plane that cuts it.
@@ -201,41 +171,6 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
         return p;
     }
     return -1;
-}
-
-__device__ __noinline__ int clip_count_general(const float c[3][4], int first) {
-    float poly[10][4], np[10][4];
-    for (int k = 0; k < 3; k++)
-        for (int j = 0; j < 4; j++) poly[k][j] = c[k][j];
-    int len = 3;
-    for (int p = first; p < 6; p++) {
-        if (len == 0) return 0;
-        int nl = 0;
-        for (int i = 0; i < len; i++) {
-            const float *cur = poly[i];
-            const float *prev = poly[(i + len - 1) % len];
-            float dc = plane_dot(p, cur[0], cur[1], cur[2], cur[3]);
-            float dp = plane_dot(p, prev[0], prev[1], prev[2], prev[3]);
-            bool ci = dc >= 0.0f, pi = dp >= 0.0f;
-            if (ci) {
-                if (!pi) {
-                    float t = dp / (dp - dc);
-                    for (int j = 0; j < 4; j++) np[nl][j] = prev[j] + (cur[j] - prev[j]) * t;
-                    nl++;
-                }
-                for (int j = 0; j < 4; j++) np[nl][j] = cur[j];
-                nl++;
-            } else if (pi) {
-                float t = dp / (dp - dc);
-                for (int j = 0; j < 4; j++) np[nl][j] = prev[j] + (cur[j] - prev[j]) * t;
-                nl++;
-            }
-        }
-        for (int i = 0; i < nl; i++)
-            for (int j = 0; j < 4; j++) poly[i][j] = np[i][j];
-        len = nl;
-    }
-    return len < 3 ? 0 : len - 2;
 }
 
 // ---------------------------------------------------------------------------
@@ -290,28 +225,138 @@ __global__ void __launch_bounds__(256) k_transform(const DevView *views, const D
     }
 }
 
+// Orders LDS writes of some lanes of a wave before reads by other lanes of it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
-// k_clip_count: fan triangles per input triangle
+// k_clip_count: fan triangles per input triangle.  Triangles inside the
+// frustum make one, rejected ones none; those that cross a plane are queued
+// for k_clip, which writes their count.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const DevDraw *draws, int ndraws,
-                                                    uint32_t total, const float4 *xv, uint32_t *counts) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+                                                    uint32_t total, const float4 *xv, uint32_t *counts,
+                                                    uint32_t *queue, uint32_t qcap, DevStatus *status) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t t = base + threadIdx.x;
+        bool q = false;
+        if (t < total) {
+            int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+            const DevDraw &D = draws[d];
+            int cls = -1;
+            if (D.flags & SR_DRAW_CLIP) {
+                const DevScene &S = *views[D.view].scene;
+                uint32_t lt = t - D.in_tri_base;
+                float c[3][4];
+                for (int k = 0; k < 3; k++) {
+                    uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
+                    float4 p = xv[3 * (size_t)(D.xv_base + vi)];
+                    c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
+                }
+                cls = clip_classify(c);
+            }
+            counts[t] = cls == -1 ? 1u : 0u;
+            q = cls >= 0;
+        }
+        uint64_t m = __ballot(q);
+        if (m) {
+            int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t slot0 = 0;
+            if (lane == leader) slot0 = atomicAdd(&status->clip_queued, (uint32_t)__popcll(m));
+            slot0 = (uint32_t)__shfl((int)slot0, leader, 64);
+            if (q) {
+                uint32_t slot = slot0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (slot < qcap) queue[slot] = t;
+                else atomicOr(&status->overflow_clip, 1u);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_clip: Sutherland-Hodgman against the six planes with all vertex
+// attributes (clip_against_frustum, src/renderer.rs:590-655).  Sixteen lanes
+// per queued triangle, one per vertex channel in CV order; the polygon lives
+// in LDS and every lane makes the same inside/outside decisions from the
+// shared positions, so each channel gets exactly the reference's
+// a + (b - a) * t at every step.  The clipped polygon goes to `buf` and its
+// fan size to counts.  In exact arithmetic a triangle clipped by six planes
+// has at most 9 vertices; rounding can make the polygon slightly non-convex,
+// so room is kept for CLIP_MAXV and a polygon that would outgrow it fails the
+// frame (status clip_overflow) rather than being cut short.
+// ---------------------------------------------------------------------------
+#define CLIP_MAXV 16
+
+__global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDraw *draws, int ndraws,
+                                              const float4 *xv, const float4 *xt, uint32_t *counts,
+                                              const uint32_t *queue, uint32_t qcap, float *buf,
+                                              DevStatus *status) {
+    __shared__ float sh[16][2][CLIP_MAXV][16];
+    const int g = threadIdx.x >> 4, c = threadIdx.x & 15;
+    uint32_t nq = min(status->clip_queued, qcap);
+    for (uint32_t e = blockIdx.x * 16u + (uint32_t)g; e < nq; e += gridDim.x * 16u) {
+        uint32_t t = queue[e];
         int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
         const DevDraw &D = draws[d];
-        if (!(D.flags & SR_DRAW_CLIP)) {
-            counts[t] = 1;
-            continue;
-        }
         const DevScene &S = *views[D.view].scene;
         uint32_t lt = t - D.in_tri_base;
-        float c[3][4];
+        float(*P)[16] = sh[g][0];
+        float(*N)[16] = sh[g][1];
         for (int k = 0; k < 3; k++) {
-            uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-            float4 p = xv[3 * (size_t)(D.xv_base + vi)];
-            c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
+            uint32_t vi = D.xv_base + S.indices[D.idx_base + 3 * lt + k];
+            float val;
+            if (c < 12) val = reinterpret_cast<const float *>(xv)[12 * (size_t)vi + c];
+            else val = ((D.flags & SR_DRAW_TANGENTS) && xt) ? reinterpret_cast<const float *>(xt)[4 * (size_t)vi + c - 12]
+                                                            : 0.0f;
+            P[k][c] = val;
         }
-        int cls = clip_classify(c);
-        counts[t] = cls == -1 ? 1u : (cls == -2 ? 0u : (uint32_t)clip_count_general(c, cls));
+        wave_lds_sync();
+        int len = 3;
+        bool over = false;
+        for (int p = 0; p < 6 && len > 0; p++) {
+            int nl = 0;
+            for (int i = 0; i < len; i++) {
+                int ip = i == 0 ? len - 1 : i - 1;
+                float dc = plane_dot(p, P[i][0], P[i][1], P[i][2], P[i][3]);
+                float dp = plane_dot(p, P[ip][0], P[ip][1], P[ip][2], P[ip][3]);
+                bool ci = dc >= 0.0f, pi = dp >= 0.0f;
+                if (nl + 2 > CLIP_MAXV) {
+                    over = true;
+                    break;
+                }
+                if (ci) {
+                    if (!pi) {
+                        float tt = dp / (dp - dc);
+                        N[nl][c] = P[ip][c] + (P[i][c] - P[ip][c]) * tt;
+                        nl++;
+                    }
+                    N[nl][c] = P[i][c];
+                    nl++;
+                } else if (pi) {
+                    float tt = dp / (dp - dc);
+                    N[nl][c] = P[ip][c] + (P[i][c] - P[ip][c]) * tt;
+                    nl++;
+                }
+            }
+            if (over) {
+                nl = 0;
+                if (c == 0) atomicOr(&status->clip_overflow, 1u);
+            }
+            float(*tmp)[16] = P;
+            P = N;
+            N = tmp;
+            len = nl;
+            wave_lds_sync();
+        }
+        if (len < 3) len = 0;
+        for (int i = 0; i < len; i++) buf[((size_t)e * CLIP_MAXV + i) * 16 + c] = P[i][c];
+        if (c == 0) counts[t] = len ? (uint32_t)(len - 2) : 0u;
+        wave_lds_sync();
     }
 }
 
@@ -569,21 +614,6 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
     return bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
 }
 
-// Clipped triangles (rare): Sutherland-Hodgman with attributes, fan, emit,
-// plain atomics for the bins.
-__device__ __noinline__ void clip_emit(const DevView &V, int view, const CV *v, int first, uint32_t id, int material,
-                                       uint32_t matflags, const SetupOut &O) {
-    CV poly[10], np[10];
-    poly[0] = v[0]; poly[1] = v[1]; poly[2] = v[2];
-    int len = clip_general(poly, np, first);
-    for (int i = 1; i + 1 < len; i++) {
-        Rect r = emit_triangle(V, view, &poly[0], &poly[i], &poly[i + 1], id + (uint32_t)(i - 1), material, matflags, O);
-        for (int gy = r.gy0; gy < r.gy1; gy++)
-            for (int gx = r.gx0; gx < r.gx1; gx++)
-                atomicAdd(&O.bin_count[V.item_base + (uint32_t)(gy * V.tiles_x + gx)], 1u);
-    }
-}
-
 __device__ __forceinline__ int wave_max(int v) {
     for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
     return v;
@@ -593,64 +623,96 @@ __device__ __forceinline__ int wave_max(int v) {
 // Bin counting / scattering for one rectangle per lane, called by the whole
 // wave.  Rectangles of at most SMALL_RECT bins (the common small-triangle
 // case, where neighbouring lanes hit the same bins) use wave-aggregated
-// atomics; larger ones are taken one at a time by the whole wave, one bin per
-// lane, so each triangle costs ceil(bins / 64) wave steps of distinct atomics.
+// atomics.  The bins of larger ones are laid out as one list over the wave
+// (a prefix sum of their sizes) and taken 64 at a time, one bin per lane, so
+// every step issues 64 independent atomics whatever the mix of sizes.
 #define SMALL_RECT 4
 
+struct RectBins {
+    int nb;            // bins in the rectangle (0 if empty)
+    int rw;            // rectangle width in tiles
+    uint32_t first;    // bin index of its top-left tile
+    int stride;        // tiles per row of its view
+};
+
+__device__ __forceinline__ RectBins rect_bins(const DevView *views, int view, Rect r) {
+    RectBins b;
+    b.rw = r.gx1 - r.gx0;
+    int rh = r.gy1 - r.gy0;
+    b.nb = (b.rw > 0 && rh > 0) ? b.rw * rh : 0;
+    b.first = 0u;
+    b.stride = 0;
+    if (b.nb > 0) {
+        const DevView &V = views[view];
+        b.stride = V.tiles_x;
+        b.first = V.item_base + (uint32_t)(r.gy0 * V.tiles_x + r.gx0);
+    }
+    return b;
+}
+
+// Owner lane of flattened element k (all lanes call): the first lane whose
+// inclusive count exceeds k; before = elements of the lanes ahead of it.
+__device__ __forceinline__ int flat_owner(int incl, int k, int &before) {
+    int o = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        int v = __shfl(incl, o + step - 1, 64);
+        if (v <= k) o += step;
+    }
+    int prev = __shfl(incl, o > 0 ? o - 1 : 0, 64);
+    before = o > 0 ? prev : 0;
+    return o;
+}
+
 __device__ __forceinline__ void bin_rect_count(const DevView *views, int view, Rect r, uint32_t *counts) {
-    int rw = r.gx1 - r.gx0, rh = r.gy1 - r.gy0;
-    int nb = (rw > 0 && rh > 0) ? rw * rh : 0;
-    const DevView &V = views[view];
-    int ns = nb <= SMALL_RECT ? nb : 0;
+    RectBins b = rect_bins(views, view, r);
+    int lane = threadIdx.x & 63;
+    int ns = b.nb <= SMALL_RECT ? b.nb : 0;
     int maxs = wave_max(ns);
     for (int k = 0; k < maxs; k++) {
         uint32_t bin = 0xFFFFFFFFu;
-        if (k < ns) bin = V.item_base + (uint32_t)((r.gy0 + k / rw) * V.tiles_x + r.gx0 + k % rw);
+        if (k < ns) bin = b.first + (uint32_t)((k / b.rw) * b.stride + k % b.rw);
         wave_count(counts, bin);
     }
-    uint64_t big = __ballot(nb > SMALL_RECT);
-    int lane = threadIdx.x & 63;
-    while (big) {
-        int L = __ffsll((unsigned long long)big) - 1;
-        big &= big - 1;
-        int bx0 = __builtin_amdgcn_readlane(r.gx0, L), by0 = __builtin_amdgcn_readlane(r.gy0, L);
-        int bw = __builtin_amdgcn_readlane(rw, L), bn = __builtin_amdgcn_readlane(nb, L);
-        int bv = __builtin_amdgcn_readlane(view, L);
-        const DevView &B = views[bv];
-        for (int k = lane; k < bn; k += 64)
-            atomicAdd(&counts[B.item_base + (uint32_t)((by0 + k / bw) * B.tiles_x + bx0 + k % bw)], 1u);
+    int incl = (int)wave_incl_scan((uint32_t)(b.nb > SMALL_RECT ? b.nb : 0));
+    int total = __builtin_amdgcn_readlane(incl, 63);
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        int k = k0 + lane, before;
+        int o = flat_owner(incl, k, before);
+        uint32_t first = (uint32_t)__shfl((int)b.first, o, 64);
+        int rw = __shfl(b.rw, o, 64), stride = __shfl(b.stride, o, 64);
+        int kk = k - before;
+        if (k < total) atomicAdd(&counts[first + (uint32_t)((kk / rw) * stride + kk % rw)], 1u);
     }
 }
 
 __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view, Rect r, uint32_t id, uint32_t *cursor,
                                                  uint32_t *list, uint32_t pair_cap, DevStatus *status) {
-    int rw = r.gx1 - r.gx0, rh = r.gy1 - r.gy0;
-    int nb = (rw > 0 && rh > 0) ? rw * rh : 0;
-    const DevView &V = views[view];
-    int ns = nb <= SMALL_RECT ? nb : 0;
+    RectBins b = rect_bins(views, view, r);
+    int lane = threadIdx.x & 63;
+    int ns = b.nb <= SMALL_RECT ? b.nb : 0;
     int maxs = wave_max(ns);
     for (int k = 0; k < maxs; k++) {
         uint32_t bin = 0xFFFFFFFFu;
-        if (k < ns) bin = V.item_base + (uint32_t)((r.gy0 + k / rw) * V.tiles_x + r.gx0 + k % rw);
+        if (k < ns) bin = b.first + (uint32_t)((k / b.rw) * b.stride + k % b.rw);
         uint32_t pos = wave_claim(cursor, bin);
         if (k < ns) {
             if (pos < pair_cap) list[pos] = id;
             else atomicOr(&status->overflow_pairs, 1u);
         }
     }
-    uint64_t big = __ballot(nb > SMALL_RECT);
-    int lane = threadIdx.x & 63;
-    while (big) {
-        int L = __ffsll((unsigned long long)big) - 1;
-        big &= big - 1;
-        int bx0 = __builtin_amdgcn_readlane(r.gx0, L), by0 = __builtin_amdgcn_readlane(r.gy0, L);
-        int bw = __builtin_amdgcn_readlane(rw, L), bn = __builtin_amdgcn_readlane(nb, L);
-        int bv = __builtin_amdgcn_readlane(view, L);
-        uint32_t bid = (uint32_t)__builtin_amdgcn_readlane((int)id, L);
-        const DevView &B = views[bv];
-        for (int k = lane; k < bn; k += 64) {
-            uint32_t pos = atomicAdd(&cursor[B.item_base + (uint32_t)((by0 + k / bw) * B.tiles_x + bx0 + k % bw)], 1u);
-            if (pos < pair_cap) list[pos] = bid;
+    int incl = (int)wave_incl_scan((uint32_t)(b.nb > SMALL_RECT ? b.nb : 0));
+    int total = __builtin_amdgcn_readlane(incl, 63);
+    for (int k0 = 0; k0 < total; k0 += 64) {
+        int k = k0 + lane, before;
+        int o = flat_owner(incl, k, before);
+        uint32_t first = (uint32_t)__shfl((int)b.first, o, 64);
+        int rw = __shfl(b.rw, o, 64), stride = __shfl(b.stride, o, 64);
+        uint32_t oid = (uint32_t)__shfl((int)id, o, 64);
+        int kk = k - before;
+        if (k < total) {
+            uint32_t pos = atomicAdd(&cursor[first + (uint32_t)((kk / rw) * stride + kk % rw)], 1u);
+            if (pos < pair_cap) list[pos] = oid;
             else atomicOr(&status->overflow_pairs, 1u);
         }
     }
@@ -684,8 +746,50 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
                 for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
                 cls = clip_classify(c);
             }
+            // triangles that cross a plane are set up by k_clip_emit
             if (cls == -1) r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
-            else if (cls >= 0) clip_emit(V, D.view, v, cls, id, D.material, matflags, O);
+        }
+        bin_rect_count(views, view, r, O.bin_count);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_clip_emit: fan triangles (v0, v_i, v_i+1) of the clipped polygons
+// (src/renderer.rs:657-666), one thread per (queued triangle, fan slot).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_clip_emit(const DevView *views, const DevDraw *draws, int ndraws,
+                                                   const uint32_t *queue, uint32_t qcap, const float *buf,
+                                                   const uint32_t *counts, const uint32_t *tri_base,
+                                                   const DevStatus *status, SetupOut O) {
+    uint32_t nq = min(status->clip_queued, qcap);
+    uint32_t total = nq * (uint32_t)(CLIP_MAXV - 2);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t k = base + threadIdx.x;
+        Rect r = {0, 0, 0, 0};
+        int view = 0;
+        if (k < total) {
+            uint32_t e = k / (uint32_t)(CLIP_MAXV - 2), f = k % (uint32_t)(CLIP_MAXV - 2);
+            uint32_t t = queue[e];
+            if (f < counts[t]) {
+                int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+                const DevDraw &D = draws[d];
+                view = D.view;
+                const DevView &V = views[D.view];
+                const DevScene &S = *V.scene;
+                CV v[3];
+                const uint32_t src[3] = {0u, f + 1u, f + 2u};
+                for (int j = 0; j < 3; j++) {
+                    const float4 *q = reinterpret_cast<const float4 *>(buf + ((size_t)e * CLIP_MAXV + src[j]) * 16);
+                    float4 a = q[0], b = q[1], cc = q[2], dd = q[3];
+                    v[j].cx = a.x; v[j].cy = a.y; v[j].cz = a.z; v[j].cw = a.w;
+                    v[j].wx = b.x; v[j].wy = b.y; v[j].wz = b.z; v[j].nx = b.w;
+                    v[j].ny = cc.x; v[j].nz = cc.y; v[j].u = cc.z; v[j].v = cc.w;
+                    v[j].tx = dd.x; v[j].ty = dd.y; v[j].tz = dd.z; v[j].tw = dd.w;
+                }
+                r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], tri_base[t] + f, D.material,
+                                  S.materials[D.material].flags, O);
+            }
         }
         bin_rect_count(views, view, r, O.bin_count);
     }
@@ -1179,12 +1283,6 @@ template <int T> struct Stage {
     static constexpr int COUNT = LO + (NR + 7) / 8;
 };
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // f32 corner test of the 16px block at (bx, by) (src/tilerasterizer.rs:258-291)
 __device__ __forceinline__ bool block_out(const TriSetup &ts, int bx, int by) {
     float cx0 = (float)(bx * 16 + 8), cx1 = (float)(bx * 16 + 256 - 8);
@@ -1640,24 +1738,31 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (a.total_xv > 0)
         hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_xv, a.xv, a.any_tangents ? a.xt : nullptr);
-    if (a.total_in > 0)
-        hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.tri_counts);
-    // tri_counts[total_in] is a zero sentinel so tri_base[total_in] = total
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, a.tri_counts + a.total_in, 1u, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.bin_count,
-                       a.total_items, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(a.status),
                        (uint32_t)(sizeof(DevStatus) / 4), 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.bin_count,
+                       a.total_items, 0u);
+    if (a.total_in > 0) {
+        hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
+                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.clip_queue, a.clip_cap, a.status);
+        hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv,
+                           a.any_tangents ? a.xt : nullptr, a.tri_counts, a.clip_queue, a.clip_cap, a.clip_buf,
+                           a.status);
+    }
+    // tri_counts[total_in] is a zero sentinel so tri_base[total_in] = total
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, a.tri_counts + a.total_in, 1u, 0u);
     hipError_t e = launch_scan(a.tri_counts, a.total_in + 1, a.tri_base, nullptr, a.scan_partials,
                                &a.status->total_records, st);
     if (e != hipSuccess) return e;
     SetupOut so;
     so.geo = a.rec_geo; so.raster = a.rec_raster; so.shade = a.rec_shade; so.tan = a.rec_tan;
     so.bin_count = a.bin_count; so.cap = a.rec_cap; so.status = a.status; so.T = a.tile_size;
-    if (a.total_in > 0)
+    if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv, a.any_tangents ? a.xt : nullptr, a.tri_base, so);
+        hipLaunchKernelGGL(k_clip_emit, dim3(256), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.clip_queue,
+                           a.clip_cap, a.clip_buf, a.tri_counts, a.tri_base, a.status, so);
+    }
     if (a.events[1]) hipEventRecord(a.events[1], st);
     e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st);
     if (e != hipSuccess) return e;

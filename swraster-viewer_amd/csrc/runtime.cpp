@@ -204,7 +204,7 @@ void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     DevBuf *bufs[] = {&views, &draws, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
-                      &list, &rec_geo, &rec_raster, &rec_shade, &rec_tan, &status, &center};
+                      &list, &rec_geo, &rec_raster, &rec_shade, &rec_tan, &status, &center, &clip_queue, &clip_buf};
     for (DevBuf *b : bufs) b->release();
     stage.release();
     status_host.release();
@@ -240,12 +240,15 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     if ((r = partials.ensure(((maxn + 4095) / 4096 + 1) * 4))) return r;
     if (rec_cap < total_in + total_in / 8 + 1024) rec_cap = total_in + total_in / 8 + 1024;
     if (pair_cap < rec_cap * 2) pair_cap = rec_cap * 2;
+    if (clip_cap < 4096) clip_cap = 4096;
     for (int attempt = 0; attempt < 4; attempt++) {
         if ((r = rec_geo.ensure((size_t)rec_cap * 16))) return r;
         if ((r = rec_raster.ensure((size_t)rec_cap * 64))) return r;
         if ((r = rec_shade.ensure((size_t)rec_cap * 128))) return r;
         if (any_tangents && (r = rec_tan.ensure((size_t)rec_cap * 48))) return r;
         if ((r = list.ensure((size_t)pair_cap * 4))) return r;
+        if ((r = clip_queue.ensure((size_t)clip_cap * 4))) return r;
+        if ((r = clip_buf.ensure((size_t)clip_cap * 16 * 16 * 4))) return r;
         HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, vbytes, hipMemcpyHostToDevice, stream));
         if (!hd.empty())
             HIP_TRY(hipMemcpyAsync(draws.ptr, (char *)stage.ptr + vbytes, hd.size() * sizeof(DevDraw),
@@ -276,6 +279,9 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.rec_tan = any_tangents ? (float4 *)rec_tan.ptr : nullptr;
         a.rec_cap = rec_cap;
         a.pair_cap = pair_cap;
+        a.clip_queue = (uint32_t *)clip_queue.ptr;
+        a.clip_buf = (float *)clip_buf.ptr;
+        a.clip_cap = clip_cap;
         a.status = (DevStatus *)status.ptr;
         a.out_ids = out_ids;
         a.out_depth = out_depth;
@@ -293,8 +299,12 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         const DevStatus &s = *(const DevStatus *)status_host.ptr;
         last_records = s.total_records;
         last_pairs = s.total_pairs;
-        if (!s.overflow_records && !s.overflow_pairs && s.total_records <= rec_cap && s.total_pairs <= pair_cap)
+        if (s.clip_overflow)
+            return fail(SR_ERR_UNSUPPORTED, "a frustum-clipped polygon exceeded 16 vertices (degenerate rounding)");
+        if (!s.overflow_records && !s.overflow_pairs && !s.overflow_clip && s.total_records <= rec_cap &&
+            s.total_pairs <= pair_cap)
             return SR_OK;
+        if (s.overflow_clip || s.clip_queued > clip_cap) clip_cap = s.clip_queued + s.clip_queued / 4 + 1024;
         if (s.total_records > rec_cap || s.overflow_records) rec_cap = s.total_records + s.total_records / 8 + 1024;
         if (s.total_pairs > pair_cap || s.overflow_pairs) pair_cap = s.total_pairs + s.total_pairs / 8 + 4096;
         if (s.overflow_pairs && s.total_pairs <= pair_cap && !s.overflow_records) pair_cap *= 2;

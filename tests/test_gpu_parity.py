@@ -107,6 +107,35 @@ def test_c1_inside_cluster_clipping(sr, orc, scenes, c1):
     compare(sr, g, o, "c1 inside")
 
 
+@pytest.mark.parametrize("seed", [31, 32])
+def test_clip_stress(sr, orc, scenes, seed):
+    """Thousands of random triangles around the camera, many crossing the
+    near and side planes (2-7 fan triangles each), plus near-collinear
+    slivers: setup count, visibility buffer and pixels match the oracle."""
+    from util import attach_dummy
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 3000
+    p = rng.uniform([-6, -5, -12], [6, 5, 1], (n, 3, 3))
+    sl = rng.random(n) < 0.2            # slivers: third vertex near the first edge
+    p[sl, 2] = p[sl, 0] + (p[sl, 1] - p[sl, 0]) * rng.uniform(0, 1, (sl.sum(), 1)) + rng.normal(0, 1e-4, (sl.sum(), 3))
+    pos = p.reshape(-1, 3).astype(np.float32)
+    e1, e2 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]
+    fn = np.cross(e1, e2)
+    fn /= np.linalg.norm(fn, axis=-1, keepdims=True) + 1e-30
+    prim = {"positions": pos, "normals": np.repeat(fn, 3, axis=0).astype(np.float32), "texcoords": None,
+            "tangents": None, "indices": np.arange(len(pos), dtype=np.uint32), "material": 0}
+    scene = sr.SceneData([prim], [(0, 1)], [(np.eye(4, dtype=np.float32), 0)],
+                         [{"base_color": (0.8, 0.5, 0.3, 1), "metallic": 0.1, "roughness": 0.6}], [])
+    attach_dummy(sr, scene)
+    cam = scenes.make_camera({"position": [0.0, 0.0, 0.0], "yaw": 0.0, "pitch": 0.0, "fov": np.pi / 3,
+                              "width": 320, "height": 240, "far": 50.0})
+    g = render_gpu(sr, sr.Scene(scene), cam, 320, 240)
+    o = render_oracle(orc, scene, cam, 320, 240)
+    assert o["setup"] > n // 2
+    assert g["stats"]["setup_triangles"] == o["setup"]
+    compare(sr, g, o, f"clip stress {seed}")
+
+
 def test_c1_exposure_adaptation(sr, orc, scenes, c1):
     scene, spec, gscene = c1
     cam = scenes.make_camera(spec, 640, 360)
