@@ -493,39 +493,48 @@ struct SetupOut {
     int T;
 };
 
-// Wave-aggregated counting: every lane of the wave calls this together; lanes
-// with bin == ~0u take no part.  One atomic per distinct bin per wave.
-__device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t bin) {
+// Wave-aggregated bin atomics: every lane of the wave calls these together;
+// lanes with bin == ~0u take no part.  Lanes are grouped by bin first (ballot
+// rounds, no memory traffic), then each group's first lane adds the group
+// size with one atomic instruction for all groups, so the wave waits for at
+// most one round trip.
+__device__ __forceinline__ void wave_groups(uint32_t bin, int &leader, uint32_t &rank, uint32_t &group) {
     bool pending = bin != 0xFFFFFFFFu;
-    while (__ballot(pending)) {
+    int lane = threadIdx.x & 63;
+    leader = lane;
+    rank = 0;
+    group = 0;
+    for (;;) {
         uint64_t pm = __ballot(pending);
+        if (!pm) break;
         int first = __ffsll((unsigned long long)pm) - 1;
-        uint32_t leader = __shfl(bin, first, 64);
-        bool mine = pending && bin == leader;
+        uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bin, first);
+        bool mine = pending && bin == lb;
         uint64_t m = __ballot(mine);
-        if (mine && (int)(threadIdx.x & 63) == first) atomicAdd(&counts[leader], (uint32_t)__popcll(m));
+        if (mine) {
+            leader = first;
+            rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        }
+        if (lane == first) group = (uint32_t)__popcll(m);
         pending = pending && !mine;
     }
 }
 
-// Wave-aggregated slot claim: returns this lane's slot (bin != ~0u).
+__device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t bin) {
+    int leader;
+    uint32_t rank, group;
+    wave_groups(bin, leader, rank, group);
+    if (group) atomicAdd(&counts[bin], group);
+}
+
+// Returns this lane's slot (bin != ~0u).
 __device__ __forceinline__ uint32_t wave_claim(uint32_t *cursor, uint32_t bin) {
-    bool pending = bin != 0xFFFFFFFFu;
-    uint32_t slot = 0;
-    int lane = threadIdx.x & 63;
-    while (__ballot(pending)) {
-        uint64_t pm = __ballot(pending);
-        int first = __ffsll((unsigned long long)pm) - 1;
-        uint32_t leader = __shfl(bin, first, 64);
-        bool mine = pending && bin == leader;
-        uint64_t m = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == first) base = atomicAdd(&cursor[leader], (uint32_t)__popcll(m));
-        base = __shfl(base, first, 64);
-        if (mine) slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        pending = pending && !mine;
-    }
-    return slot;
+    int leader;
+    uint32_t rank, group;
+    wave_groups(bin, leader, rank, group);
+    uint32_t base = 0;
+    if (group) base = atomicAdd(&cursor[bin], group);
+    return (uint32_t)__shfl((int)base, leader, 64) + rank;
 }
 
 // Snap, cull, bbox and setup of one (post-clip) triangle (bin_triangle,
@@ -1766,7 +1775,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (a.events[1]) hipEventRecord(a.events[1], st);
     e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scatter, dim3(GRID_CAP), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(a.rec_cap, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
                        a.bin_end, a.list, a.pair_cap, a.status, a.tile_size);
     if (a.events[2]) hipEventRecord(a.events[2], st);
     RasterArgs P;
