@@ -98,11 +98,18 @@ __device__ __forceinline__ f3 srgb_fast(f3 x) {
     r.z = x.z * (x.z * (x.z * 0.305306011f + 0.682171111f) + 0.012522878f);
     return r;
 }
-__device__ __forceinline__ f3 unpack_rgb(uint32_t c) {
-    return {(float)((c >> 24) & 0xFFu) / 255.0f, (float)((c >> 16) & 0xFFu) / 255.0f,
-            (float)((c >> 8) & 0xFFu) / 255.0f};
+// n / 255.0f, correctly rounded, for integers n in [0, 255]: the product with
+// RN(1/255) is corrected by one fused multiply-add on its exact residual
+// (checked against IEEE division for all 256 inputs).
+__device__ __forceinline__ float div255(float n) {
+    const float inv = 1.0f / 255.0f;
+    float q = n * inv;
+    return fmaf(fmaf(-255.0f, q, n), inv, q);
 }
-__device__ __forceinline__ float unpack_a(uint32_t c) { return (float)(c & 0xFFu) / 255.0f; }
+__device__ __forceinline__ f3 unpack_rgb(uint32_t c) {
+    return {div255((float)((c >> 24) & 0xFFu)), div255((float)((c >> 16) & 0xFFu)), div255((float)((c >> 8) & 0xFFu))};
+}
+__device__ __forceinline__ float unpack_a(uint32_t c) { return div255((float)(c & 0xFFu)); }
 
 template <class T>
 __device__ __forceinline__ int find_last_le(const T *arr, int n, uint32_t key, uint32_t T::*field) {
@@ -943,15 +950,24 @@ __device__ __forceinline__ float edge_chain(const TriSetup &S, int e, int bx, in
 // ---------------------------------------------------------------------------
 // Texture sampling (src/texture.rs:578-863)
 // ---------------------------------------------------------------------------
+// x / d; for d a power of two (texture sizes usually are) the product with the
+// exact reciprocal 2^-k, which rounds the same exact quotient.
+__device__ __forceinline__ float div_dim(float x, float d) {
+    uint32_t b = __float_as_uint(d);
+    uint32_t e = (b >> 23) & 0xFFu;
+    if ((b & 0x807FFFFFu) == 0u && e >= 1u && e <= 253u) return x * __uint_as_float((254u - e) << 23);
+    return x / d;
+}
+
 __device__ __forceinline__ float apply_wrap(float texel, float dim, int mode) {
     float t2;
     if (mode == SR_WRAP_CLAMP_TO_EDGE) {
         t2 = texel;
     } else if (mode == SR_WRAP_REPEAT) {
-        t2 = texel - floorf(texel / dim) * dim;
+        t2 = texel - floorf(div_dim(texel, dim)) * dim;
     } else {
         float two = dim * 2.0f;
-        float t = texel - floorf(texel / two) * two;
+        float t = texel - floorf(div_dim(texel, two)) * two;
         t2 = minps(t, two - t);
     }
     return minps(t2, dim - 1.0f);

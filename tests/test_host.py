@@ -140,3 +140,42 @@ def test_dummy_voxels_helper(sr, scenes):
     scene, _ = scenes.make_c1()
     v = dummy_voxels(sr, scene)
     assert v.dims == (2, 2, 2)
+
+
+def _fma32(a, b, c):
+    """Correctly rounded float32 fused multiply-add, from exact rationals."""
+    from fractions import Fraction
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    cand = np.float32(float(exact))
+    best = None
+    for f in (np.nextafter(cand, np.float32(-np.inf)), cand, np.nextafter(cand, np.float32(np.inf))):
+        d = abs(Fraction(float(f)) - exact)
+        if best is None or d < best[0] or (d == best[0] and (f.view(np.uint32) & 1) == 0):
+            best = (d, f)
+    return best[1]
+
+
+def test_div255_identity():
+    """The shading kernels turn a texel byte n into n / 255 with
+    q = n * RN(1/255), then q + (n - 255 q) * RN(1/255) in two fused
+    multiply-adds; that equals IEEE division for every n in [0, 255]."""
+    inv = np.float32(1.0) / np.float32(255.0)
+    plain_wrong = 0
+    for n in range(256):
+        x = np.float32(n)
+        ref = x / np.float32(255.0)
+        q = x * inv
+        plain_wrong += int(q != ref)
+        q2 = _fma32(_fma32(np.float32(-255.0), q, x), inv, q)
+        assert q2 == ref, n
+    assert plain_wrong > 0          # the correction step is needed
+
+
+def test_power_of_two_reciprocal_division():
+    """x / 2^k equals x * 2^-k (both round the same exact value), which the
+    texture wrap uses for power-of-two sizes; spot-check across magnitudes."""
+    rng = np.random.Generator(np.random.PCG64(4))
+    x = (rng.standard_normal(100000) * np.exp2(rng.integers(-140, 120, 100000))).astype(np.float32)
+    for k in (0, 1, 5, 11, 12, 20):
+        d = np.float32(2.0 ** k)
+        assert np.array_equal((x / d).view(np.uint32), (x * np.float32(2.0 ** -k)).view(np.uint32))
