@@ -11,12 +11,21 @@ Prints one JSON line:
   value          Gpix/s (W*H / frame time), whole job (one GPU)
   mtri_per_s     submitted triangles / frame time
   p50/p99 ms     per-frame latency over the timed steps
-  roofline       dominant kernel (raster+shade): algorithmic bytes per launch
-                 (vbuffer write 8 + read 8 + RGBA8 write 4 per pixel, plus one
-                 4-byte base-colour texel per covered textured pixel) over its
-                 HIP-event time, against 8 TB/s
-  frame_roofline the same for the whole frame with the full per-frame
-                 algorithmic bytes (vertices, indices, pixels, texels)
+  roofline       the dominant kernel of the frame (k_raster or k_shade, the
+                 longer by HIP events on the renderer's stream): algorithmic
+                 bytes per launch over its average launch time, against 8 TB/s;
+                   k_raster: 80 B per set-up triangle (geometry + raster
+                     records, read once) + 4 B per bin entry + 4 B per pixel
+                     (triangle id written)
+                   k_shade: 20 B per pixel (id read, depth and three colour
+                     planes written) + 192 B per visible triangle (raster +
+                     shading records) + 4 B per covered pixel of a textured
+                     material (one base-colour texel)
+                 traffic: HBM bytes per launch measured with rocprofv3
+                 FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a
+                 device copy of known size (profiles/traffic_<config>.json)
+  frame_roofline the whole frame with its algorithmic bytes (vertices,
+                 indices, pixels, texels)
   cpu_baseline   the CPU oracle in reference-like threaded mode (128-triangle
                  jobs into shared 64px-tile queues, one job per tile) on this
                  host's cores, on a bounded sample of the same frames
@@ -36,14 +45,32 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mtri/s & Gpix/s @1080p; batched frames/s vs plain; p50/p99 frame latency"
 
 
-def algorithmic_bytes(scene, w, h, n_cov):
+def algorithmic_bytes(scene, w, h, n_cov, setup_tris, pairs, visible_tris):
     textured = any(m.get("base_color_tex") is not None for m in scene.materials)
     nv = sum(len(p["positions"]) for p in scene.primitives)
     nt = scene.triangles
     per_vertex = 24 + (8 if textured else 0)
-    frame = per_vertex * nv + 12 * nt + 20 * w * h + (4 * n_cov if textured else 0)
-    kernel = 20 * w * h + (4 * n_cov if textured else 0)
-    return frame, kernel
+    texels = 4 * n_cov if textured else 0
+    frame = per_vertex * nv + 12 * nt + 12 * w * h + texels
+    kernels = {"k_raster": 80 * setup_tris + 4 * pairs + 4 * w * h,
+               "k_shade": 20 * w * h + 192 * visible_tris + texels}
+    return frame, kernels
+
+
+def copy_calibration(mib):
+    """Device-to-device copy of a known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, n, 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
 def main():
@@ -55,7 +82,11 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
@@ -89,7 +120,6 @@ def main():
         frame()
     hip.hipDeviceSynchronize()
     lat = []
-    kernel_ms = []
     stage = {"ms_setup": 0.0, "ms_bin": 0.0, "ms_raster": 0.0, "ms_shade": 0.0, "ms_raster_shade": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -97,7 +127,6 @@ def main():
         frame()
         lat.append((time.perf_counter() - t) * 1e3)
         st = r.stats()
-        kernel_ms.append(st["ms_raster_shade"])
         for k in stage:
             stage[k] += st[k]
     hip.hipDeviceSynchronize()
@@ -105,11 +134,15 @@ def main():
     ms = total * 1e3 / args.steps
     ids, _ = r.read_vbuffer()
     n_cov = int((ids != sr.NO_ID).sum())
-    frame_b, kernel_b = algorithmic_bytes(scene, w, h, n_cov)
-    k_ms = float(np.mean(kernel_ms))
+    visible = int(np.unique(ids[ids != sr.NO_ID]).size)
+    st = r.stats()
+    frame_b, kernel_bytes = algorithmic_bytes(scene, w, h, n_cov, st["setup_triangles"], st["binned_pairs"], visible)
+    kernel_ms = {"k_raster": stage["ms_raster"] / args.steps, "k_shade": stage["ms_shade"] / args.steps}
+    dom = max(kernel_ms, key=kernel_ms.get)
+    k_ms = kernel_ms[dom]
+    kernel_b = kernel_bytes[dom]
     achieved = kernel_b / (k_ms * 1e-3) / 1e9
     frame_achieved = frame_b / (ms * 1e-3) / 1e9
-    st = r.stats()
     out = {
         "metric": METRIC,
         "value": round(w * h / (ms * 1e-3) / 1e9, 4),
@@ -136,17 +169,22 @@ def main():
         "setup_triangles": st["setup_triangles"],
         "binned_pairs": st["binned_pairs"],
         "covered_pixels": n_cov,
-        "roofline": {"kernel": "k_raster_shade", "bound": "hbm", "achieved": round(achieved, 2),
+        "visible_triangles": visible,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None, "algorithmic_bytes_per_launch": kernel_b,
-                     "avg_launch_ms": round(k_ms, 5)},
+                     "avg_launch_ms": round(k_ms, 5),
+                     "kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
+                     "kernels_algorithmic_bytes": kernel_bytes},
         "frame_roofline": {"achieved": round(frame_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(frame_achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": frame_b},
     }
     traffic_path = os.path.join(REPO, "profiles", f"traffic_{args.config}.json")
     if os.path.exists(traffic_path):
         with open(traffic_path) as fh:
-            out["roofline"]["traffic"] = json.load(fh).get("bytes_per_launch")
+            tr = json.load(fh).get("bytes_per_launch", {})
+        if dom in tr:
+            out["roofline"]["traffic"] = tr[dom]
 
     if not args.no_cpu_baseline:
         orc = entry.load_oracle()
