@@ -73,12 +73,89 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}))
 
 
+def run_batched(args):
+    """C4: 256 mixed-resolution views over the heightfield pool, rendered as
+    one batch (one launch sequence, persistent raster kernel, fused blit)
+    against the same views as plain frames (render_scene +
+    update_auto_exposure(0) + blit to device, one after another)."""
+    import __graft_entry__ as entry
+    sr = entry.load_package()
+    import swraster_viewer_amd.scenes as sc
+    import ctypes as C
+
+    pool = sc.make_c4_pool()
+    views = sc.make_c4_views(pool, views=args.views)
+    env = sr.Environment.load()
+    scenes = []
+    for p in pool:
+        p.prepare(env, cache_key="c2_s2_g724")      # same terrain as C2: its voxel grid
+        scenes.append(sr.Scene(p))
+    cams = [sc.make_camera(spec) for _, spec in views]
+    L = sr.lib()
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    renderers, targets = {}, {}
+    for _, spec in views:
+        key = (spec["width"], spec["height"])
+        if key not in renderers:
+            renderers[key] = sr.Renderer(*key, tile_size=args.tile)
+            targets[key] = C.c_void_p()
+            assert hip.hipMalloc(C.byref(targets[key]), key[0] * key[1] * 4) == 0
+
+    def plain():
+        for (k, spec), cam in zip(views, cams):
+            r = renderers[(spec["width"], spec["height"])]
+            sr.check(L.sr_renderer_render_scene(r.handle, scenes[k].handle, C.byref(cam.c)))
+            sr.check(L.sr_renderer_update_auto_exposure(r.handle, 0.0))
+            sr.check(L.sr_renderer_blit_to_device(r.handle, targets[(spec["width"], spec["height"])]))
+
+    batch = sr.Batch(tile_size=args.tile)
+    bviews = [(scenes[k], cam, spec["width"], spec["height"], 0.0) for (k, spec), cam in zip(views, cams)]
+
+    def batched():
+        batch.render(bviews)
+
+    reps = max(1, min(args.steps, 5))
+    times = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        fn()                                       # warm-up: buffers sized, code paged in
+        hip.hipDeviceSynchronize()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            hip.hipDeviceSynchronize()
+            ts.append(time.perf_counter() - t)
+        times[name] = float(np.median(ts))
+    n = len(views)
+    tris = sum(pool[k].triangles for k, _ in views)
+    px = sum(spec["width"] * spec["height"] for _, spec in views)
+    st = batch.stats()
+    return {
+        "metric": "batched frames/s vs plain (256 mixed-resolution views)",
+        "value": round(n / times["batched"], 2), "unit": "frames/s", "n_gpus": 1, "steps": reps,
+        "warmup": 1, "ms_per_step": round(times["batched"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded generator, swraster-viewer_amd/scenes.py make_c4_pool/make_c4_views)",
+        "config": {"workload": "c4_batched_heightfields", "views": n, "pool_scenes": len(pool),
+                   "triangles": tris, "pixels": px, "tile_size": args.tile or 16,
+                   "step": "one batch of all views (batched) vs the same views as plain frames"},
+        "plain_frames_per_s": round(n / times["plain"], 2),
+        "plain_ms": round(times["plain"] * 1e3, 3),
+        "batched_vs_plain": round(times["plain"] / times["batched"], 3),
+        "mtri_per_s": round(tris / times["batched"] / 1e6, 1),
+        "gpix_per_s": round(px / times["batched"] / 1e9, 3),
+        "batch_stats": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--views", type=int, default=256, help="c4: views per batch")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -86,6 +163,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.config == "c4":
+        print(json.dumps(run_batched(args)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",

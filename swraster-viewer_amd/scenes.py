@@ -21,6 +21,13 @@ is roll * pitch * yaw, so a positive pitch field looks *down*).
       camera at the origin (60 deg, far 150).  Untextured; the triangles are
       split over 1,024 primitives with one random material each (materials
       are per primitive in the reference, so colour varies per primitive).
+  C4  256 views over a pool of heightfield scenes built as C2's (same terrain
+      seed, one 2048^2 texture each) with triangle counts log-uniform in
+      10k-2M; each view picks a scene, a resolution uniformly from
+      {320x240, 640x480, 1280x720, 1920x1080} and an orbit camera aimed at
+      the terrain centre (azimuth U[0, 2pi), distance U[45, 75], height
+      U[12, 30]).  The scenes share the terrain function, so the voxel grid
+      built for C2 lights all of them.
 """
 import json
 import os
@@ -236,6 +243,59 @@ def make_c3(seed=3, width=3840, height=2160, triangles=100_000, primitives=1024,
 
 
 CONFIGS = {"c1": make_c1, "c2": make_c2, "c3": make_c3}
+
+
+# ---------------------------------------------------------------------------
+# C4: batched views over a pool of heightfields
+# ---------------------------------------------------------------------------
+C4_RESOLUTIONS = ((320, 240), (640, 480), (1280, 720), (1920, 1080))
+
+
+def look_pose(position, target):
+    """(yaw, pitch) in the reference convention whose view direction points
+    from position to target: forward = (sin y cos p, -sin p, -cos y cos p)."""
+    d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+    d /= np.linalg.norm(d)
+    return float(np.arctan2(d[0], -d[2])), float(np.arcsin(-d[1]))
+
+
+def make_c4_pool(seed=4, scenes=16, tris_min=10_000, tris_max=2_000_000, tex_size=2048):
+    """Heightfield scenes with log-uniform triangle counts; each samples the
+    C2 terrain (seed 2) at its own grid resolution."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = np.sort(np.exp(rng.uniform(np.log(tris_min), np.log(tris_max), scenes)))
+    tex_rng = np.random.Generator(np.random.PCG64(seed + 1000))
+    t = value_noise((tex_size, tex_size), tex_rng, octaves=4, base_cells=16)
+    g = ((t * 0.5 + 0.5) * 255).clip(0, 255).astype(np.uint8)
+    rgb = (g[..., None] * np.array([0.55, 0.75, 0.45])[None, None, :]).clip(0, 255).astype(np.uint8)
+    rgba = np.concatenate([rgb, np.full(rgb.shape[:2] + (1,), 255, np.uint8)], -1)
+    tex = (tex_size, tex_size, SR_TEX_SRGB, pack_rgba(rgba).ravel())
+    pool = []
+    for c in counts:
+        n = max(3, int(round(np.sqrt(c / 2.0))) + 1)
+        pos, nrm, uv, idx = heightfield(n, 50.0, 5.0, np.random.Generator(np.random.PCG64(2)))
+        prims = [{"positions": pos, "normals": nrm, "texcoords": uv, "tangents": None, "indices": idx,
+                  "material": 0}]
+        mats = [{"base_color": (1.0, 1.0, 1.0, 1.0), "metallic": 0.0, "roughness": 0.5,
+                 "base_color_tex": (0, SR_WRAP_REPEAT, SR_WRAP_REPEAT)}]
+        pool.append(SceneData(prims, [(0, 1)], [(np.eye(4, dtype=np.float32), 0)], mats, [tex],
+                              name=f"c4_g{n}"))
+    return pool
+
+
+def make_c4_views(pool, seed=4, views=256, fov=np.pi / 3):
+    """[(scene index, camera_spec)] for the batched configuration."""
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    out = []
+    for _ in range(views):
+        k = int(rng.integers(len(pool)))
+        w, h = C4_RESOLUTIONS[int(rng.integers(len(C4_RESOLUTIONS)))]
+        az, dist, hgt = rng.uniform(0, 2 * np.pi), rng.uniform(45, 75), rng.uniform(12, 30)
+        pos = (dist * np.sin(az), hgt, dist * np.cos(az))
+        yaw, pitch = look_pose(pos, (0.0, 0.0, 0.0))
+        _, _, _, diag = pool[k].bounds()
+        out.append((k, camera_spec(pos, yaw, pitch, fov, w, h, 2.0 * diag)))
+    return out
 
 
 # ---------------------------------------------------------------------------
