@@ -30,8 +30,6 @@ struct FrameArgs {
     uint32_t *list;               // pair_cap
     uint4 *rec_geo;               // rec_cap
     float4 *rec_raster;           // rec_cap * 4
-    float4 *rec_shade;            // rec_cap * 8
-    float4 *rec_tan;              // rec_cap * 3 (or null)
     uint32_t rec_cap, pair_cap;
     uint32_t *clip_queue;         // clip_cap input triangles that cross a frustum plane
     float *clip_buf;              // clip_cap * 16 vertices * 16 channels

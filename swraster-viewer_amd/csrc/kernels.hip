@@ -492,8 +492,6 @@ __device__ __forceinline__ Rect bin_rect(const DevView &V, int mnx, int mny, int
 struct SetupOut {
     uint4 *geo;
     float4 *raster;
-    float4 *shade;
-    float4 *tan;
     uint32_t *bin_count;
     uint32_t cap;
     DevStatus *status;
@@ -545,10 +543,15 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t *cursor, uint32_t bin) {
 }
 
 // Snap, cull, bbox and setup of one (post-clip) triangle (bin_triangle,
-// src/renderer.rs:668-831).  Writes the records and returns the bin rectangle
-// (empty when culled or when the box is empty).
+// src/renderer.rs:668-831).  Writes the geometry and raster records and
+// returns the bin rectangle (empty when culled or when the box is empty).
+// The shading interpolators of the reference's packet (:701-754) are not
+// stored: tri_shade_attrs recomputes them, with the same operations, for the
+// few triangles that win a pixel, from the vertices that `src` locates
+// (SRC_* encoding below).
+#define SRC_CLIPPED 0x80000000u       // src0 = SRC_CLIPPED | fan << 27 | clip slot
 __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV *v1, const CV *v2, uint32_t id,
-                              int material, uint32_t matflags, const SetupOut &O) {
+                              int material, uint32_t matflags, uint32_t src0, uint32_t src1, const SetupOut &O) {
     Rect none = {0, 0, 0, 0};
     if (id >= O.cap) {
         atomicOr(&O.status->overflow_records, 1u);
@@ -579,14 +582,85 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
         return none;
     }
     float ooa = 1.0f / (float)wabs(area);
-    float oow[3], zow[3], uw[3], vw[3];
-    f3 pw[3];
+    float oow[3], zow[3];
     for (int k = 0; k < 3; k++) oow[k] = 1.0f / tv[k]->cw;
     for (int k = 0; k < 3; k++) zow[k] = tv[k]->cz * oow[k];
+    uint32_t flags = 0;
+    if (matflags & SR_MAT_ALPHA_TESTED) flags |= SR_GEO_ALPHA;
+    if (matflags & SR_MAT_NORMAL_MAP) flags |= SR_GEO_NORMAL_MAP;
+    O.geo[id] = make_uint4((uint32_t)pmnx | ((uint32_t)pmny << 16), (uint32_t)pmxx | ((uint32_t)pmxy << 16), flags,
+                           (uint32_t)view);
+    float4 *R = O.raster + 4 * (size_t)id;
+    R[0] = make_float4(__int_as_float(px[0]), __int_as_float(py[0]), __int_as_float(px[1]), __int_as_float(py[1]));
+    R[1] = make_float4(__int_as_float(px[2]), __int_as_float(py[2]), ooa, oow[0]);
+    R[2] = make_float4(oow[1] - oow[0], oow[2] - oow[0], zow[0], zow[1] - zow[0]);
+    R[3] = make_float4(zow[2] - zow[0], __int_as_float(material), __uint_as_float(src0), __uint_as_float(src1));
+    return bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
+}
+
+// Sources of a set-up triangle's vertices.
+struct TriSource {
+    const DevView *views;
+    const DevDraw *draws;
+    const float4 *xv;         // transformed vertices
+    const float4 *xt;         // transformed tangents (or null)
+    const float *clip_buf;    // clipped polygons
+};
+
+// The three vertices of the triangle whose raster record is R: the indexed
+// transformed vertices of an unclipped input triangle (src0 = input triangle,
+// src1 = draw), or vertices 0, f+1, f+2 of a clipped polygon.
+__device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R, CV v[3]) {
+    uint32_t src0 = __float_as_uint(R[3].z), src1 = __float_as_uint(R[3].w);
+    float4 q[3][4];
+    if (src0 & SRC_CLIPPED) {
+        uint32_t e = src0 & 0x07FFFFFFu, f = (src0 >> 27) & 15u;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            uint32_t vi = j == 0 ? 0u : f + (uint32_t)j;
+            const float4 *p = reinterpret_cast<const float4 *>(T.clip_buf + ((size_t)e * 16u + vi) * 16u);
+            q[j][0] = p[0]; q[j][1] = p[1]; q[j][2] = p[2]; q[j][3] = p[3];
+        }
+    } else {
+        const DevDraw &D = T.draws[src1];
+        const DevScene &S = *T.views[D.view].scene;
+        uint32_t lt = src0 - D.in_tri_base;
+        bool tan = (D.flags & SR_DRAW_TANGENTS) && T.xt;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            size_t i = (size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k];
+            q[k][0] = T.xv[3 * i]; q[k][1] = T.xv[3 * i + 1]; q[k][2] = T.xv[3 * i + 2];
+            q[k][3] = tan ? T.xt[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        v[j].cx = q[j][0].x; v[j].cy = q[j][0].y; v[j].cz = q[j][0].z; v[j].cw = q[j][0].w;
+        v[j].wx = q[j][1].x; v[j].wy = q[j][1].y; v[j].wz = q[j][1].z; v[j].nx = q[j][1].w;
+        v[j].ny = q[j][2].x; v[j].nz = q[j][2].y; v[j].u = q[j][2].z; v[j].v = q[j][2].w;
+        v[j].tx = q[j][3].x; v[j].ty = q[j][3].y; v[j].tz = q[j][3].z; v[j].tw = q[j][3].w;
+    }
+}
+
+// The reference packet's shading interpolators (src/renderer.rs:701-754):
+// normal, position/w, u/w, v/w as (a, b - a, c - a), UV derivatives, and the
+// tangent frame's when normal mapped.  Sh[0..6] / Tn[0..2] as read by
+// shade_pixel and alpha_pass.
+__device__ __forceinline__ void tri_shade_attrs(const CV v[3], const float4 *R, bool tangents, float4 Sh[7],
+                                                float4 Tn[3]) {
+    float4 r0 = R[0], r1 = R[1];
+    int32_t px[3] = {__float_as_int(r0.x), __float_as_int(r0.z), __float_as_int(r1.x)};
+    int32_t py[3] = {__float_as_int(r0.y), __float_as_int(r0.w), __float_as_int(r1.y)};
+    float ooa = r1.z;
+    float oow[3], uw[3], vw[3];
+    f3 pw[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) oow[k] = 1.0f / v[k].cw;
+#pragma unroll
     for (int k = 0; k < 3; k++) {
-        uw[k] = tv[k]->u * oow[k];
-        vw[k] = tv[k]->v * oow[k];
-        pw[k] = mk3(tv[k]->wx * oow[k], tv[k]->wy * oow[k], tv[k]->wz * oow[k]);
+        uw[k] = v[k].u * oow[k];
+        vw[k] = v[k].v * oow[k];
+        pw[k] = mk3(v[k].wx * oow[k], v[k].wy * oow[k], v[k].wz * oow[k]);
     }
     float dx1 = (float)wsub(px[1], px[0]), dx2 = (float)wsub(px[2], px[0]);
     float dy1 = (float)wsub(py[1], py[0]), dy2 = (float)wsub(py[2], py[0]);
@@ -597,21 +671,9 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
     float dv_dx = dv1 * dy2 - dv2 * dy1;
     float dv_dy = dv2 * dx1 - dv1 * dx2;
     float scale = ooa * 16.0f;
-
-    uint32_t flags = 0;
-    if (matflags & SR_MAT_ALPHA_TESTED) flags |= SR_GEO_ALPHA;
-    if (matflags & SR_MAT_NORMAL_MAP) flags |= SR_GEO_NORMAL_MAP;
-    O.geo[id] = make_uint4((uint32_t)pmnx | ((uint32_t)pmny << 16), (uint32_t)pmxx | ((uint32_t)pmxy << 16), flags,
-                           (uint32_t)view);
-    float4 *R = O.raster + 4 * (size_t)id;
-    R[0] = make_float4(__int_as_float(px[0]), __int_as_float(py[0]), __int_as_float(px[1]), __int_as_float(py[1]));
-    R[1] = make_float4(__int_as_float(px[2]), __int_as_float(py[2]), ooa, oow[0]);
-    R[2] = make_float4(oow[1] - oow[0], oow[2] - oow[0], zow[0], zow[1] - zow[0]);
-    R[3] = make_float4(zow[2] - zow[0], __int_as_float(material), 0.0f, 0.0f);
-    f3 n0 = mk3(v0->nx, v0->ny, v0->nz), n1 = mk3(v1->nx, v1->ny, v1->nz), n2 = mk3(v2->nx, v2->ny, v2->nz);
+    f3 n0 = mk3(v[0].nx, v[0].ny, v[0].nz), n1 = mk3(v[1].nx, v[1].ny, v[1].nz), n2 = mk3(v[2].nx, v[2].ny, v[2].nz);
     f3 nda = sub3(n1, n0), ndb = sub3(n2, n0);
     f3 pda = sub3(pw[1], pw[0]), pdb = sub3(pw[2], pw[0]);
-    float4 *Sh = O.shade + 8 * (size_t)id;
     Sh[0] = make_float4(n0.x, n0.y, n0.z, nda.x);
     Sh[1] = make_float4(nda.y, nda.z, ndb.x, ndb.y);
     Sh[2] = make_float4(ndb.z, pw[0].x, pw[0].y, pw[0].z);
@@ -619,15 +681,13 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
     Sh[4] = make_float4(pdb.y, pdb.z, uw[0], uw[1] - uw[0]);
     Sh[5] = make_float4(uw[2] - uw[0], vw[0], vw[1] - vw[0], vw[2] - vw[0]);
     Sh[6] = make_float4(du_dx * scale, du_dy * scale, dv_dx * scale, dv_dy * scale);
-    if ((matflags & SR_MAT_NORMAL_MAP) && O.tan) {
-        f3 t0 = mk3(v0->tx, v0->ty, v0->tz), t1 = mk3(v1->tx, v1->ty, v1->tz), t2 = mk3(v2->tx, v2->ty, v2->tz);
+    if (tangents) {
+        f3 t0 = mk3(v[0].tx, v[0].ty, v[0].tz), t1 = mk3(v[1].tx, v[1].ty, v[1].tz), t2 = mk3(v[2].tx, v[2].ty, v[2].tz);
         f3 tda = sub3(t1, t0), tdb = sub3(t2, t0);
-        float4 *Tn = O.tan + 3 * (size_t)id;
         Tn[0] = make_float4(t0.x, t0.y, t0.z, tda.x);
         Tn[1] = make_float4(tda.y, tda.z, tdb.x, tdb.y);
-        Tn[2] = make_float4(tdb.z, v0->tw, v1->tw - v0->tw, v2->tw - v0->tw);
+        Tn[2] = make_float4(tdb.z, v[0].tw, v[1].tw - v[0].tw, v[2].tw - v[0].tw);
     }
-    return bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
 }
 
 __device__ __forceinline__ int wave_max(int v) {
@@ -763,7 +823,8 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
                 cls = clip_classify(c);
             }
             // triangles that cross a plane are set up by k_clip_emit
-            if (cls == -1) r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, O);
+            if (cls == -1)
+                r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, t, (uint32_t)d, O);
         }
         bin_rect_count(views, view, r, O.bin_count);
     }
@@ -804,7 +865,7 @@ __global__ void __launch_bounds__(256) k_clip_emit(const DevView *views, const D
                     v[j].tx = dd.x; v[j].ty = dd.y; v[j].tz = dd.z; v[j].tw = dd.w;
                 }
                 r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], tri_base[t] + f, D.material,
-                                  S.materials[D.material].flags, O);
+                                  S.materials[D.material].flags, SRC_CLIPPED | (f << 27) | e, 0u, O);
             }
         }
         bin_rect_count(views, view, r, O.bin_count);
@@ -1039,7 +1100,7 @@ __device__ __forceinline__ float tri_const(float k, float fx, float fy, float fz
     return b * (1.0f - fz) + b * fz;
 }
 
-__device__ void voxel_lookup(const DevScene &S, f3 p, f3 rgb[4], float &w0, float &w1) {
+__device__ __forceinline__ void voxel_lookup(const DevScene &S, f3 p, f3 rgb[4], float &w0, float &w1) {
     float vx = (p.x - S.vox_min[0]) / S.vox_size[0];
     float vy = (p.y - S.vox_min[1]) / S.vox_size[1];
     float vz = (p.z - S.vox_min[2]) / S.vox_size[2];
@@ -1096,7 +1157,7 @@ __device__ void voxel_lookup(const DevScene &S, f3 p, f3 rgb[4], float &w0, floa
 // PBR shading of one pixel (pbr_shader, src/shader.rs:110-309, opaque path)
 // b1[4], b2[4]: the quad's four barycentric pairs; lane: this pixel's lane.
 // ---------------------------------------------------------------------------
-__device__ f3 shade_pixel(const DevScene &S, const DevView &V, const float4 *R, const float4 *Sh, const float4 *Tn,
+__device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, const float4 *R, const float4 *Sh, const float4 *Tn, bool nmap,
                           const float b1[4], const float b2[4], int lane) {
     float4 r1 = R[1], r2 = R[2], r3 = R[3];
     float oa = r1.w, oda = r2.x, odb = r2.y;
@@ -1117,7 +1178,7 @@ __device__ f3 shade_pixel(const DevScene &S, const DevView &V, const float4 *R, 
     float uvy = ((s5.y + B1 * s5.z) + B2 * s5.w) * w;
     float dd[4] = {s6.x * wv[0], s6.y * wv[1], s6.z * wv[2], s6.w * wv[3]};
     f3 nw = normalize3(in_n);
-    if ((M.flags & SR_MAT_NORMAL_MAP) && Tn) {
+    if ((M.flags & SR_MAT_NORMAL_MAP) && nmap) {
         float4 t0 = Tn[0], t1 = Tn[1], t2 = Tn[2];
         f3 ta = mk3(t0.x, t0.y, t0.z), tda = mk3(t0.w, t1.x, t1.y), tdb = mk3(t1.z, t1.w, t2.x);
         f3 in_t = mk3((ta.x + B1 * tda.x) + B2 * tdb.x, (ta.y + B1 * tda.y) + B2 * tdb.y, (ta.z + B1 * tda.z) + B2 * tdb.z);
@@ -1260,8 +1321,7 @@ struct RasterArgs {
     uint32_t pair_cap;
     const uint4 *geo;
     const float4 *raster;
-    const float4 *shade;
-    const float4 *tan;
+    TriSource src;
     const uint32_t *tri_base;
     uint32_t *out_ids;
     float *out_depth;
@@ -1280,10 +1340,14 @@ __device__ __forceinline__ uint32_t depth_key(float z) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-__device__ __forceinline__ bool alpha_pass(const DevScene &S, const float4 *R, const float4 *Sh, float b1, float b2,
+__device__ __forceinline__ bool alpha_pass(const DevScene &S, const TriSource &T, const float4 *R, float b1, float b2,
                                            float w) {
     int material = __float_as_int(R[3].y);
     const DevMaterial &M = S.materials[material];
+    CV tv[3];
+    float4 Sh[7], Tn[3];
+    load_tri_cv(T, R, tv);
+    tri_shade_attrs(tv, R, false, Sh, Tn);
     float4 s4 = Sh[4], s5 = Sh[5], s6 = Sh[6];
     float u = ((s4.z + b1 * s4.w) + b2 * s5.x) * w;
     float v = ((s5.y + b1 * s5.z) + b2 * s5.w) * w;
@@ -1518,7 +1582,7 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
         float wv = 1.0f / ((oa + b1 * oda) + b2 * odb);
         float z = ((za + b1 * zda) + b2 * zdb) * wv;
         if (z != z) continue;
-        if (alpha && !alpha_pass(S, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id, b1, b2, wv)) continue;
+        if (alpha && !alpha_pass(S, P.src, P.raster + 4 * (size_t)id, b1, b2, wv)) continue;
         unsigned long long key = ((unsigned long long)depth_key(z) << 32) | (unsigned long long)(~id);
         atomicMin(&kq[(s >> 1) * T + (s & 1)], key);
     }
@@ -1692,8 +1756,13 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
         if (on_screen || center) {
             bool covered = has && z != INFINITY;
             if (covered) {
-                color = shade_pixel(S, V, P.raster + 4 * (size_t)id, P.shade + 8 * (size_t)id,
-                                    (P.tan ? P.tan + 3 * (size_t)id : nullptr), qb1, qb2, s);
+                const float4 *R = P.raster + 4 * (size_t)id;
+                CV tv[3];
+                float4 Sh[7], Tn[3];
+                load_tri_cv(P.src, R, tv);
+                bool nmap = (P.geo[id].z & SR_GEO_NORMAL_MAP) != 0;
+                tri_shade_attrs(tv, R, nmap, Sh, Tn);
+                color = shade_pixel(S, V, R, Sh, Tn, nmap, qb1, qb2, s);
             } else {
                 color = skybox_pixel(S, V, tx0 + 2 * qx, ty0 + 2 * qy, sx, sy);
             }
@@ -1780,7 +1849,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
                                &a.status->total_records, st);
     if (e != hipSuccess) return e;
     SetupOut so;
-    so.geo = a.rec_geo; so.raster = a.rec_raster; so.shade = a.rec_shade; so.tan = a.rec_tan;
+    so.geo = a.rec_geo; so.raster = a.rec_raster;
     so.bin_count = a.bin_count; so.cap = a.rec_cap; so.status = a.status; so.T = a.tile_size;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
@@ -1797,7 +1866,9 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     RasterArgs P;
     P.views = a.views; P.num_views = a.num_views; P.total_items = a.total_items;
     P.bin_start = a.bin_start; P.bin_end = a.bin_end; P.list = a.list; P.pair_cap = a.pair_cap;
-    P.geo = a.rec_geo; P.raster = a.rec_raster; P.shade = a.rec_shade; P.tan = a.any_tangents ? a.rec_tan : nullptr;
+    P.geo = a.rec_geo; P.raster = a.rec_raster;
+    P.src.views = a.views; P.src.draws = a.draws; P.src.xv = a.xv; P.src.xt = a.any_tangents ? a.xt : nullptr;
+    P.src.clip_buf = a.clip_buf;
     P.tri_base = a.tri_base; P.out_ids = a.out_ids; P.out_depth = a.out_depth; P.out_color = a.out_color;
     P.color_plane = a.color_plane; P.out_rgba = a.out_rgba; P.out_lum = a.out_lum; P.status = a.status;
     P.center_ids = a.center_ids;

@@ -204,7 +204,7 @@ void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     DevBuf *bufs[] = {&views, &draws, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
-                      &list, &rec_geo, &rec_raster, &rec_shade, &rec_tan, &status, &center, &clip_queue, &clip_buf};
+                      &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf};
     for (DevBuf *b : bufs) b->release();
     stage.release();
     status_host.release();
@@ -244,8 +244,6 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     for (int attempt = 0; attempt < 4; attempt++) {
         if ((r = rec_geo.ensure((size_t)rec_cap * 16))) return r;
         if ((r = rec_raster.ensure((size_t)rec_cap * 64))) return r;
-        if ((r = rec_shade.ensure((size_t)rec_cap * 128))) return r;
-        if (any_tangents && (r = rec_tan.ensure((size_t)rec_cap * 48))) return r;
         if ((r = list.ensure((size_t)pair_cap * 4))) return r;
         if ((r = clip_queue.ensure((size_t)clip_cap * 4))) return r;
         if ((r = clip_buf.ensure((size_t)clip_cap * 16 * 16 * 4))) return r;
@@ -275,8 +273,6 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.list = (uint32_t *)list.ptr;
         a.rec_geo = (uint4 *)rec_geo.ptr;
         a.rec_raster = (float4 *)rec_raster.ptr;
-        a.rec_shade = (float4 *)rec_shade.ptr;
-        a.rec_tan = any_tangents ? (float4 *)rec_tan.ptr : nullptr;
         a.rec_cap = rec_cap;
         a.pair_cap = pair_cap;
         a.clip_queue = (uint32_t *)clip_queue.ptr;

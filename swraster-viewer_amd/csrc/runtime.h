@@ -57,7 +57,7 @@ struct Pipeline {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     DevBuf views, draws, xv, xt, tri_counts, tri_base, bin_count, bin_start, bin_end, partials, list;
-    DevBuf rec_geo, rec_raster, rec_shade, rec_tan, status, center, clip_queue, clip_buf;
+    DevBuf rec_geo, rec_raster, status, center, clip_queue, clip_buf;
     PinBuf stage, status_host;
     uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0;
     uint32_t last_records = 0, last_pairs = 0;
