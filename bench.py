@@ -109,8 +109,14 @@ def run_batched(args):
             sr.check(L.sr_renderer_update_auto_exposure(r.handle, 0.0))
             sr.check(L.sr_renderer_blit_to_device(r.handle, targets[(spec["width"], spec["height"])]))
 
-    batch = sr.Batch(tile_size=args.tile)
     bviews = [(scenes[k], cam, spec["width"], spec["height"], 0.0) for (k, spec), cam in zip(views, cams)]
+    import swraster_viewer_amd.autotune as autotune
+    if args.tile:
+        tuned = {"tile_size": args.tile, "persistent_blocks": 2048, "cached": None}
+    else:
+        tuned = autotune.tune(sr, bviews, reps=1)
+    batch = sr.Batch(tile_size=tuned["tile_size"])
+    batch.configure(persistent_blocks=tuned["persistent_blocks"], use_graph=not args.no_graph)
 
     def batched():
         batch.render(bviews)
@@ -138,8 +144,10 @@ def run_batched(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded generator, swraster-viewer_amd/scenes.py make_c4_pool/make_c4_views)",
         "config": {"workload": "c4_batched_heightfields", "views": n, "pool_scenes": len(pool),
-                   "triangles": tris, "pixels": px, "tile_size": args.tile or 16,
+                   "triangles": tris, "pixels": px, "tile_size": tuned["tile_size"],
+                   "persistent_blocks": tuned["persistent_blocks"], "hip_graph": not args.no_graph,
                    "step": "one batch of all views (batched) vs the same views as plain frames"},
+        "autotune": {k: tuned.get(k) for k in ("tile_size", "persistent_blocks", "ms", "cached", "table")},
         "plain_frames_per_s": round(n / times["plain"], 2),
         "plain_ms": round(times["plain"] * 1e3, 3),
         "batched_vs_plain": round(times["plain"] / times["batched"], 3),
@@ -156,6 +164,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--views", type=int, default=256, help="c4: views per batch")
+    ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch without a hipGraph")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -191,6 +191,7 @@ _SIGS = {
     "sr_renderer_device_buffers": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "sr_batch_create": (C.c_int, [C.POINTER(sr_batch_opts), C.POINTER(_P)]),
     "sr_batch_destroy": (C.c_int, [_P]),
+    "sr_batch_configure": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "sr_batch_render": (C.c_int, [_P, C.POINTER(sr_view), C.c_int32]),
     "sr_batch_read_vbuffer": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "sr_batch_read_pixels": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint32)]),
@@ -601,6 +602,10 @@ class Batch:
         self.handle = C.c_void_p()
         check(lib().sr_batch_create(C.byref(o), C.byref(self.handle)))
         self.sizes = []
+
+    def configure(self, persistent_blocks=0, use_graph=False):
+        """Persistent raster workgroup count (0 keeps it) and hipGraph use."""
+        check(lib().sr_batch_configure(self.handle, int(persistent_blocks), int(use_graph)))
 
     def render(self, views):
         arr = (sr_view * len(views))()

@@ -278,6 +278,14 @@ SR_API int sr_batch_create(const sr_batch_opts *opts, sr_batch **out) {
     return SR_OK;
 }
 
+SR_API int sr_batch_configure(sr_batch *b, int32_t persistent_blocks, int32_t use_graph) {
+    if (!b) return fail(SR_ERR_INVALID_ARGUMENT, "null batch");
+    if (persistent_blocks < 0) return fail(SR_ERR_INVALID_ARGUMENT, "negative workgroup count");
+    if (persistent_blocks > 0) b->persistent = persistent_blocks;
+    b->use_graph = use_graph != 0;
+    return SR_OK;
+}
+
 SR_API int sr_batch_destroy(sr_batch *b) {
     if (!b) return SR_OK;
     hipSetDevice(b->device);
@@ -326,7 +334,7 @@ SR_API int sr_batch_render(sr_batch *b, const sr_view *views, int32_t n) {
     if ((rc = b->rgba.ensure(fb * 4))) return rc;
     if ((rc = b->lum.ensure((size_t)lum * 4))) return rc;
     rc = b->pipe.run(hv, draws, xv, in_tri, items, tangents, (uint32_t *)b->ids.ptr, (float *)b->depth.ptr, nullptr, 0,
-                     (uint32_t *)b->rgba.ptr, (float *)b->lum.ptr, b->persistent, true);
+                     (uint32_t *)b->rgba.ptr, (float *)b->lum.ptr, b->persistent, true, b->use_graph);
     if (rc) return rc;
     b->stats.input_triangles = in_tri;
     b->stats.setup_triangles = b->pipe.last_records;

@@ -203,6 +203,7 @@ int Pipeline::init(int dev, int tile) {
 void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
+    drop_graph();
     DevBuf *bufs[] = {&views, &draws, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
                       &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf};
     for (DevBuf *b : bufs) b->release();
@@ -214,9 +215,59 @@ void Pipeline::destroy() {
     stream = nullptr;
 }
 
+void Pipeline::drop_graph() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    memset(&graph_key, 0, sizeof(graph_key));
+}
+
+// The frame's whole launch sequence (uploads of the view and draw tables from
+// pinned staging, every kernel, the status read-back) as one executable
+// graph, re-captured only when the launch shape or a buffer changes.  The
+// sequence clears memory with fill kernels only (no memset nodes).
+int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
+    GraphKey k;
+    memset(&k, 0, sizeof(k));
+    k.n[0] = (uint64_t)a.num_views; k.n[1] = (uint64_t)a.num_draws; k.n[2] = a.total_xv; k.n[3] = a.total_in;
+    k.n[4] = a.total_items; k.n[5] = (uint64_t)a.tile_size; k.n[6] = (uint64_t)a.any_tangents; k.n[7] = a.rec_cap;
+    k.n[8] = a.pair_cap; k.n[9] = a.clip_cap; k.n[10] = (uint64_t)a.persistent_blocks; k.n[11] = vbytes;
+    k.n[12] = dbytes; k.n[13] = a.color_plane;
+    const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
+                          a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
+                          a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
+                          status_host.ptr};
+    static_assert(sizeof(ptrs) / sizeof(ptrs[0]) <= sizeof(k.p) / sizeof(k.p[0]), "graph key too small");
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) k.p[i] = (uint64_t)(uintptr_t)ptrs[i];
+    if (!graph_exec || memcmp(&k, &graph_key, sizeof(k)) != 0) {
+        drop_graph();
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipMemcpyAsync((void *)a.views, stage.ptr, vbytes, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess && a.num_draws > 0)
+            e = hipMemcpyAsync((void *)a.draws, (char *)stage.ptr + vbytes, dbytes, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = launch_frame(a, stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(status_host.ptr, a.status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(stream, &g);
+        if (e == hipSuccess) e = ce;
+        if (e == hipSuccess) e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+        if (g) hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            graph_exec = nullptr;
+            return fail(SR_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        }
+        graph_key = k;
+    }
+    HIP_TRY(hipEventRecord(ev[0], stream));
+    HIP_TRY(hipGraphLaunch(graph_exec, stream));
+    HIP_TRY(hipEventRecord(ev[3], stream));
+    return SR_OK;
+}
+
 int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd, uint32_t total_xv, uint32_t total_in,
                   uint32_t total_items, bool any_tangents, uint32_t *out_ids, float *out_depth, float *out_color,
-                  uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks, bool sync) {
+                  uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks, bool sync,
+                  bool use_graph) {
     HIP_TRY(hipSetDevice(device));
     size_t vbytes = hv.size() * sizeof(DevView), dbytes = std::max<size_t>(hd.size(), 1) * sizeof(DevDraw);
     int r;
@@ -247,10 +298,6 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         if ((r = list.ensure((size_t)pair_cap * 4))) return r;
         if ((r = clip_queue.ensure((size_t)clip_cap * 4))) return r;
         if ((r = clip_buf.ensure((size_t)clip_cap * 16 * 16 * 4))) return r;
-        HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, vbytes, hipMemcpyHostToDevice, stream));
-        if (!hd.empty())
-            HIP_TRY(hipMemcpyAsync(draws.ptr, (char *)stage.ptr + vbytes, hd.size() * sizeof(DevDraw),
-                                   hipMemcpyHostToDevice, stream));
         FrameArgs a;
         memset(&a, 0, sizeof(a));
         a.views = (const DevView *)views.ptr;
@@ -287,9 +334,17 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.out_lum = out_lum;
         a.center_ids = (uint32_t *)center.ptr;
         a.persistent_blocks = persistent_blocks;
-        for (int i = 0; i < 5; i++) a.events[i] = ev[i];
-        HIP_TRY(launch_frame(a, stream));
-        HIP_TRY(hipMemcpyAsync(status_host.ptr, status.ptr, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+        if (use_graph) {
+            if ((r = launch_graph(a, vbytes, hd.empty() ? 0 : hd.size() * sizeof(DevDraw)))) return r;
+        } else {
+            HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, vbytes, hipMemcpyHostToDevice, stream));
+            if (!hd.empty())
+                HIP_TRY(hipMemcpyAsync(draws.ptr, (char *)stage.ptr + vbytes, hd.size() * sizeof(DevDraw),
+                                       hipMemcpyHostToDevice, stream));
+            for (int i = 0; i < 5; i++) a.events[i] = ev[i];
+            HIP_TRY(launch_frame(a, stream));
+            HIP_TRY(hipMemcpyAsync(status_host.ptr, status.ptr, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+        }
         if (!sync) return SR_OK;
         HIP_TRY(hipStreamSynchronize(stream));
         const DevStatus &s = *(const DevStatus *)status_host.ptr;

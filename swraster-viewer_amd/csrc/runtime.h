@@ -50,6 +50,8 @@ struct HostNode {
     Sphere bs_world;
 };
 
+struct FrameArgs;
+
 // Per-frame pipeline state (transient buffers, stream, events).
 struct Pipeline {
     int device = 0;
@@ -61,12 +63,19 @@ struct Pipeline {
     PinBuf stage, status_host;
     uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0;
     uint32_t last_records = 0, last_pairs = 0;
+    struct GraphKey {
+        uint64_t n[16];
+        uint64_t p[32];
+    } graph_key = {};
+    hipGraphExec_t graph_exec = nullptr;
     int init(int device, int tile);
     void destroy();
+    void drop_graph();
+    int launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes);
     int run(const std::vector<DevView> &views, const std::vector<DevDraw> &draws, uint32_t total_xv,
             uint32_t total_in, uint32_t total_items, bool any_tangents, uint32_t *out_ids, float *out_depth,
             float *out_color, uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks,
-            bool sync);
+            bool sync, bool use_graph = false);
 };
 
 }  // namespace sr

@@ -191,6 +191,50 @@ def test_batch_matches_plain(sr, scenes, c1):
         assert np.array_equal(b.read_pixels(i), p["px"])
 
 
+def test_batch_graph_replays_match_eager(sr, scenes, c1):
+    """The batch's launch sequence captured as a hipGraph, replayed three
+    times with eager frames and synchronisations in between: every replay
+    gives the eager batch's visibility buffers and pixels bit for bit."""
+    scene, spec, gscene = c1
+    sizes = [(320, 240), (97, 61), (640, 360), (128, 128), (333, 202), (64, 40)]
+    views = []
+    for i, (w, h) in enumerate(sizes):
+        s = dict(spec)
+        s["yaw"] = 0.15 * i - 0.3
+        views.append((gscene, scenes.make_camera(s, w, h), w, h, 0.0))
+    eager = sr.Batch()
+    eager.render(views)
+    want = [(eager.read_vbuffer(i), eager.read_pixels(i)) for i in range(len(views))]
+    eager.close()
+    g = sr.Batch()
+    g.configure(use_graph=True)
+    other = sr.Renderer(200, 150)
+    for rep in range(3):
+        g.render(views)
+        for i in range(len(views)):
+            (ids, depth), px = g.read_vbuffer(i), g.read_pixels(i)
+            (wids, wdepth), wpx = want[i]
+            assert np.array_equal(ids, wids), f"replay {rep} view {i}: ids differ"
+            assert np.array_equal(bits(depth), bits(wdepth)), f"replay {rep} view {i}: depth differs"
+            assert np.array_equal(px, wpx), f"replay {rep} view {i}: pixels differ"
+        other.render_scene(gscene, scenes.make_camera(spec, 200, 150))     # eager work between replays
+        other.blit_to_buffer()
+    other.close()
+    g.close()
+
+
+def test_autotuner_picks_a_measured_config(sr, scenes, c1, tmp_path):
+    import importlib
+    at = importlib.import_module(sr.__name__ + ".autotune")
+    scene, spec, gscene = c1
+    views = [(gscene, scenes.make_camera(spec, 320, 240), 320, 240, 0.0)] * 4
+    r = at.tune(sr, views, tiles=(8, 16), workgroups=(256, 1024), reps=1, cache_path=str(tmp_path / "t.json"))
+    assert r["tile_size"] in (8, 16) and r["persistent_blocks"] in (256, 1024)
+    assert len(r["table"]) == 4 and r["ms"] == min(t["ms"] for t in r["table"])
+    again = at.tune(sr, views, tiles=(8, 16), workgroups=(256, 1024), reps=1, cache_path=str(tmp_path / "t.json"))
+    assert again["cached"] and again["tile_size"] == r["tile_size"]
+
+
 def test_gpu_matches_committed_c1_frame(sr, scenes, env):
     """The GPU reproduces the committed oracle frame (tests/golden/) without
     running the oracle."""
