@@ -644,3 +644,72 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Render server (sr_server_*): requests are batched continuously as they
+    arrive, rendered on `streams` overlapping pipeline slots, and admitted by
+    framebuffer-plus-bin memory."""
+
+    def __init__(self, admission_bytes=0, max_batch_views=0, streams=0, tile_size=0, device=0):
+        o = sr_server_opts()
+        o.admission_bytes, o.max_batch_views, o.streams = admission_bytes, max_batch_views, streams
+        o.tile_size, o.device = tile_size, device
+        self.handle = C.c_void_p()
+        check(lib().sr_server_create(C.byref(o), C.byref(self.handle)))
+        self.sizes = {}
+        self._scenes = []
+
+    def add_scene(self, scene):
+        idx = C.c_int32()
+        check(lib().sr_server_add_scene(self.handle, scene.handle, C.byref(idx)))
+        self._scenes.append(scene)
+        return idx.value
+
+    def submit(self, scene_index, camera, width, height, exposure=0.0, tag=0):
+        r = sr_request()
+        r.scene, r.camera, r.width, r.height, r.exposure, r.user_tag = scene_index, camera.c, width, height, exposure, tag
+        rid = C.c_uint64()
+        check(lib().sr_server_submit(self.handle, C.byref(r), C.byref(rid)))
+        self.sizes[rid.value] = (width, height)
+        return rid.value
+
+    def poll(self, max_items=256):
+        buf = (sr_completion * max_items)()
+        n = C.c_int32()
+        check(lib().sr_server_poll(self.handle, buf, max_items, C.byref(n)))
+        return [{k: getattr(buf[i], k) for k, _ in sr_completion._fields_} for i in range(n.value)]
+
+    def read_pixels(self, rid):
+        w, h = self.sizes[rid]
+        px = np.zeros((h, w), np.uint32)
+        check(lib().sr_server_read_pixels(self.handle, rid, _uptr(px)))
+        return px
+
+    def read_vbuffer(self, rid):
+        w, h = self.sizes[rid]
+        ids = np.empty((h, w), np.uint32)
+        depth = np.empty((h, w), np.float32)
+        check(lib().sr_server_read_vbuffer(self.handle, rid, _uptr(ids), _fptr(depth)))
+        return ids, depth
+
+    def release(self, rid):
+        check(lib().sr_server_release(self.handle, rid))
+        self.sizes.pop(rid, None)
+
+    def drain(self):
+        check(lib().sr_server_drain(self.handle))
+
+    def clock(self):
+        return lib().sr_server_clock(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().sr_server_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

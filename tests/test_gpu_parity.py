@@ -235,6 +235,53 @@ def test_autotuner_picks_a_measured_config(sr, scenes, c1, tmp_path):
     assert again["cached"] and again["tile_size"] == r["tile_size"]
 
 
+def test_server_same_alone_and_in_the_mix(sr, scenes, c1, env):
+    """Requests served one at a time and the same requests submitted together
+    (batched continuously over two overlapping slots, with a small admission
+    budget that splits them into several batches) produce identical
+    visibility buffers and pixels, and equal the plain renderer's."""
+    scene, spec, gscene = c1
+    c3s, c3spec = scenes.make_c3(width=640, height=360, triangles=5000, primitives=16)
+    c3s.prepare(env, cache_key=None, grid=16)
+    g3 = sr.Scene(c3s)
+    reqs = []
+    sizes = [(320, 240), (640, 360), (97, 61), (256, 144), (640, 480), (128, 72), (333, 202), (160, 90)]
+    for i, (w, h) in enumerate(sizes):
+        sp = dict(spec if i % 3 else c3spec)
+        sp["yaw"] = sp["yaw"] + 0.07 * i
+        reqs.append((0 if i % 3 else 1, scenes.make_camera(sp, w, h), w, h))
+    srv = sr.Server(admission_bytes=64 << 20, max_batch_views=3, streams=2)
+    idx = [srv.add_scene(gscene), srv.add_scene(g3)]
+    alone = []
+    for k, cam, w, h in reqs:
+        rid = srv.submit(idx[k], cam, w, h)
+        srv.drain()
+        alone.append((srv.read_vbuffer(rid), srv.read_pixels(rid)))
+        srv.release(rid)
+    rids = [srv.submit(idx[k], cam, w, h, tag=i) for i, (k, cam, w, h) in enumerate(reqs)]
+    srv.drain()
+    done = srv.poll(64)
+    done += srv.poll(64)
+    assert sorted(d["request_id"] for d in done if d["request_id"] in rids) == sorted(rids)
+    assert all(d["status"] == 0 for d in done)
+    for i, rid in enumerate(rids):
+        (ids, depth), px = srv.read_vbuffer(rid), srv.read_pixels(rid)
+        (aids, adepth), apx = alone[i]
+        assert np.array_equal(ids, aids) and np.array_equal(bits(depth), bits(adepth)), f"request {i}: vbuffer"
+        assert np.array_equal(px, apx), f"request {i}: pixels"
+        k, cam, w, h = reqs[i]
+        r = sr.Renderer(w, h)
+        r.render_scene([gscene, g3][k], cam)
+        pids, pdepth = r.read_vbuffer()
+        r.update_auto_exposure(0.0)
+        ppx = r.blit_to_buffer()
+        r.close()
+        assert np.array_equal(ids, pids) and np.array_equal(bits(depth), bits(pdepth)), f"request {i}: vs plain"
+        assert np.array_equal(px[: h & ~1], ppx[: h & ~1]), f"request {i}: pixels vs plain"
+        srv.release(rid)
+    srv.close()
+
+
 def test_gpu_matches_committed_c1_frame(sr, scenes, env):
     """The GPU reproduces the committed oracle frame (tests/golden/) without
     running the oracle."""
