@@ -236,7 +236,7 @@ typedef struct sr_batch_stats {
 
 typedef struct sr_server_opts {
     uint64_t admission_bytes; /* framebuffer + bin budget (0 = 64 GiB) */
-    int32_t max_batch_views;  /* 0 = 64 */
+    int32_t max_batch_views;  /* 0 = 16 */
     int32_t streams;          /* batches in flight (0 = 2) */
     int32_t tile_size;        /* 0 = tuned default */
     int32_t device;

@@ -145,8 +145,9 @@ struct sr_server {
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
     void run();
-    bool launch_batch(Slot &s);                 // called with mu held
-    void retire(Slot &s);                       // called with mu held
+    // both are called with mu held and release it around host-heavy work
+    bool launch_batch(Slot &s, std::unique_lock<std::mutex> &lk);
+    void retire(Slot &s, std::unique_lock<std::mutex> &lk);
     void complete(const std::vector<uint64_t> &ids, int rc);
 };
 
@@ -183,8 +184,8 @@ static uint64_t bin_estimate(const sr_scene *s, const sr_request &r) {
     return s->triangles * 96 + (uint64_t)r.width * r.height * 4 + s->vertices * 48;
 }
 
-bool sr_server::launch_batch(Slot &s) {
-    // take requests FIFO while they fit the view limit and the budget
+bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
+    // (locked) take requests FIFO while they fit the view limit and the budget
     std::vector<uint64_t> take;
     uint64_t bytes = 0, bins = 0;
     for (uint64_t id : pending) {
@@ -216,67 +217,87 @@ bool sr_server::launch_batch(Slot &s) {
     regions[rg].off = off;
     regions[rg].size = bytes;
     regions[rg].refs = (int)take.size();
-
     int n = (int)take.size();
+    std::vector<sr_request> rq(n);
+    std::vector<const sr_scene *> rs(n);
+    for (int i = 0; i < n; i++) {
+        rq[i] = reqs[take[i]].r;
+        rs[i] = scenes[rq[i].scene];
+    }
+    s.reqs = take;
+    s.busy = true;                  // the slot and these requests now belong to this launch
+    inflight++;
+    char *base = (char *)arena_mem.ptr + off;
+    double t = now();
+
+    // (unlocked) host set-up and the asynchronous launch
+    lk.unlock();
+    std::vector<int> st(n, SR_OK);
+    std::vector<uint64_t> px_off(n, 0);
     s.hv.assign(n, DevView());
     s.draws.clear();
     s.xv = s.in_tri = s.items = 0;
     s.tangents = false;
     uint64_t px_total = 0;
     uint32_t lum_total = 0;
-    double t = now();
     for (int i = 0; i < n; i++) {
-        Request &q = reqs[take[i]];
-        const sr_scene *sc = scenes[q.r.scene];
         uint32_t tri0 = s.in_tri, xv0 = s.xv;
         size_t nd = s.draws.size();
-        int rc = build_draws(sc, q.r.camera, i, s.in_tri, s.xv, s.draws);
+        int rc = build_draws(rs[i], rq[i].camera, i, s.in_tri, s.xv, s.draws);
         if (rc) {                      // the view renders nothing; the request reports rc
-            q.status = rc;
+            st[i] = rc;
             s.draws.resize(nd);
             s.in_tri = tri0;
             s.xv = xv0;
         }
-        fill_view(s.hv[i], sc, q.r.camera, q.r.width, q.r.height, tile, s.items, tri0, px_total, lum_total,
-                  q.r.exposure > 0.0f ? q.r.exposure : DEFAULT_EXPOSURE, 1u);
-        q.region = rg;
-        q.px_off = px_total;
-        q.t_start = t;
+        fill_view(s.hv[i], rs[i], rq[i].camera, rq[i].width, rq[i].height, tile, s.items, tri0, px_total, lum_total,
+                  rq[i].exposure > 0.0f ? rq[i].exposure : DEFAULT_EXPOSURE, 1u);
+        px_off[i] = px_total;
         s.items += (uint32_t)(s.hv[i].tiles_x * s.hv[i].tiles_y);
         lum_total += (uint32_t)(s.hv[i].ref_tiles_x * s.hv[i].ref_tiles_y);
-        px_total += (uint64_t)q.r.width * q.r.height;
-        s.tangents = s.tangents || sc->any_tangents;
+        px_total += (uint64_t)rq[i].width * rq[i].height;
+        s.tangents = s.tangents || rs[i]->any_tangents;
     }
-    for (uint64_t id : take) reqs[id].plane = px_total;
-    char *base = (char *)arena_mem.ptr + off;
     s.ids = (uint32_t *)base;
     s.depth = (float *)(base + px_total * 4);
     s.rgba = (uint32_t *)(base + px_total * 8);
     s.lum = (float *)(base + px_total * 12);
-    s.reqs = take;
     int rc = s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth, nullptr, 0, s.rgba, s.lum,
                         persistent, false);
     if (rc == SR_OK && hipEventRecord(s.done, s.pipe.stream) != hipSuccess) rc = fail(SR_ERR_HIP, "event record");
+    if (rc != SR_OK) hipStreamSynchronize(s.pipe.stream);
+    lk.lock();
+
+    // (locked) publish
+    for (int i = 0; i < n; i++) {
+        Request &q = reqs[take[i]];
+        q.region = rg;
+        q.px_off = px_off[i];
+        q.plane = px_total;
+        q.t_start = t;
+        q.status = st[i];
+    }
     if (rc != SR_OK) {
-        hipStreamSynchronize(s.pipe.stream);
         complete(take, rc);
         s.reqs.clear();
-        return true;
+        s.busy = false;
+        inflight--;
     }
-    s.busy = true;
-    inflight++;
     return true;
 }
 
-void sr_server::retire(Slot &s) {
+void sr_server::retire(Slot &s, std::unique_lock<std::mutex> &lk) {
     // a capacity overflow observed after the fact: re-run this batch
     // synchronously (the pipeline grows its buffers and repeats the frame)
     const DevStatus &st = *(const DevStatus *)s.pipe.status_host.ptr;
     int rc = SR_OK;
     if (st.overflow_records || st.overflow_pairs || st.overflow_clip || st.total_records > s.pipe.rec_cap ||
-        st.total_pairs > s.pipe.pair_cap || st.clip_overflow)
+        st.total_pairs > s.pipe.pair_cap || st.clip_overflow) {
+        lk.unlock();
         rc = s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth, nullptr, 0, s.rgba,
                         s.lum, persistent, true);
+        lk.lock();
+    }
     complete(s.reqs, rc);
     s.reqs.clear();
     s.busy = false;
@@ -290,13 +311,13 @@ void sr_server::run() {
     while (!stop) {
         bool progressed = false;
         for (Slot &s : slots) {
-            if (s.busy && hipEventQuery(s.done) == hipSuccess) {
-                retire(s);
+            if (s.busy && !s.reqs.empty() && hipEventQuery(s.done) == hipSuccess) {
+                retire(s, lk);
                 progressed = true;
             }
         }
         for (Slot &s : slots) {
-            if (!s.busy && !pending.empty() && launch_batch(s)) progressed = true;
+            if (!s.busy && !pending.empty() && launch_batch(s, lk)) progressed = true;
         }
         if (progressed) {
             cv_done.notify_all();
@@ -326,7 +347,7 @@ SR_API int sr_server_create(const sr_server_opts *opts, sr_server **out) {
     sr_server *s = new sr_server();
     s->device = dev;
     s->budget = opts && opts->admission_bytes ? opts->admission_bytes : (uint64_t)64 << 30;
-    s->max_views = opts && opts->max_batch_views > 0 ? opts->max_batch_views : 64;
+    s->max_views = opts && opts->max_batch_views > 0 ? opts->max_batch_views : 16;
     int nslots = opts && opts->streams > 0 ? opts->streams : 2;
     int tile = opts ? opts->tile_size : 0;
     s->tile = (tile == 8 || tile == 16 || tile == 32) ? tile : 16;
