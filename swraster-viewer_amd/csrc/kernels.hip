@@ -1403,7 +1403,7 @@ __device__ __forceinline__ bool block_out(const TriSetup &ts, int bx, int by) {
 // the looser of the row's two sample lines; the bounds are evaluated in
 // double with a slack far above its rounding error.  The argument needs
 // SX = 32 A and SY = 32 B (no i32 wrap), else the full rows are kept.
-template <int T>
+template <int T, int SW>
 __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint32_t gflags, int tx0, int ty0,
                                            uint32_t *st, int lane) {
     using SG = Stage<T>;
@@ -1414,22 +1414,22 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
     int nqy = qy1 > qy0 ? (qy1 - qy0 + 1) >> 1 : 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        st[(F_A + k) * 64 + lane] = __float_as_uint(ts.A[k]);
-        st[(F_B + k) * 64 + lane] = __float_as_uint(ts.B[k]);
-        st[(F_C + k) * 64 + lane] = __float_as_uint(ts.C[k]);
-        st[(F_SX + k) * 64 + lane] = __float_as_uint(ts.SX[k]);
-        st[(F_SY + k) * 64 + lane] = __float_as_uint(ts.SY[k]);
+        st[(F_A + k) * SW + lane] = __float_as_uint(ts.A[k]);
+        st[(F_B + k) * SW + lane] = __float_as_uint(ts.B[k]);
+        st[(F_C + k) * SW + lane] = __float_as_uint(ts.C[k]);
+        st[(F_SX + k) * SW + lane] = __float_as_uint(ts.SX[k]);
+        st[(F_SY + k) * SW + lane] = __float_as_uint(ts.SY[k]);
     }
-    st[F_OOA * 64 + lane] = __float_as_uint(ts.ooa);
-    st[F_OA * 64 + lane] = __float_as_uint(ts.oa);
-    st[F_ODA * 64 + lane] = __float_as_uint(ts.oda);
-    st[F_ODB * 64 + lane] = __float_as_uint(ts.odb);
-    st[F_ZA * 64 + lane] = __float_as_uint(ts.za);
-    st[F_ZDA * 64 + lane] = __float_as_uint(ts.zda);
-    st[F_ZDB * 64 + lane] = __float_as_uint(ts.zdb);
-    st[F_ID * 64 + lane] = id;
-    st[F_ORG * 64 + lane] = (uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16);
-    st[F_Q0 * 64 + lane] = (uint32_t)qx0 | ((uint32_t)qy0 << 16);
+    st[F_OOA * SW + lane] = __float_as_uint(ts.ooa);
+    st[F_OA * SW + lane] = __float_as_uint(ts.oa);
+    st[F_ODA * SW + lane] = __float_as_uint(ts.oda);
+    st[F_ODB * SW + lane] = __float_as_uint(ts.odb);
+    st[F_ZA * SW + lane] = __float_as_uint(ts.za);
+    st[F_ZDA * SW + lane] = __float_as_uint(ts.zda);
+    st[F_ZDB * SW + lane] = __float_as_uint(ts.zdb);
+    st[F_ID * SW + lane] = id;
+    st[F_ORG * SW + lane] = (uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16);
+    st[F_Q0 * SW + lane] = (uint32_t)qx0 | ((uint32_t)qy0 << 16);
 
     // rejected 16px blocks among the (at most 3x3) that meet the tile
     uint32_t flags = (ts.coarse ? 1u : 0u) | ((gflags & SR_GEO_ALPHA) ? 2u : 0u);
@@ -1442,7 +1442,7 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
                 if (block_out(ts, ts.ox0 + 16 * kx, ts.oy0 + 16 * ky)) mask |= 1u << ((ky - ky0) * 3 + (kx - kx0));
         flags |= (mask << 8) | ((uint32_t)kx0 << 20) | ((uint32_t)ky0 << 24);
     }
-    st[F_FLAGS * 64 + lane] = flags;
+    st[F_FLAGS * SW + lane] = flags;
 
     bool spans = nqx * nqy > 4;
 #pragma unroll
@@ -1500,11 +1500,11 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
         cw |= (uint32_t)cum << (16 * (r & 1));
         lw |= (uint32_t)l << (4 * (r & 7));
         if (r & 1) {
-            st[(SG::CUM + r / 2) * 64 + lane] = cw;
+            st[(SG::CUM + r / 2) * SW + lane] = cw;
             cw = 0u;
         }
         if ((r & 7) == 7 || r == NR - 1) {
-            st[(SG::LO + r / 8) * 64 + lane] = lw;
+            st[(SG::LO + r / 8) * SW + lane] = lw;
             lw = 0u;
         }
     }
@@ -1515,26 +1515,26 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
 // reference's block choice and rejection, then the four samples' edge
 // chains (src/tilerasterizer.rs:220-381); covered samples with a non-NaN
 // depth enter the 64-bit minimum.
-template <int T>
+template <int T, int SW>
 __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P, const uint32_t *st, int o, int local,
                                           int tx0, int ty0, unsigned long long *keys) {
     using SG = Stage<T>;
     constexpr int NR = SG::NR;
-#define SF(f) __uint_as_float(st[(f) * 64 + o])
+#define SF(f) __uint_as_float(st[(f) * SW + o])
     // row = number of rows whose inclusive count is <= local
     int row = 0;
 #pragma unroll
     for (int r = 0; r < NR; r += 2) {
-        uint32_t cw = st[(SG::CUM + r / 2) * 64 + o];
+        uint32_t cw = st[(SG::CUM + r / 2) * SW + o];
         row += ((int)(cw & 0xFFFFu) <= local) + ((int)(cw >> 16) <= local);
     }
     int before = 0;
-    if (row > 0) before = (int)((st[(SG::CUM + (row - 1) / 2) * 64 + o] >> (16 * ((row - 1) & 1))) & 0xFFFFu);
-    int col = (int)((st[(SG::LO + row / 8) * 64 + o] >> (4 * (row & 7))) & 15u) + (local - before);
-    uint32_t q0 = st[F_Q0 * 64 + o];
+    if (row > 0) before = (int)((st[(SG::CUM + (row - 1) / 2) * SW + o] >> (16 * ((row - 1) & 1))) & 0xFFFFu);
+    int col = (int)((st[(SG::LO + row / 8) * SW + o] >> (4 * (row & 7))) & 15u) + (local - before);
+    uint32_t q0 = st[F_Q0 * SW + o];
     int qpx = (int)(q0 & 0xFFFFu) + 2 * col, qpy = (int)(q0 >> 16) + 2 * row;
 
-    uint32_t org = st[F_ORG * 64 + o], flags = st[F_FLAGS * 64 + o];
+    uint32_t org = st[F_ORG * SW + o], flags = st[F_FLAGS * SW + o];
     int ox0 = (int)(org & 0xFFFFu), oy0 = (int)(org >> 16);
     int bx = ox0, by = oy0;
     if (flags & 1u) {
@@ -1570,7 +1570,7 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
             for (int s = 0; s < 4; s++) w[e][s] = chain_serial(v[e][s], sy, sx, j, i);
         }
     }
-    uint32_t id = st[F_ID * 64 + o];
+    uint32_t id = st[F_ID * SW + o];
     bool alpha = (flags & 2u) != 0;
     float ooa = SF(F_OOA), oa = SF(F_OA), oda = SF(F_ODA), odb = SF(F_ODB);
     float za = SF(F_ZA), zda = SF(F_ZDA), zdb = SF(F_ZDB);
@@ -1621,24 +1621,25 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
     __syncthreads();
 
     // ---- phase 1: coverage and depth ----
-    // Each wave takes 64 bin entries at a time: every lane loads and sets up
-    // one triangle, stages it in LDS, and counts the quads of its bounding
-    // box inside this tile.  A wave prefix sum turns the counts into a dense
-    // list of (entry, quad) tasks that the wave then walks 64 at a time, so
-    // small and large triangles alike keep every lane busy.
-    uint32_t *st = stage + wave * (Stage<T>::COUNT * 64);
+    // The workgroup takes NT bin entries at a time: every thread loads and
+    // sets up one triangle, stages it in LDS, and counts the quads of its
+    // bounding box inside this tile that can hold a covered sample.  A
+    // workgroup prefix sum turns the counts into a dense list of (entry,
+    // quad) tasks that all NT threads then walk, so one-pixel and full-tile
+    // triangles, and tiles with few or many entries, keep every lane busy.
+    __shared__ int wtot[NW];
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
     if (end > P.pair_cap) end = P.pair_cap;
-    for (uint32_t c0 = start + (uint32_t)wave * 64u; c0 < end; c0 += (uint32_t)NW * 64u) {
-        uint32_t e = c0 + (uint32_t)lane;
+    for (uint32_t c0 = start; c0 < end; c0 += (uint32_t)NT) {
+        uint32_t e = c0 + (uint32_t)tid;
         int nq = 0;
         if (e < end) {
             uint32_t id = P.list[e];
             uint4 g = P.geo[id];
             TriSetup ts;
             tri_setup(g, P.raster + 4 * (size_t)id, tx0, ty0, ts);
-            nq = stage_entry<T>(ts, id, g.z, tx0, ty0, st, lane);
+            nq = stage_entry<T, NT>(ts, id, g.z, tx0, ty0, stage, tid);
         }
         int incl = nq;
 #pragma unroll
@@ -1646,24 +1647,30 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
             int o = __shfl_up(incl, d, 64);
             if (lane >= d) incl += o;
         }
-        st[F_INCL * 64 + lane] = (uint32_t)incl;
-        int total = __builtin_amdgcn_readlane(incl, 63);
-        wave_lds_sync();
-        for (int k0 = 0; k0 < total; k0 += 64) {
-            int k = k0 + lane;
+        if (lane == 63) wtot[wave] = incl;
+        __syncthreads();
+        int total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            if (w < wave) incl += wtot[w];
+            total += wtot[w];
+        }
+        stage[F_INCL * NT + tid] = (uint32_t)incl;
+        __syncthreads();
+        for (int k0 = 0; k0 < total; k0 += NT) {
+            int k = k0 + tid;
             if (k < total) {
                 // owner = number of entries whose inclusive count is <= k
                 int o = 0;
 #pragma unroll
-                for (int step = 32; step > 0; step >>= 1)
-                    if ((int)st[F_INCL * 64 + o + step - 1] <= k) o += step;
-                int before = o ? (int)st[F_INCL * 64 + o - 1] : 0;
-                walk_quad<T>(S, P, st, o, k - before, tx0, ty0, keys);
+                for (int step = NT / 2; step > 0; step >>= 1)
+                    if ((int)stage[F_INCL * NT + o + step - 1] <= k) o += step;
+                int before = o ? (int)stage[F_INCL * NT + o - 1] : 0;
+                walk_quad<T, NT>(S, P, stage, o, k - before, tx0, ty0, keys);
             }
         }
-        wave_lds_sync();
+        __syncthreads();
     }
-    __syncthreads();
 
     // ---- resolve: canonical triangle id per pixel ----
     uint32_t rec0 = P.tri_base[V.in_tri_base];
@@ -1683,7 +1690,7 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
 template <int T, int NT>
 __global__ void __launch_bounds__(NT) k_raster(RasterArgs P) {
     __shared__ unsigned long long keys[T * T];
-    __shared__ uint32_t stage[(NT / 64) * Stage<T>::COUNT * 64];
+    __shared__ uint32_t stage[Stage<T>::COUNT * NT];
     if (!P.persistent) {
         raster_item<T, NT>(P, blockIdx.x, keys, stage);
         return;
