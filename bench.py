@@ -312,6 +312,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="c2: skip the batched (C4) and served (C5) summaries in the same line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -442,6 +444,22 @@ def main():
                                          f"(median {med * 1e3:.1f} ms/frame, "
                                          f"{scene.triangles / med / 1e6:.2f} Mtri/s)"}
         out["gpu_vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+    if args.config == "c2" and not args.no_extras:
+        # the metric's other two parts, on their own workloads (C4, C5)
+        del r, gscene
+        try:
+            b = run_batched(args)
+            out["batched"] = {k: b[k] for k in ("value", "unit", "plain_frames_per_s", "batched_vs_plain",
+                                                "ms_per_step", "plain_ms", "mtri_per_s", "gpix_per_s")}
+            out["batched"]["config"] = b["config"]
+        except Exception as ex:  # reported, never fatal to the headline line
+            out["batched"] = {"error": repr(ex)}
+        try:
+            sv = run_served(args)
+            out["served"] = {"max_frames_per_s": sv["value"], "equal_p99": sv["equal_p99"],
+                             "rates": sv["rates"], "config": sv["config"]}
+        except Exception as ex:
+            out["served"] = {"error": repr(ex)}
     print(json.dumps(out))
 
 
