@@ -74,7 +74,7 @@ struct DevScene {
     const uint32_t *specular;
     const uint32_t *lut;
     uint32_t lut_size;
-    const float *vox_light;       // constant-SH form: per-voxel w0
+    const float *vox_light;       // constant-SH form: per cell, w0 at its 8 trilinear corners
     const float *vox_sh4;         // full form: 16 floats per voxel (or null)
     int32_t vox_dims[3];
     float vox_min[3];

@@ -276,7 +276,22 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
         if ((r = upload(s, vg.sh4, nvox * 16 * 4, &p))) return bail(r);
         ds.vox_sh4 = (const float *)p;
     } else {
-        if ((r = upload(s, vg.light_intensity, nvox * 4, &p))) return bail(r);
+        // the eight trilinear corners of every cell, contiguous and already
+        // clamped at the far faces as the lookup clamps them
+        // (src/voxelgrid.rs:275-300): one 32-byte read per lookup
+        const int W = vg.dims[0], H = vg.dims[1], D = vg.dims[2];
+        std::vector<float> c8(nvox * 8);
+        const float *L = vg.light_intensity;
+        for (int z = 0; z < D; z++)
+            for (int y = 0; y < H; y++)
+                for (int x = 0; x < W; x++) {
+                    int x1 = std::min(x + 1, W - 1), y1 = std::min(y + 1, H - 1), z1 = std::min(z + 1, D - 1);
+                    auto at = [&](int xx, int yy, int zz) { return L[((size_t)zz * H + yy) * W + xx]; };
+                    float *o = &c8[(((size_t)z * H + y) * W + x) * 8];
+                    o[0] = at(x, y, z);   o[1] = at(x, y, z1);  o[2] = at(x, y1, z);  o[3] = at(x, y1, z1);
+                    o[4] = at(x1, y, z);  o[5] = at(x1, y, z1); o[6] = at(x1, y1, z); o[7] = at(x1, y1, z1);
+                }
+        if ((r = upload(s, c8.data(), nvox * 8 * 4, &p))) return bail(r);
         ds.vox_light = (const float *)p;
     }
     for (int k = 0; k < 3; k++) {
