@@ -452,19 +452,66 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, u
     }
 }
 
+// One workgroup scans arrays of up to SCAN_SINGLE_MAX elements in a single
+// launch; larger arrays use the
+// partials / top / apply sequence above.
+#define SCAN_SINGLE_MAX 16384u
+
+__global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2,
+                                                      uint32_t *total_out) {
+    __shared__ uint32_t sh[17];
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < n; c += 1024u * SCAN_ITEMS) {
+        uint32_t base = c + threadIdx.x * SCAN_ITEMS;
+        uint32_t v[SCAN_ITEMS];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            v[k] = base + k < n ? in[base + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t ex;
+        uint32_t tot = block_excl_scan(sum, ex, sh);
+        uint32_t acc = carry + ex;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) {
+            if (base + k < n) {
+                out[base + k] = acc;
+                if (out2) out2[base + k] = acc;
+            }
+            acc += v[k];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
 static hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
                               uint32_t *total, hipStream_t st) {
+    if (n <= SCAN_SINGLE_MAX) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, in, n, out, out2, total);
+        return hipGetLastError();
+    }
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb == 0) nb = 1;
     hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2);
     return hipGetLastError();
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+// Frame start in one launch: clear the status, zero the bin counts and set
+// the zero sentinel after the per-triangle counts.
+__global__ void __launch_bounds__(256) k_frame_begin(DevStatus *status, uint32_t *bin_count, uint32_t items,
+                                                     uint32_t *sentinel) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        uint32_t *w = reinterpret_cast<uint32_t *>(status);
+        for (uint32_t k = 0; k < (uint32_t)(sizeof(DevStatus) / 4); k++) w[k] = 0u;
+        *sentinel = 0u;
+    }
+    for (; i < items; i += gridDim.x * blockDim.x) bin_count[i] = 0u;
 }
+
 
 // ---------------------------------------------------------------------------
 // Binning rectangle: GPU tiles touched by the quad-aligned pixel bbox
@@ -1863,10 +1910,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (a.total_xv > 0)
         hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_xv, a.xv, a.any_tangents ? a.xt : nullptr);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(a.status),
-                       (uint32_t)(sizeof(DevStatus) / 4), 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.bin_count,
-                       a.total_items, 0u);
+    hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.status,
+                       a.bin_count, a.total_items, a.tri_counts + a.total_in);
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv, a.tri_counts, a.clip_queue, a.clip_cap, a.status);
@@ -1875,7 +1920,6 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
                            a.status);
     }
     // tri_counts[total_in] is a zero sentinel so tri_base[total_in] = total
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, a.tri_counts + a.total_in, 1u, 0u);
     hipError_t e = launch_scan(a.tri_counts, a.total_in + 1, a.tri_base, nullptr, a.scan_partials,
                                &a.status->total_records, st);
     if (e != hipSuccess) return e;
