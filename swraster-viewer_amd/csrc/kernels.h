@@ -19,7 +19,7 @@ struct FrameArgs {
     int32_t tile_size;            // 8, 16 or 32
     int32_t any_tangents;
     // transient buffers
-    float4 *xv;                   // 3 float4 per transformed vertex
+    float4 *xv;                   // total_xv clip positions, then 2 * total_xv attribute float4
     float4 *xt;                   // tangent float4 per transformed vertex (or null)
     uint32_t *tri_counts;         // total_in (+1)
     uint32_t *tri_base;           // total_in (+1)

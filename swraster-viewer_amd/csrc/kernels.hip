@@ -133,18 +133,9 @@ struct CV {
     float tx, ty, tz, tw;         // tangent
 };
 
-__device__ __forceinline__ void load_cv(const float4 *xv, const float4 *xt, uint32_t i, CV &o) {
-    float4 a = xv[3 * (size_t)i], b = xv[3 * (size_t)i + 1], c = xv[3 * (size_t)i + 2];
-    o.cx = a.x; o.cy = a.y; o.cz = a.z; o.cw = a.w;
-    o.wx = b.x; o.wy = b.y; o.wz = b.z; o.nx = b.w;
-    o.ny = c.x; o.nz = c.y; o.u = c.z; o.v = c.w;
-    if (xt) {
-        float4 t = xt[i];
-        o.tx = t.x; o.ty = t.y; o.tz = t.z; o.tw = t.w;
-    } else {
-        o.tx = o.ty = o.tz = o.tw = 0.0f;
-    }
-}
+// Transformed vertices are two arrays: clip-space positions (one float4 per
+// vertex, all that setup and clipping classification read) and attributes
+// (two float4: world position and normal x; normal y, z and uv).
 
 // Plane dot products in glam's SSE Vec4::dot order (x*x' + z*z') + (y*y' + w*w')
 // for the six clip planes of src/renderer.rs:581-588.
@@ -184,7 +175,7 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
 // k_transform: one thread per (draw, vertex)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_transform(const DevView *views, const DevDraw *draws, int ndraws,
-                                                   uint32_t total, float4 *xv, float4 *xt) {
+                                                   uint32_t total, float4 *xp, float4 *xa, float4 *xt) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         int d = find_last_le(draws, ndraws, i, &DevDraw::xv_base);
         const DevDraw &D = draws[d];
@@ -211,9 +202,9 @@ __global__ void __launch_bounds__(256) k_transform(const DevView *views, const D
             an = an + D.model[8 + r] * nz;
             n[r] = an;
         }
-        xv[3 * (size_t)i] = make_float4(c[0], c[1], c[2], c[3]);
-        xv[3 * (size_t)i + 1] = make_float4(w[0], w[1], w[2], n[0]);
-        xv[3 * (size_t)i + 2] = make_float4(n[1], n[2], S.tu[s], S.tv[s]);
+        xp[i] = make_float4(c[0], c[1], c[2], c[3]);
+        xa[2 * (size_t)i] = make_float4(w[0], w[1], w[2], n[0]);
+        xa[2 * (size_t)i + 1] = make_float4(n[1], n[2], S.tu[s], S.tv[s]);
         if (xt) {
             if ((D.flags & SR_DRAW_TANGENTS) && S.tangent) {
                 float4 t = reinterpret_cast<const float4 *>(S.tangent)[s];
@@ -245,7 +236,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // for k_clip, which writes their count.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const DevDraw *draws, int ndraws,
-                                                    uint32_t total, const float4 *xv, uint32_t *counts,
+                                                    uint32_t total, const float4 *xp, uint32_t *counts,
                                                     uint32_t *queue, uint32_t qcap, DevStatus *status) {
     const uint32_t stride = gridDim.x * blockDim.x;
     const int lane = threadIdx.x & 63;
@@ -262,7 +253,7 @@ __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const 
                 float c[3][4];
                 for (int k = 0; k < 3; k++) {
                     uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-                    float4 p = xv[3 * (size_t)(D.xv_base + vi)];
+                    float4 p = xp[D.xv_base + vi];
                     c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
                 }
                 cls = clip_classify(c);
@@ -300,7 +291,7 @@ __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const 
 #define CLIP_MAXV 16
 
 __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDraw *draws, int ndraws,
-                                              const float4 *xv, const float4 *xt, uint32_t *counts,
+                                              const float4 *xp, const float4 *xa, const float4 *xt, uint32_t *counts,
                                               const uint32_t *queue, uint32_t qcap, float *buf,
                                               DevStatus *status) {
     __shared__ float sh[16][2][CLIP_MAXV][16];
@@ -317,7 +308,8 @@ __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDra
         for (int k = 0; k < 3; k++) {
             uint32_t vi = D.xv_base + S.indices[D.idx_base + 3 * lt + k];
             float val;
-            if (c < 12) val = reinterpret_cast<const float *>(xv)[12 * (size_t)vi + c];
+            if (c < 4) val = reinterpret_cast<const float *>(xp)[4 * (size_t)vi + c];
+            else if (c < 12) val = reinterpret_cast<const float *>(xa)[8 * (size_t)vi + c - 4];
             else val = ((D.flags & SR_DRAW_TANGENTS) && xt) ? reinterpret_cast<const float *>(xt)[4 * (size_t)vi + c - 12]
                                                             : 0.0f;
             P[k][c] = val;
@@ -649,7 +641,8 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
 struct TriSource {
     const DevView *views;
     const DevDraw *draws;
-    const float4 *xv;         // transformed vertices
+    const float4 *xp;         // transformed clip-space positions
+    const float4 *xa;         // transformed attributes (2 per vertex)
     const float4 *xt;         // transformed tangents (or null)
     const float *clip_buf;    // clipped polygons
 };
@@ -676,7 +669,7 @@ __device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R,
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             size_t i = (size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k];
-            q[k][0] = T.xv[3 * i]; q[k][1] = T.xv[3 * i + 1]; q[k][2] = T.xv[3 * i + 2];
+            q[k][0] = T.xp[i]; q[k][1] = T.xa[2 * i]; q[k][2] = T.xa[2 * i + 1];
             q[k][3] = tan ? T.xt[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
@@ -842,7 +835,7 @@ __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view,
 }
 
 __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
-                                               const float4 *xv, const float4 *xt, const uint32_t *tri_base,
+                                               const float4 *xp, const uint32_t *tri_base,
                                                SetupOut O) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
@@ -859,7 +852,8 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
             CV v[3];
             for (int k = 0; k < 3; k++) {
                 uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-                load_cv(xv, (D.flags & SR_DRAW_TANGENTS) ? xt : nullptr, D.xv_base + vi, v[k]);
+                float4 q = xp[D.xv_base + vi];    // setup reads clip positions only
+                v[k].cx = q.x; v[k].cy = q.y; v[k].cz = q.z; v[k].cw = q.w;
             }
             uint32_t matflags = S.materials[D.material].flags;
             uint32_t id = tri_base[t];
@@ -1909,13 +1903,13 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (a.events[0]) hipEventRecord(a.events[0], st);
     if (a.total_xv > 0)
         hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_xv, a.xv, a.any_tangents ? a.xt : nullptr);
+                           a.num_draws, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr);
     hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.status,
                        a.bin_count, a.total_items, a.tri_counts + a.total_in);
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv, a.tri_counts, a.clip_queue, a.clip_cap, a.status);
-        hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv,
+        hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv, a.xv + a.total_xv,
                            a.any_tangents ? a.xt : nullptr, a.tri_counts, a.clip_queue, a.clip_cap, a.clip_buf,
                            a.status);
     }
@@ -1928,7 +1922,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     so.bin_count = a.bin_count; so.cap = a.rec_cap; so.status = a.status; so.T = a.tile_size;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.any_tangents ? a.xt : nullptr, a.tri_base, so);
+                           a.num_draws, a.total_in, a.xv, a.tri_base, so);
         hipLaunchKernelGGL(k_clip_emit, dim3(256), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.clip_queue,
                            a.clip_cap, a.clip_buf, a.tri_counts, a.tri_base, a.status, so);
     }
@@ -1942,7 +1936,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     P.views = a.views; P.num_views = a.num_views; P.total_items = a.total_items;
     P.bin_start = a.bin_start; P.bin_end = a.bin_end; P.list = a.list; P.pair_cap = a.pair_cap;
     P.geo = a.rec_geo; P.raster = a.rec_raster;
-    P.src.views = a.views; P.src.draws = a.draws; P.src.xv = a.xv; P.src.xt = a.any_tangents ? a.xt : nullptr;
+    P.src.views = a.views; P.src.draws = a.draws; P.src.xp = a.xv; P.src.xa = a.xv + a.total_xv;
+    P.src.xt = a.any_tangents ? a.xt : nullptr;
     P.src.clip_buf = a.clip_buf;
     P.tri_base = a.tri_base; P.out_ids = a.out_ids; P.out_depth = a.out_depth; P.out_color = a.out_color;
     P.color_plane = a.color_plane; P.out_rgba = a.out_rgba; P.out_lum = a.out_lum; P.status = a.status;
