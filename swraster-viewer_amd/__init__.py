@@ -398,6 +398,11 @@ class SceneData:
         self.voxels = None
         self._build_desc()
 
+    def rebuild(self):
+        """Rebuild the C descriptor after editing primitives, materials or
+        textures in place (before prepare/attach)."""
+        self._build_desc()
+
     @property
     def triangles(self):
         return sum(len(p["indices"]) // 3 for p in self.primitives)

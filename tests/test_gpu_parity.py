@@ -282,6 +282,34 @@ def test_server_same_alone_and_in_the_mix(sr, scenes, c1, env):
     srv.close()
 
 
+def test_alpha_tested_materials(sr, orc, scenes, env):
+    """MASK materials (src/shader.rs:40-61, :311-329): the base texture's
+    nearest-sampled alpha against the cutoff decides coverage; alpha from
+    block noise so whole regions pass or fail and edges fall inside
+    triangles.  Half the spheres are masked, with different cutoffs."""
+    scene, spec = scenes.make_c1(width=640, height=360)
+    rng = np.random.Generator(np.random.PCG64(77))
+    w, h, fmt, px = scene.textures[0]
+    rgba = sr.unpack_rgba(np.asarray(px, np.uint32)).reshape(h, w, 4).copy()
+    blocks = rng.integers(0, 256, (h // 8, w // 8), dtype=np.uint8)
+    rgba[..., 3] = np.kron(blocks, np.ones((8, 8), np.uint8))
+    scene.textures[0] = (w, h, fmt, sr.pack_rgba(rgba.reshape(-1, 4)).ravel())
+    for i, m in enumerate(scene.materials):
+        if i % 2 == 0:
+            m["alpha_mode"] = sr.SR_ALPHA_MASK
+            m["alpha_cutoff"] = 0.25 + 0.05 * i
+    scene.rebuild()
+    scene.prepare(env, cache_key=None, grid=32)
+    cam = scenes.make_camera(spec)
+    g = render_gpu(sr, sr.Scene(scene), cam, 640, 360)
+    o = render_oracle(orc, scene, cam, 640, 360)
+    compare(sr, g, o, "alpha-tested c1")
+    plain, _ = scenes.make_c1(width=640, height=360)
+    plain.prepare(env, cache_key=plain.name)
+    opaque = render_oracle(orc, plain, cam, 640, 360)
+    assert (g["ids"] != sr.NO_ID).sum() < (opaque["ids"] != sr.NO_ID).sum(), "the mask removed nothing"
+
+
 def test_gpu_matches_committed_c1_frame(sr, scenes, env):
     """The GPU reproduces the committed oracle frame (tests/golden/) without
     running the oracle."""
