@@ -1820,6 +1820,19 @@ __attribute__((visibility("default"))) int orc_camera_update(sr_camera *c) {
     return 0;
 }
 // Tonemap + quantize one linear value with a given exposure (closed-form tests).
+// The per-tile translucent sort on its own: order[i] = the index of the
+// packet that ends at position i, packets given by avg_z in queue order.
+__attribute__((visibility("default"))) int orc_sort_translucent(const float *avg_z, uint32_t *order, size_t n) {
+    std::vector<RasterPacket> v(n);
+    for (size_t i = 0; i < n; i++) {
+        v[i].avg_z = avg_z[i];
+        v[i].tri_id = (uint32_t)i;
+    }
+    if (n > 1) lomuto_quicksort(v, 0, n - 1);
+    for (size_t i = 0; i < n; i++) order[i] = v[i].tri_id;
+    return 0;
+}
+
 __attribute__((visibility("default"))) uint32_t orc_quantize(float c, float exposure) {
     float x = c * exposure;
     float k = 0.2f;

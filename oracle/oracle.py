@@ -43,6 +43,7 @@ def lib():
         L.orc_quantize.restype = C.c_uint32
         L.orc_quantize.argtypes = [C.c_float, C.c_float]
         L.orc_mip_chain.argtypes = [P, P, P]
+        L.orc_sort_translucent.argtypes = [P, P, C.c_size_t]
         _lib = L
     return _lib
 
@@ -142,3 +143,12 @@ def mip_chain(tex_desc):
     out = np.empty(n.value, np.uint32)
     lib().orc_mip_chain(C.byref(tex_desc), _p(out), C.byref(n))
     return out
+
+
+def sort_translucent(avg_z):
+    """The oracle's per-tile translucent packet order for packets with these
+    avg_z values in queue order (returns the permutation)."""
+    z = np.ascontiguousarray(avg_z, np.float32)
+    order = np.empty(len(z), np.uint32)
+    lib().orc_sort_translucent(_p(z), _p(order), len(z))
+    return order

@@ -31,6 +31,7 @@
 #define SR_GEO_EMPTY 1u          // culled or empty bounding box: no bins
 #define SR_GEO_ALPHA 2u          // alpha-tested material
 #define SR_GEO_NORMAL_MAP 4u
+#define SR_GEO_TRANSLUCENT 8u    // translucent material: 64-px reference-tile lists, forward pass
 
 struct DevTexRef {
     int32_t tex;                  // -1 = none
@@ -130,4 +131,6 @@ struct DevStatus {
     uint32_t clip_queued;         // triangles queued for clipping
     uint32_t overflow_clip;
     uint32_t clip_overflow;       // a clipped polygon outgrew its room (frame fails)
+    uint32_t total_tr_pairs;      // (translucent triangle, 64-px tile) pairs
+    uint32_t overflow_tr;
 };

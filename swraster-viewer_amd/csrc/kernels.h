@@ -43,6 +43,16 @@ struct FrameArgs {
     uint32_t *out_rgba;           // fused blit target (views with flags & 1)
     float *out_lum;
     uint32_t *center_ids;         // 4 per 64px tile (sum over views)
+    // translucent triangles (views without the fused blit only)
+    int32_t has_translucent;
+    uint32_t tr_slots;            // 64px tiles (sum over views)
+    uint32_t *tr_count;           // tr_slots
+    uint32_t *tr_start;           // tr_slots
+    uint32_t *tr_end;             // tr_slots (cursor, then end)
+    uint32_t *tr_list;            // tr_cap record ids
+    unsigned long long *tr_scratch;  // tr_cap: sort room for lists longer than LDS holds
+    uint32_t tr_cap;
+    float4 *center_color;         // 4 (colour, depth) per 64px tile
     int32_t persistent_blocks;    // 0 = one block per item
     hipEvent_t events[5];         // optional stage markers: start, setup done, bins done, shade done, raster done
 };

@@ -494,14 +494,16 @@ static hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uin
 // Frame start in one launch: clear the status, zero the bin counts and set
 // the zero sentinel after the per-triangle counts.
 __global__ void __launch_bounds__(256) k_frame_begin(DevStatus *status, uint32_t *bin_count, uint32_t items,
-                                                     uint32_t *sentinel) {
+                                                     uint32_t *sentinel, uint32_t *tr_count, uint32_t tr_slots) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         uint32_t *w = reinterpret_cast<uint32_t *>(status);
         for (uint32_t k = 0; k < (uint32_t)(sizeof(DevStatus) / 4); k++) w[k] = 0u;
         *sentinel = 0u;
     }
-    for (; i < items; i += gridDim.x * blockDim.x) bin_count[i] = 0u;
+    for (uint32_t k = i; k < items; k += gridDim.x * blockDim.x) bin_count[k] = 0u;
+    if (tr_count)
+        for (uint32_t k = i; k < tr_slots; k += gridDim.x * blockDim.x) tr_count[k] = 0u;
 }
 
 
@@ -590,7 +592,8 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t *cursor, uint32_t bin) {
 // (SRC_* encoding below).
 #define SRC_CLIPPED 0x80000000u       // src0 = SRC_CLIPPED | fan << 27 | clip slot
 __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV *v1, const CV *v2, uint32_t id,
-                              int material, uint32_t matflags, uint32_t src0, uint32_t src1, const SetupOut &O) {
+                              int material, uint32_t matflags, bool translucent, uint32_t src0, uint32_t src1,
+                              const SetupOut &O) {
     Rect none = {0, 0, 0, 0};
     if (id >= O.cap) {
         atomicOr(&O.status->overflow_records, 1u);
@@ -627,6 +630,7 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
     uint32_t flags = 0;
     if (matflags & SR_MAT_ALPHA_TESTED) flags |= SR_GEO_ALPHA;
     if (matflags & SR_MAT_NORMAL_MAP) flags |= SR_GEO_NORMAL_MAP;
+    if (translucent) flags |= SR_GEO_TRANSLUCENT;
     O.geo[id] = make_uint4((uint32_t)pmnx | ((uint32_t)pmny << 16), (uint32_t)pmxx | ((uint32_t)pmxy << 16), flags,
                            (uint32_t)view);
     float4 *R = O.raster + 4 * (size_t)id;
@@ -634,6 +638,7 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
     R[1] = make_float4(__int_as_float(px[2]), __int_as_float(py[2]), ooa, oow[0]);
     R[2] = make_float4(oow[1] - oow[0], oow[2] - oow[0], zow[0], zow[1] - zow[0]);
     R[3] = make_float4(zow[2] - zow[0], __int_as_float(material), __uint_as_float(src0), __uint_as_float(src1));
+    if (translucent) return none;     // 64-px reference-tile lists instead (k_tr_bins)
     return bin_rect(V, pmnx, pmny, pmxx, pmxy, O.T);
 }
 
@@ -865,7 +870,8 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
             }
             // triangles that cross a plane are set up by k_clip_emit
             if (cls == -1)
-                r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags, t, (uint32_t)d, O);
+                r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags,
+                                  !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
         }
         bin_rect_count(views, view, r, O.bin_count);
     }
@@ -906,7 +912,8 @@ __global__ void __launch_bounds__(256) k_clip_emit(const DevView *views, const D
                     v[j].tx = dd.x; v[j].ty = dd.y; v[j].tz = dd.z; v[j].tw = dd.w;
                 }
                 r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], tri_base[t] + f, D.material,
-                                  S.materials[D.material].flags, SRC_CLIPPED | (f << 27) | e, 0u, O);
+                                  S.materials[D.material].flags, !(D.flags & SR_DRAW_OPAQUE),
+                                  SRC_CLIPPED | (f << 27) | e, 0u, O);
             }
         }
         bin_rect_count(views, view, r, O.bin_count);
@@ -929,7 +936,7 @@ __global__ void __launch_bounds__(256) k_scatter(const DevView *views, const uin
         if (id < total) {
             uint4 g = geo[id];
             view = g.w;
-            if (!(g.z & SR_GEO_EMPTY)) {
+            if (!(g.z & (SR_GEO_EMPTY | SR_GEO_TRANSLUCENT))) {
                 int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
                 r = bin_rect(views[view], mnx, mny, mxx, mxy, T);
             }
@@ -1196,11 +1203,13 @@ __device__ __forceinline__ void voxel_lookup(const DevScene &S, f3 p, f3 rgb[4],
 }
 
 // ---------------------------------------------------------------------------
-// PBR shading of one pixel (pbr_shader, src/shader.rs:110-309, opaque path)
+// PBR shading of one pixel (pbr_shader, src/shader.rs:110-309)
 // b1[4], b2[4]: the quad's four barycentric pairs; lane: this pixel's lane.
+// TR: the translucent form, which blends over cur (src/shader.rs:66-76).
 // ---------------------------------------------------------------------------
+template <bool TR>
 __device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, const float4 *R, const float4 *Sh, const float4 *Tn, bool nmap,
-                          const float b1[4], const float b2[4], int lane) {
+                          const float b1[4], const float b2[4], int lane, f3 cur) {
     float4 r1 = R[1], r2 = R[2], r3 = R[3];
     float oa = r1.w, oda = r2.x, odb = r2.y;
     int material = __float_as_int(r3.y);
@@ -1328,7 +1337,18 @@ __device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, c
     f3 cis = mul3(env, bsf);
     float ao_spec = 1.0f + (ao - 1.0f) * 0.5f;
     cis = muls(cis, ao_spec * skyv);
-    f3 color = add3(add3(add3(cdd, cds), cid), cis);
+    f3 color;
+    if (TR) {
+        // transmission factor times the transmission texture's red channel
+        float tr = M.transmission;
+        if (M.transmission_tex.tex >= 0) tr = tr * unpack_rgb(texel_nearest(S, M.transmission_tex, uvx, uvy, dd)).x;
+        tr = clampv(tr, 0.0f, 1.0f);
+        float inv_tr = 1.0f - tr;
+        color = add3(add3(muls(mul3(cur, base), tr), muls(add3(cdd, cid), inv_tr)), cds);
+        color = add3(color, cis);
+    } else {
+        color = add3(add3(add3(cdd, cds), cid), cis);
+    }
     f3 em = mk3(1.0f, 1.0f, 1.0f);
     if (M.emissive_tex.tex >= 0) em = srgb_fast(unpack_rgb(texel_nearest(S, M.emissive_tex, uvx, uvy, dd)));
     color = add3(color, mul3(em, mk3(M.emissive[0], M.emissive[1], M.emissive[2])));
@@ -1372,6 +1392,7 @@ struct RasterArgs {
     uint32_t *out_rgba;
     float *out_lum;
     uint32_t *center_ids;         // 4 winners per 64px tile: its centre quad, on screen or not
+    float4 *center_color;         // 4 (colour, depth) per 64px tile for the translucent pass (or null)
     DevStatus *status;
     int32_t persistent;
 };
@@ -1829,7 +1850,7 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
                 load_tri_cv(P.src, R, tv);
                 bool nmap = (P.geo[id].z & SR_GEO_NORMAL_MAP) != 0;
                 tri_shade_attrs(tv, R, nmap, Sh, Tn);
-                color = shade_pixel(S, V, R, Sh, Tn, nmap, qb1, qb2, s);
+                color = shade_pixel<false>(S, V, R, Sh, Tn, nmap, qb1, qb2, s, color);
             } else {
                 color = skybox_pixel(S, V, tx0 + 2 * qx, ty0 + 2 * qy, sx, sy);
             }
@@ -1850,6 +1871,10 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
                 P.out_color[2 * P.color_plane + o] = color.z;
             }
         }
+        if (center && P.center_color) {
+            int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
+            P.center_color[4 * (V.lum_offset + (uint32_t)ri) + s] = make_float4(color.x, color.y, color.z, z);
+        }
         // centre-quad luminance of the 64px tile (src/tilerasterizer.rs:103-106)
         float lum = (color.x * 0.2126f + color.y * 0.7152f) + color.z * 0.0722f;
         float l0 = __shfl(lum, qbase, 64), l1 = __shfl(lum, qbase + 1, 64);
@@ -1869,6 +1894,397 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
 template <int T, int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32 ? 4 : 6))) k_shade(RasterArgs P) {
     shade_item<T, NT>(P, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Translucent triangles: 64-px reference-tile lists, a back-to-front sort per
+// tile and forward shading over the opaque frame (src/renderer.rs:765-829,
+// src/tilerasterizer.rs:92-101, src/shader.rs:66-76).
+// ---------------------------------------------------------------------------
+// The reference's bin loop for one triangle (src/renderer.rs:756-793): the
+// 64-px tile range of its pixel box, bin index y * tiles_x + x in wrapping
+// i32 with its bounds check, and the clip rectangle's emptiness test.  A box
+// that reaches the right edge of a screen whose width is a multiple of 64
+// runs one column past the last tile; that index names the first tile of the
+// next row, so the triangle is listed there twice and blends twice, as in the
+// reference.
+template <class F>
+__device__ __forceinline__ void ref_tile_bins(const DevView &V, uint4 g, F f) {
+    int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
+    int bx0 = mnx / SR_REF_TILE, by0 = mny / SR_REF_TILE;
+    int bx1 = wadd(mxx, SR_REF_TILE - 1) / SR_REF_TILE, by1 = wadd(mxy, SR_REF_TILE - 1) / SR_REF_TILE;
+    int ntiles = V.ref_tiles_x * V.ref_tiles_y;
+    for (int y = by0; y < by1; y++)
+        for (int x = bx0; x < bx1; x++) {
+            int bi = wadd(wmul(y, V.ref_tiles_x), x);
+            if (bi < 0 || bi >= ntiles) continue;
+            int tx = (bi % V.ref_tiles_x) * SR_REF_TILE, ty = (bi / V.ref_tiles_x) * SR_REF_TILE;
+            if (min(mxx, tx + SR_REF_TILE) - max(mnx, tx) < 1 || min(mxy, ty + SR_REF_TILE) - max(mny, ty) < 1) continue;
+            f((uint32_t)bi);
+        }
+}
+
+// Count (list null) or scatter pass over the set-up records: translucent
+// ones into the lists of their 64-px tiles (slot = view's first tile + bin).
+__global__ void __launch_bounds__(256) k_tr_bins(const DevView *views, const uint4 *geo, uint32_t rec_cap,
+                                                 uint32_t *counts, uint32_t *list, uint32_t cap, DevStatus *status) {
+    uint32_t total = min(status->total_records, rec_cap);
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < total; id += gridDim.x * blockDim.x) {
+        uint4 g = geo[id];
+        if ((g.z & (SR_GEO_TRANSLUCENT | SR_GEO_EMPTY)) != SR_GEO_TRANSLUCENT) continue;
+        const DevView &V = views[g.w];
+        uint32_t *c = counts + V.lum_offset;
+        ref_tile_bins(V, g, [&](uint32_t bi) {
+            uint32_t pos = atomicAdd(&c[bi], 1u);
+            if (!list) return;
+            if (pos < cap) list[pos] = id;
+            else atomicOr(&status->overflow_tr, 1u);
+        });
+    }
+}
+
+// avg_z of the packet: mean clip-space z of the three vertices
+// (src/renderer.rs:765-775).
+__device__ __forceinline__ float tri_avg_z(const TriSource &T, const float4 *R) {
+    uint32_t src0 = __float_as_uint(R[3].z), src1 = __float_as_uint(R[3].w);
+    float z[3];
+    if (src0 & SRC_CLIPPED) {
+        uint32_t e = src0 & 0x07FFFFFFu, f = (src0 >> 27) & 15u;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            uint32_t vi = j == 0 ? 0u : f + (uint32_t)j;
+            z[j] = T.clip_buf[((size_t)e * 16u + vi) * 16u + 2u];
+        }
+    } else {
+        const DevDraw &D = T.draws[src1];
+        const DevScene &S = *T.views[D.view].scene;
+        uint32_t lt = src0 - D.in_tri_base;
+#pragma unroll
+        for (int k = 0; k < 3; k++) z[k] = T.xp[(size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k]].z;
+    }
+    return ((z[0] + z[1]) + z[2]) / 3.0f;
+}
+
+// Key that orders z descending under OrderedFloat (src/tilerasterizer.rs:
+// 93-94: b.avg_z.cmp(&a.avg_z)): every NaN first (greatest), -0 == +0.
+__device__ __forceinline__ uint32_t z_desc_key(float z) {
+    if (z != z) return 0u;
+    if (z == 0.0f) z = 0.0f;
+    uint32_t b = __float_as_uint(z);
+    uint32_t asc = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ~asc;
+}
+
+// Ascending bitonic sort of v[0..P), P a power of two, by the whole workgroup.
+__device__ void bitonic_sort(unsigned long long *v, int P) {
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    unsigned long long a = v[i], b = v[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        v[i] = b;
+                        v[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Ascending heap sort of v[0..n) by one thread.
+__device__ void heap_sort(unsigned long long *v, int n) {
+    auto sift = [&](int root, int end) {
+        for (;;) {
+            int c = 2 * root + 1;
+            if (c >= end) return;
+            if (c + 1 < end && v[c] < v[c + 1]) c++;
+            if (!(v[root] < v[c])) return;
+            unsigned long long t = v[root];
+            v[root] = v[c];
+            v[c] = t;
+            root = c;
+        }
+    };
+    for (int r = n / 2 - 1; r >= 0; r--) sift(r, n);
+    for (int end = n - 1; end > 0; end--) {
+        unsigned long long t = v[0];
+        v[0] = v[end];
+        v[end] = t;
+        sift(0, end);
+    }
+}
+
+// BumpQueue::sort_by (src/bumpqueue.rs:155-200) by one thread: Lomuto
+// quicksort with the rightmost element as pivot, an element moving left when
+// it compares Less, i.e. its z key is below the pivot's (its avg_z above).
+// The two parts of a partition are disjoint, so their order of processing
+// does not change the result: the smaller is sorted first and the larger
+// stacked, which bounds the stack by log2(n) ranges.  Keys in the low 32 bits.
+__device__ void lomuto_sort(unsigned long long *v, int n, int *stack) {
+    int sp = 0, lo = 0, hi = n - 1;
+    for (;;) {
+        while (lo < hi) {
+            uint32_t pivot = (uint32_t)v[hi];
+            int i = lo;
+            for (int j = lo; j < hi; j++) {
+                unsigned long long x = v[j];
+                if ((uint32_t)x < pivot) {
+                    if (i != j) {
+                        v[j] = v[i];
+                        v[i] = x;
+                    }
+                    i++;
+                }
+            }
+            if (i != hi) {
+                unsigned long long t = v[i];
+                v[i] = v[hi];
+                v[hi] = t;
+            }
+            if (i - lo < hi - i) {
+                stack[sp++] = i + 1;
+                stack[sp++] = hi;
+                hi = i - 1;
+            } else {
+                stack[sp++] = lo;
+                stack[sp++] = i - 1;
+                lo = i + 1;
+            }
+        }
+        if (sp == 0) break;
+        hi = stack[--sp];
+        lo = stack[--sp];
+    }
+}
+
+// k_tr_sort: one workgroup per 64-px tile orders its list back to front as
+// the reference does.  With no two different triangles on the same key every
+// correct sort gives the reference's order, and a bitonic sort of
+// (z key, id) in LDS is final.  Otherwise Lomuto's result depends on the
+// starting order, which is the canonical submission order (ascending record
+// id, the copies of a wrapped entry adjacent and identical): the list is put
+// in that order and quicksorted as the reference does.  Lists longer than
+// the LDS holds are sorted in global scratch by one thread.
+#define TR_SORT_MAX 4096
+__global__ void __launch_bounds__(256) k_tr_sort(const uint32_t *tr_start, const uint32_t *tr_end, uint32_t *list,
+                                                 uint32_t cap, unsigned long long *scratch, const float4 *raster,
+                                                 TriSource src) {
+    __shared__ unsigned long long sk[TR_SORT_MAX];
+    __shared__ int stack[2 * 40];
+    __shared__ int serial;
+    uint32_t slot = blockIdx.x;
+    uint32_t s = tr_start[slot], e = min(tr_end[slot], cap);
+    if (e <= s + 1) return;
+    int n = (int)(e - s);
+    if (n <= TR_SORT_MAX) {
+        int P = 1;
+        while (P < n) P <<= 1;
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            unsigned long long key = ~0ull;
+            if (i < n) {
+                uint32_t id = list[s + i];
+                key = ((unsigned long long)z_desc_key(tri_avg_z(src, raster + 4 * (size_t)id)) << 32) | id;
+            }
+            sk[i] = key;
+        }
+        if (threadIdx.x == 0) serial = 0;
+        __syncthreads();
+        bitonic_sort(sk, P);
+        for (int i = threadIdx.x; i + 1 < n; i += blockDim.x)
+            if ((sk[i] >> 32) == (sk[i + 1] >> 32) && (uint32_t)sk[i] != (uint32_t)sk[i + 1]) serial = 1;
+        __syncthreads();
+        if (!serial) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) list[s + i] = (uint32_t)sk[i];
+            return;
+        }
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            unsigned long long x = sk[i];
+            sk[i] = (x << 32) | (x >> 32);          // (id, z key)
+        }
+        __syncthreads();
+        bitonic_sort(sk, P);
+        if (threadIdx.x == 0) lomuto_sort(sk, n, stack);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += blockDim.x) list[s + i] = (uint32_t)(sk[i] >> 32);
+        return;
+    }
+    unsigned long long *v = scratch + s;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t id = list[s + i];
+        v[i] = ((unsigned long long)id << 32) | z_desc_key(tri_avg_z(src, raster + 4 * (size_t)id));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        heap_sort(v, n);
+        lomuto_sort(v, n, stack);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) list[s + i] = (uint32_t)(v[i] >> 32);
+}
+
+struct TrArgs {
+    const uint32_t *start;
+    const uint32_t *end;
+    const uint32_t *list;
+    uint32_t cap;
+};
+
+// Staged entry fields for k_translucent (one 256-wide row each).
+enum {
+    G_A = 0, G_B = 3, G_C = 6, G_SX = 9, G_SY = 12,
+    G_OOA = 15, G_OA, G_ODA, G_ODB, G_ZA, G_ZDA, G_ZDB, G_ID, G_ORG, G_LIM, G_COARSE, G_COUNT
+};
+
+__device__ __forceinline__ int find_view_tile(const RasterArgs &P, uint32_t slot) {
+    int lo = 0, hi = P.num_views - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (P.views[mid].lum_offset <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// k_translucent: the forward pass of each 64-px tile's sorted list
+// (src/tilerasterizer.rs:96-101 with TranslucentForwardShader,
+// src/shader.rs:66-76).  Sixteen workgroups per tile, each a 16x16 pixel
+// block with one thread per pixel in 2x2-quad order, walk the list in order:
+// the reference's quad walk and coverage of the entry (the same walk as the
+// opaque pass), the depth test against the opaque depth without writing it,
+// and on passing samples the translucent shader over the current colour.
+// A pixel's colour depends only on its own samples and its quad's
+// barycentrics, so threads never wait on each other.  The block holding the
+// tile's centre quad also updates it (on screen or not) and the tile
+// luminance.
+__global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
+    __shared__ uint32_t st[G_COUNT * 256];
+    uint32_t slot = blockIdx.x >> 4, sub = blockIdx.x & 15u;
+    uint32_t s = Q.start[slot], e = min(Q.end[slot], Q.cap);
+    if (e <= s) return;
+    const DevView &V = P.views[find_view_tile(P, slot)];
+    const DevScene &S = *V.scene;
+    uint32_t ri = slot - V.lum_offset;
+    int rx0 = (int)(ri % (uint32_t)V.ref_tiles_x) * SR_REF_TILE, ry0 = (int)(ri / (uint32_t)V.ref_tiles_x) * SR_REF_TILE;
+    int bx0 = rx0 + 16 * (int)(sub & 3u), by0 = ry0 + 16 * (int)(sub >> 2);
+    bool centre_block = sub == 8u;            // holds the quad at (0, 32) of the tile
+    if (!(bx0 < V.W && by0 < V.H) && !centre_block) return;
+    const int tid = threadIdx.x, lane = tid & 63, qbase = lane & ~3;
+    const int q = tid >> 2, sl = tid & 3, sx = sl & 1, sy = sl >> 1;
+    const int qpx = bx0 + 2 * (q & 7), qpy = by0 + 2 * (q >> 3);
+    const int px = qpx + sx, py = qpy + sy;
+    const bool on_screen = px < V.W && py < V.H;
+    const bool centre = centre_block && q == 0;
+    const bool valid = on_screen || centre;
+    size_t o = V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px;
+    float4 *cc = P.center_color + 4 * (size_t)slot + sl;
+    f3 cur = mk3(0.0f, 0.0f, 0.0f);
+    float depth = INFINITY;
+    if (centre) {
+        float4 c = *cc;
+        cur = mk3(c.x, c.y, c.z);
+        depth = c.w;
+    } else if (on_screen) {
+        cur = mk3(P.out_color[o], P.out_color[P.color_plane + o], P.out_color[2 * P.color_plane + o]);
+        depth = P.out_depth[o];
+    }
+    for (uint32_t b0 = s; b0 < e; b0 += 256u) {
+        uint32_t nb = min(e - b0, 256u);
+        __syncthreads();
+        if ((uint32_t)tid < nb) {
+            uint32_t id = Q.list[b0 + (uint32_t)tid];
+            TriSetup ts;
+            tri_setup(P.geo[id], P.raster + 4 * (size_t)id, rx0, ry0, ts);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                st[(G_A + k) * 256 + tid] = __float_as_uint(ts.A[k]);
+                st[(G_B + k) * 256 + tid] = __float_as_uint(ts.B[k]);
+                st[(G_C + k) * 256 + tid] = __float_as_uint(ts.C[k]);
+                st[(G_SX + k) * 256 + tid] = __float_as_uint(ts.SX[k]);
+                st[(G_SY + k) * 256 + tid] = __float_as_uint(ts.SY[k]);
+            }
+            st[G_OOA * 256 + tid] = __float_as_uint(ts.ooa);
+            st[G_OA * 256 + tid] = __float_as_uint(ts.oa);
+            st[G_ODA * 256 + tid] = __float_as_uint(ts.oda);
+            st[G_ODB * 256 + tid] = __float_as_uint(ts.odb);
+            st[G_ZA * 256 + tid] = __float_as_uint(ts.za);
+            st[G_ZDA * 256 + tid] = __float_as_uint(ts.zda);
+            st[G_ZDB * 256 + tid] = __float_as_uint(ts.zdb);
+            st[G_ID * 256 + tid] = id;
+            st[G_ORG * 256 + tid] = (uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16);
+            st[G_LIM * 256 + tid] = (uint32_t)ts.cmxx | ((uint32_t)ts.cmxy << 16);
+            st[G_COARSE * 256 + tid] = ts.coarse ? 1u : 0u;
+        }
+        __syncthreads();
+        for (uint32_t k = 0; k < nb; k++) {
+#define GF(f) __uint_as_float(st[(f) * 256 + k])
+            uint32_t org = st[G_ORG * 256 + k], lim = st[G_LIM * 256 + k];
+            int ox0 = (int)(org & 0xFFFFu), oy0 = (int)(org >> 16), cmxx = (int)(lim & 0xFFFFu), cmxy = (int)(lim >> 16);
+            if (!(max(ox0, bx0) < min(cmxx, bx0 + 16) && max(oy0, by0) < min(cmxy, by0 + 16))) continue;
+            bool fm = false;
+            float b1 = 0.0f, b2 = 0.0f;
+            if (qpx >= ox0 && qpx < cmxx && qpy >= oy0 && qpy < cmxy) {
+                TriSetup ts;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    ts.A[m] = GF(G_A + m);
+                    ts.B[m] = GF(G_B + m);
+                    ts.C[m] = GF(G_C + m);
+                    ts.SX[m] = GF(G_SX + m);
+                    ts.SY[m] = GF(G_SY + m);
+                }
+                bool coarse = st[G_COARSE * 256 + k] != 0u;
+                int bx = ox0, by = oy0;
+                if (coarse) {
+                    bx = ox0 + ((qpx - ox0) & ~15);
+                    by = oy0 + ((qpy - oy0) & ~15);
+                }
+                if (!(coarse && block_out(ts, bx, by))) {
+                    int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
+                    float w0 = edge_chain(ts, 0, bx, by, sx, sy, i, j);
+                    float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
+                    float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
+                    float ooa = GF(G_OOA);
+                    b1 = w1 * ooa;
+                    b2 = w2 * ooa;
+                    float wv = 1.0f / ((GF(G_OA) + b1 * GF(G_ODA)) + b2 * GF(G_ODB));
+                    float z = ((GF(G_ZA) + b1 * GF(G_ZDA)) + b2 * GF(G_ZDB)) * wv;
+                    fm = valid && w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z <= depth;
+                }
+            }
+#undef GF
+            if (__ballot(fm)) {
+                float qb1[4], qb2[4];
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    qb1[m] = __shfl(b1, qbase + m, 64);
+                    qb2[m] = __shfl(b2, qbase + m, 64);
+                }
+                if (fm) {
+                    uint32_t id = st[G_ID * 256 + k];
+                    const float4 *R = P.raster + 4 * (size_t)id;
+                    CV tv[3];
+                    float4 Sh[7], Tn[3];
+                    load_tri_cv(P.src, R, tv);
+                    bool nmap = (P.geo[id].z & SR_GEO_NORMAL_MAP) != 0;
+                    tri_shade_attrs(tv, R, nmap, Sh, Tn);
+                    cur = shade_pixel<true>(S, V, R, Sh, Tn, nmap, qb1, qb2, sl, cur);
+                }
+            }
+        }
+    }
+    if (on_screen) {
+        P.out_color[o] = cur.x;
+        P.out_color[P.color_plane + o] = cur.y;
+        P.out_color[2 * P.color_plane + o] = cur.z;
+    }
+    if (centre) *cc = make_float4(cur.x, cur.y, cur.z, depth);
+    // centre-quad luminance after the translucent pass (src/tilerasterizer.rs:103-106)
+    float lum = (cur.x * 0.2126f + cur.y * 0.7152f) + cur.z * 0.0722f;
+    float l0 = __shfl(lum, qbase, 64), l1 = __shfl(lum, qbase + 1, 64);
+    float l2 = __shfl(lum, qbase + 2, 64), l3 = __shfl(lum, qbase + 3, 64);
+    if (centre && sl == 0) P.out_lum[slot] = (((l0 + l1) + l2) + l3) * 0.25f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1905,7 +2321,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr);
     hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.status,
-                       a.bin_count, a.total_items, a.tri_counts + a.total_in);
+                       a.bin_count, a.total_items, a.tri_counts + a.total_in,
+                       a.has_translucent ? a.tr_count : nullptr, a.tr_slots);
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv, a.tri_counts, a.clip_queue, a.clip_cap, a.status);
@@ -1931,17 +2348,31 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(a.rec_cap, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
                        a.bin_end, a.list, a.pair_cap, a.status, a.tile_size);
+    TriSource src;
+    src.views = a.views; src.draws = a.draws; src.xp = a.xv; src.xa = a.xv + a.total_xv;
+    src.xt = a.any_tangents ? a.xt : nullptr;
+    src.clip_buf = a.clip_buf;
+    if (a.has_translucent) {
+        uint32_t g = grid_for(a.rec_cap, 256, GRID_CAP);
+        hipLaunchKernelGGL(k_tr_bins, dim3(g), dim3(256), 0, st, a.views, a.rec_geo, a.rec_cap, a.tr_count, nullptr,
+                           0u, a.status);
+        e = launch_scan(a.tr_count, a.tr_slots, a.tr_start, a.tr_end, a.scan_partials, &a.status->total_tr_pairs, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_tr_bins, dim3(g), dim3(256), 0, st, a.views, a.rec_geo, a.rec_cap, a.tr_end, a.tr_list,
+                           a.tr_cap, a.status);
+        hipLaunchKernelGGL(k_tr_sort, dim3(a.tr_slots), dim3(256), 0, st, a.tr_start, a.tr_end, a.tr_list, a.tr_cap,
+                           a.tr_scratch, a.rec_raster, src);
+    }
     if (a.events[2]) hipEventRecord(a.events[2], st);
     RasterArgs P;
     P.views = a.views; P.num_views = a.num_views; P.total_items = a.total_items;
     P.bin_start = a.bin_start; P.bin_end = a.bin_end; P.list = a.list; P.pair_cap = a.pair_cap;
     P.geo = a.rec_geo; P.raster = a.rec_raster;
-    P.src.views = a.views; P.src.draws = a.draws; P.src.xp = a.xv; P.src.xa = a.xv + a.total_xv;
-    P.src.xt = a.any_tangents ? a.xt : nullptr;
-    P.src.clip_buf = a.clip_buf;
+    P.src = src;
     P.tri_base = a.tri_base; P.out_ids = a.out_ids; P.out_depth = a.out_depth; P.out_color = a.out_color;
     P.color_plane = a.color_plane; P.out_rgba = a.out_rgba; P.out_lum = a.out_lum; P.status = a.status;
     P.center_ids = a.center_ids;
+    P.center_color = a.has_translucent ? a.center_color : nullptr;
     P.persistent = a.persistent_blocks > 0;
     uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : a.total_items;
     switch (a.tile_size) {
@@ -1954,6 +2385,11 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         case 8: hipLaunchKernelGGL((k_shade<8, 64>), dim3(a.total_items), dim3(64), 0, st, P); break;
         case 32: hipLaunchKernelGGL((k_shade<32, 256>), dim3(a.total_items), dim3(256), 0, st, P); break;
         default: hipLaunchKernelGGL((k_shade<16, 256>), dim3(a.total_items), dim3(256), 0, st, P); break;
+    }
+    if (a.has_translucent) {
+        TrArgs Q;
+        Q.start = a.tr_start; Q.end = a.tr_end; Q.list = a.tr_list; Q.cap = a.tr_cap;
+        hipLaunchKernelGGL(k_translucent, dim3(a.tr_slots * 16u), dim3(256), 0, st, P, Q);
     }
     if (a.events[3]) hipEventRecord(a.events[3], st);
     return hipGetLastError();

@@ -106,7 +106,7 @@ SR_API int sr_renderer_render_scene(sr_renderer *r, const sr_scene *scene, const
     if (scene->device != r->device) return fail(SR_ERR_INVALID_ARGUMENT, "scene and renderer are on different devices");
     std::vector<DevDraw> draws;
     uint32_t in_tri = 0, xv = 0;
-    int rc = build_draws(scene, *camera, 0, in_tri, xv, draws);
+    int rc = build_draws(scene, *camera, 0, in_tri, xv, draws, true);
     if (rc) return rc;
     std::vector<DevView> views(1);
     int T = r->pipe.tile_size;
@@ -315,7 +315,7 @@ SR_API int sr_batch_render(sr_batch *b, const sr_view *views, int32_t n) {
             return fail(SR_ERR_INVALID_ARGUMENT, "view %d is invalid", i);
         if (v.scene->device != b->device) return fail(SR_ERR_INVALID_ARGUMENT, "view %d scene on another device", i);
         uint32_t tri0 = in_tri;
-        int rc = build_draws(v.scene, v.camera, i, in_tri, xv, draws);
+        int rc = build_draws(v.scene, v.camera, i, in_tri, xv, draws, false);
         if (rc) return rc;
         fill_view(hv[i], v.scene, v.camera, v.width, v.height, T, items, tri0, fb, lum,
                   v.exposure > 0.0f ? v.exposure : DEFAULT_EXPOSURE, 1u);

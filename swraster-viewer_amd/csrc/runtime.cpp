@@ -131,7 +131,7 @@ static int sphere_frustum(const Sphere &s, const sr_camera &c) {
 
 // Draw list of one view in canonical order (src/renderer.rs:201-468).
 int build_draws(const sr_scene *scene, const sr_camera &cam, int view, uint32_t &in_tri, uint32_t &xv,
-                std::vector<DevDraw> &out) {
+                std::vector<DevDraw> &out, bool allow_translucent) {
     const size_t nn = scene->nodes.size();
     std::vector<int> order(nn);
     std::vector<float> key(nn);
@@ -161,7 +161,8 @@ int build_draws(const sr_scene *scene, const sr_camera &cam, int view, uint32_t 
                 Sphere bw = transform_sphere(n.transform, p.bs);
                 int ft = sphere_frustum(bw, cam);
                 if (ft == FT_OUTSIDE) continue;
-                if (pass == 1) return fail(SR_ERR_UNSUPPORTED, "translucent primitives are not supported on the GPU path");
+                if (pass == 1 && !allow_translucent)
+                    return fail(SR_ERR_UNSUPPORTED, "translucent primitives are not supported on this path");
                 DevDraw d;
                 memset(&d, 0, sizeof(d));
                 memcpy(d.mvp, mvp, 64);
@@ -174,7 +175,7 @@ int build_draws(const sr_scene *scene, const sr_camera &cam, int view, uint32_t 
                 d.xv_base = xv;
                 d.view = view;
                 d.material = p.material;
-                d.flags = (ft == FT_INTERSECTING ? SR_DRAW_CLIP : 0u) | SR_DRAW_OPAQUE |
+                d.flags = (ft == FT_INTERSECTING ? SR_DRAW_CLIP : 0u) | (pass == 0 ? SR_DRAW_OPAQUE : 0u) |
                           (p.tangents ? SR_DRAW_TANGENTS : 0u);
                 in_tri += p.num_tri;
                 xv += p.num_vtx;
@@ -205,7 +206,8 @@ void Pipeline::destroy() {
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
     DevBuf *bufs[] = {&views, &draws, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
-                      &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf};
+                      &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf, &tr_count,
+                      &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color};
     for (DevBuf *b : bufs) b->release();
     stage.release();
     status_host.release();
@@ -231,11 +233,13 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
     k.n[0] = (uint64_t)a.num_views; k.n[1] = (uint64_t)a.num_draws; k.n[2] = a.total_xv; k.n[3] = a.total_in;
     k.n[4] = a.total_items; k.n[5] = (uint64_t)a.tile_size; k.n[6] = (uint64_t)a.any_tangents; k.n[7] = a.rec_cap;
     k.n[8] = a.pair_cap; k.n[9] = a.clip_cap; k.n[10] = (uint64_t)a.persistent_blocks; k.n[11] = vbytes;
-    k.n[12] = dbytes; k.n[13] = a.color_plane;
+    k.n[12] = dbytes; k.n[13] = a.color_plane; k.n[14] = (uint64_t)a.has_translucent;
+    k.n[15] = ((uint64_t)a.tr_cap << 32) | a.tr_slots;
     const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
                           a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
-                          status_host.ptr};
+                          status_host.ptr, a.tr_count, a.tr_start, a.tr_end, a.tr_list, a.tr_scratch,
+                          a.center_color};
     static_assert(sizeof(ptrs) / sizeof(ptrs[0]) <= sizeof(k.p) / sizeof(k.p[0]), "graph key too small");
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) k.p[i] = (uint64_t)(uintptr_t)ptrs[i];
     if (!graph_exec || memcmp(&k, &graph_key, sizeof(k)) != 0) {
@@ -287,6 +291,15 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     for (const DevView &v : hv)
         lum_slots = std::max<size_t>(lum_slots, (size_t)v.lum_offset + (size_t)v.ref_tiles_x * v.ref_tiles_y);
     if ((r = center.ensure(lum_slots * 16))) return r;
+    bool has_tr = false;
+    for (const DevDraw &d : hd) has_tr = has_tr || !(d.flags & SR_DRAW_OPAQUE);
+    if (has_tr) {
+        if ((r = tr_count.ensure(lum_slots * 4))) return r;
+        if ((r = tr_start.ensure(lum_slots * 4))) return r;
+        if ((r = tr_end.ensure(lum_slots * 4))) return r;
+        if ((r = center_color.ensure(lum_slots * 64))) return r;
+        if (tr_cap < 4096) tr_cap = 4096;
+    }
     size_t maxn = std::max<size_t>((size_t)total_in + 1, total_items);
     if ((r = partials.ensure(((maxn + 4095) / 4096 + 1) * 4))) return r;
     if (rec_cap < total_in + total_in / 8 + 1024) rec_cap = total_in + total_in / 8 + 1024;
@@ -298,6 +311,8 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         if ((r = list.ensure((size_t)pair_cap * 4))) return r;
         if ((r = clip_queue.ensure((size_t)clip_cap * 4))) return r;
         if ((r = clip_buf.ensure((size_t)clip_cap * 16 * 16 * 4))) return r;
+        if (has_tr && (r = tr_list.ensure((size_t)tr_cap * 4))) return r;
+        if (has_tr && (r = tr_scratch.ensure((size_t)tr_cap * 8))) return r;
         FrameArgs a;
         memset(&a, 0, sizeof(a));
         a.views = (const DevView *)views.ptr;
@@ -333,6 +348,17 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.out_rgba = out_rgba;
         a.out_lum = out_lum;
         a.center_ids = (uint32_t *)center.ptr;
+        if (has_tr) {
+            a.has_translucent = 1;
+            a.tr_slots = (uint32_t)lum_slots;
+            a.tr_count = (uint32_t *)tr_count.ptr;
+            a.tr_start = (uint32_t *)tr_start.ptr;
+            a.tr_end = (uint32_t *)tr_end.ptr;
+            a.tr_list = (uint32_t *)tr_list.ptr;
+            a.tr_scratch = (unsigned long long *)tr_scratch.ptr;
+            a.tr_cap = tr_cap;
+            a.center_color = (float4 *)center_color.ptr;
+        }
         a.persistent_blocks = persistent_blocks;
         if (use_graph) {
             if ((r = launch_graph(a, vbytes, hd.empty() ? 0 : hd.size() * sizeof(DevDraw)))) return r;
@@ -352,9 +378,11 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         last_pairs = s.total_pairs;
         if (s.clip_overflow)
             return fail(SR_ERR_UNSUPPORTED, "a frustum-clipped polygon exceeded 16 vertices (degenerate rounding)");
+        bool tr_ok = !has_tr || (!s.overflow_tr && s.total_tr_pairs <= tr_cap);
         if (!s.overflow_records && !s.overflow_pairs && !s.overflow_clip && s.total_records <= rec_cap &&
-            s.total_pairs <= pair_cap)
+            s.total_pairs <= pair_cap && tr_ok)
             return SR_OK;
+        if (!tr_ok) tr_cap = s.total_tr_pairs + s.total_tr_pairs / 8 + 4096;
         if (s.overflow_clip || s.clip_queued > clip_cap) clip_cap = s.clip_queued + s.clip_queued / 4 + 1024;
         if (s.total_records > rec_cap || s.overflow_records) rec_cap = s.total_records + s.total_records / 8 + 1024;
         if (s.total_pairs > pair_cap || s.overflow_pairs) pair_cap = s.total_pairs + s.total_pairs / 8 + 4096;

@@ -60,8 +60,9 @@ struct Pipeline {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     DevBuf views, draws, xv, xt, tri_counts, tri_base, bin_count, bin_start, bin_end, partials, list;
     DevBuf rec_geo, rec_raster, status, center, clip_queue, clip_buf;
+    DevBuf tr_count, tr_start, tr_end, tr_list, tr_scratch, center_color;
     PinBuf stage, status_host;
-    uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0;
+    uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0, tr_cap = 0;
     uint32_t last_records = 0, last_pairs = 0;
     struct GraphKey {
         uint64_t n[16];
@@ -92,8 +93,10 @@ struct sr_scene {
 };
 
 namespace sr {
+// allow_translucent: translucent primitives become draws without SR_DRAW_OPAQUE
+// (else they fail with SR_ERR_UNSUPPORTED).
 int build_draws(const sr_scene *scene, const sr_camera &cam, int view, uint32_t &in_tri, uint32_t &xv,
-                std::vector<DevDraw> &out);
+                std::vector<DevDraw> &out, bool allow_translucent);
 void fill_view(DevView &v, const sr_scene *scene, const sr_camera &cam, int W, int H, int T, uint32_t item_base,
                uint32_t in_tri_base, uint64_t fb_offset, uint32_t lum_offset, float exposure, uint32_t flags);
 }  // namespace sr

@@ -243,7 +243,7 @@ bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     for (int i = 0; i < n; i++) {
         uint32_t tri0 = s.in_tri, xv0 = s.xv;
         size_t nd = s.draws.size();
-        int rc = build_draws(rs[i], rq[i].camera, i, s.in_tri, s.xv, s.draws);
+        int rc = build_draws(rs[i], rq[i].camera, i, s.in_tri, s.xv, s.draws, false);
         if (rc) {                      // the view renders nothing; the request reports rc
             st[i] = rc;
             s.draws.resize(nd);

@@ -310,6 +310,68 @@ def test_alpha_tested_materials(sr, orc, scenes, env):
     assert (g["ids"] != sr.NO_ID).sum() < (opaque["ids"] != sr.NO_ID).sum(), "the mask removed nothing"
 
 
+def _world_quad(x0, x1, y0, y1, z):
+    """Axis-aligned rectangle at world z facing +z (toward the C1 camera)."""
+    pos = np.array([[x0, y1, z], [x1, y1, z], [x1, y0, z], [x0, y0, z]], np.float32)
+    return {"positions": pos, "normals": np.tile(np.array([0, 0, 1], np.float32), (4, 1)),
+            "texcoords": np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32), "tangents": None,
+            "indices": np.array([0, 2, 1, 0, 3, 2], np.uint32)}
+
+
+@pytest.mark.parametrize("size", [(640, 360), (640, 336)])
+def test_translucent_materials(sr, orc, scenes, env, size):
+    """Translucent primitives (src/tilerasterizer.rs:92-101, src/shader.rs:
+    66-76): per 64-px tile lists sorted back to front (src/bumpqueue.rs:
+    155-200) and blended over the opaque frame.  C1 with a third of the
+    spheres translucent (one through a transmission texture), two
+    camera-facing quads at one depth that overlap (tied sort keys, so the
+    quicksort's own order decides the blend), and a band wider than the
+    frustum (clipped; a width of 640 = 10 x 64 lists it a second time in the
+    first tile of the next row).  336 rows leave the last tile row's centre
+    quads off screen."""
+    W, H = size
+    scene, spec = scenes.make_c1(width=W, height=H)
+    for i, m in enumerate(scene.materials):
+        if i % 3 == 1:
+            m["is_translucent"] = 1
+            m["transmission"] = 0.2 + 0.05 * i
+            if i == 4:
+                m["transmission_texture_tex"] = (0, sr.SR_WRAP_REPEAT, sr.SR_WRAP_REPEAT)
+
+    def add(prim, mat):
+        prim["material"] = len(scene.materials)
+        scene.materials.append(mat)
+        scene.primitives.append(prim)
+        scene.meshes.append((len(scene.primitives) - 1, 1))
+        scene.nodes.append((np.eye(4, dtype=np.float32), len(scene.meshes) - 1))
+
+    def glass(base, t):
+        return {"base_color": base + (1.0,), "metallic": 0.1, "roughness": 0.3, "is_translucent": 1,
+                "transmission": t, "base_color_tex": (0, sr.SR_WRAP_REPEAT, sr.SR_WRAP_REPEAT)}
+
+    add(_world_quad(-2.0, 0.5, -1.5, 1.0, 8.0), glass((0.9, 0.3, 0.2), 0.6))
+    add(_world_quad(-0.5, 2.0, -1.0, 1.5, 8.0), glass((0.2, 0.4, 0.9), 0.5))
+    add(_world_quad(-40.0, 40.0, -0.75, 0.25, 6.0), glass((0.7, 0.7, 0.3), 0.35))
+    scene.rebuild()
+    scene.prepare(env, cache_key=None, grid=32)
+    cam = scenes.make_camera(spec)
+    g = render_gpu(sr, sr.Scene(scene), cam, W, H)
+    o = render_oracle(orc, scene, cam, W, H)
+    compare(sr, g, o, f"translucent c1 {W}x{H}")
+    assert np.array_equal(bits(g["rgb"]), bits(o["rgb"])), "linear colour differs"
+    assert np.array_equal(bits(g["lum"]), bits(o["lum"])), "tile luminance differs"
+    # the layers are there: dropping the translucent quads changes many pixels
+    environment, voxels = scene.environment, scene.voxels
+    for k in range(3):
+        scene.nodes.pop()
+    scene.rebuild()
+    scene.attach(environment, voxels)        # the same lighting inputs
+    g0 = render_gpu(sr, sr.Scene(scene), cam, W, H)
+    changed = int(np.any(g0["rgb"] != g["rgb"], -1).sum())
+    print(f"[translucent c1 {W}x{H}] the three quads change {changed} px")
+    assert changed > 20000
+
+
 def test_gpu_matches_committed_c1_frame(sr, scenes, env):
     """The GPU reproduces the committed oracle frame (tests/golden/) without
     running the oracle."""

@@ -76,3 +76,37 @@ def exact_coverage(tri, width, height):
 def signed_area(tri):
     (x0, y0), (x1, y1), (x2, y2) = tri
     return (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0)
+
+
+def quad_layers_scene(sr, layers, width, height, fov=np.pi / 3, far=150.0):
+    """Camera-facing screen rectangles at given view depths, one primitive,
+    mesh and node each.  layers: list of ((x0, y0, x1, y1) in pixels,
+    depth, material dict).  Texcoords run 0..1 across each rectangle; the
+    camera sits at the origin looking down -z."""
+    prims, meshes, nodes, mats = [], [], [], []
+    for i, ((x0, y0, x1, y1), depth, mat) in enumerate(layers):
+        corners = unproject(np.array([x0, x1, x1, x0], np.float64), np.array([y0, y0, y1, y1], np.float64),
+                            np.full(4, depth), width, height, fov).astype(np.float32)
+        uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+        # corners TL, TR, BR, BL; (TL, BR, TR) and (TL, BL, BR) face the camera
+        idx = np.array([0, 2, 1, 0, 3, 2], np.uint32)
+        prims.append({"positions": corners, "normals": np.tile(np.array([0, 0, 1], np.float32), (4, 1)),
+                      "texcoords": uv, "tangents": None, "indices": idx, "material": i})
+        meshes.append((i, 1))
+        nodes.append((np.eye(4, dtype=np.float32), i))
+        mats.append(mat)
+    scene = sr.SceneData(prims, meshes, nodes, mats, [], name="quad_layers")
+    cam = {"position": [0.0, 0.0, 0.0], "yaw": 0.0, "pitch": 0.0, "fov": float(fov), "width": width,
+           "height": height, "far": float(far)}
+    return scene, cam
+
+
+def dark_environment(sr, sky=(128, 160, 200)):
+    """Uniform sky, black specular cubemap, zero SH irradiance: shading then
+    reduces to the emissive term (and the translucent blend over it)."""
+    c = sr.pack_rgba(np.array([[sky[0], sky[1], sky[2], 255]], np.uint8))[0]
+    black = sr.pack_rgba(np.array([[0, 0, 0, 255]], np.uint8))[0]
+    cube = np.full((6, 4, 4), c, np.uint32)
+    spec = np.full(3 * 6 * 16, black, np.uint32)
+    lut = np.full(64, sr.pack_rgba(np.array([[200, 40, 0, 255]], np.uint8))[0], np.uint32)
+    return sr.Environment(cube, spec, 3, lut, np.zeros((4, 3), np.float32))
