@@ -13,6 +13,17 @@
 
 #include <cstdint>
 
+// Pointers into device memory held in structs.  In the device pass they are
+// global-address-space pointers, so loads through them are global (or, when
+// uniform, scalar) loads rather than generic FLAT loads, which also occupy
+// the LDS/scalar wait counter; everywhere else they are plain pointers of the
+// same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SR_GLOBAL __attribute__((address_space(1)))
+#else
+#define SR_GLOBAL
+#endif
+
 #define SR_MAX_MIPS 16
 #define SR_REF_TILE 64          // reference raster tile (src/renderer.rs:14)
 #define SR_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
@@ -61,22 +72,22 @@ struct DevMaterial {
 };
 
 struct DevScene {
-    const float *px, *py, *pz;    // positions
-    const float *nx, *ny, *nz;    // normals
-    const float *tu, *tv;         // texcoords
-    const float *tangent;         // float4 per vertex (may be null)
-    const uint32_t *indices;
-    const DevMaterial *materials;
-    const DevTexture *textures;
-    const uint32_t *texels;
-    const uint32_t *cubemap;
+    const SR_GLOBAL float *px, *py, *pz;    // positions
+    const SR_GLOBAL float *nx, *ny, *nz;    // normals
+    const SR_GLOBAL float *tu, *tv;         // texcoords
+    const SR_GLOBAL float *tangent;         // float4 per vertex (may be null)
+    const SR_GLOBAL uint32_t *indices;
+    const SR_GLOBAL DevMaterial *materials;
+    const SR_GLOBAL DevTexture *textures;
+    const SR_GLOBAL uint32_t *texels;
+    const SR_GLOBAL uint32_t *cubemap;
     uint32_t cube_size;
     uint32_t spec_size, spec_mips;
-    const uint32_t *specular;
-    const uint32_t *lut;
+    const SR_GLOBAL uint32_t *specular;
+    const SR_GLOBAL uint32_t *lut;
     uint32_t lut_size;
-    const float *vox_light;       // constant-SH form: per cell, w0 at its 8 trilinear corners
-    const float *vox_sh4;         // full form: 16 floats per voxel (or null)
+    const SR_GLOBAL float *vox_light;       // constant-SH form: per cell, w0 at its 8 trilinear corners
+    const SR_GLOBAL float *vox_sh4;         // full form: 16 floats per voxel (or null)
     int32_t vox_dims[3];
     float vox_min[3];
     float vox_size[3];            // (max - min) / dims, as src/voxelgrid.rs:154-161
@@ -88,7 +99,7 @@ struct DevScene {
 
 // One view = one frame of one camera at one resolution.
 struct DevView {
-    const DevScene *scene;
+    const SR_GLOBAL DevScene *scene;
     float cam_pos[3];
     float one_over_w, one_over_h;
     float sky[16];                // skybox_matrix_transposed

@@ -174,51 +174,72 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
 // ---------------------------------------------------------------------------
 // k_transform: one thread per (draw, vertex)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_transform(const DevView *views, const DevDraw *draws, int ndraws,
-                                                   uint32_t total, float4 *xp, float4 *xa, float4 *xt) {
+// Transform of vertex i of draw D (render_primitive_inner, src/renderer.rs:
+// 496-521): clip = MVP p, world = M p, normal = M3 n, tangent = M3 t.
+__device__ __forceinline__ void transform_vertex(const DevDraw &D, const DevScene &S, uint32_t i, float4 *xp,
+                                                 float4 *xa, float4 *xt) {
+    uint32_t s = D.vtx_base + (i - D.xv_base);
+    float x = S.px[s], y = S.py[s], z = S.pz[s];
+    float nx = S.nx[s], ny = S.ny[s], nz = S.nz[s];
+    float c[4], w[3], n[3];
+    for (int r = 0; r < 4; r++) {
+        float acc = D.mvp[r] * x;
+        acc = acc + D.mvp[4 + r] * y;
+        acc = acc + D.mvp[8 + r] * z;
+        acc = acc + D.mvp[12 + r] * 1.0f;
+        c[r] = acc;
+    }
+    for (int r = 0; r < 3; r++) {
+        float acc = D.model[r] * x;
+        acc = acc + D.model[4 + r] * y;
+        acc = acc + D.model[8 + r] * z;
+        acc = acc + D.model[12 + r] * 1.0f;
+        w[r] = acc;
+        float an = D.model[r] * nx;
+        an = an + D.model[4 + r] * ny;
+        an = an + D.model[8 + r] * nz;
+        n[r] = an;
+    }
+    xp[i] = make_float4(c[0], c[1], c[2], c[3]);
+    xa[2 * (size_t)i] = make_float4(w[0], w[1], w[2], n[0]);
+    xa[2 * (size_t)i + 1] = make_float4(n[1], n[2], S.tu[s], S.tv[s]);
+    if (xt) {
+        if ((D.flags & SR_DRAW_TANGENTS) && S.tangent) {
+            float4 t = reinterpret_cast<const float4 *>(S.tangent)[s];
+            float tw[3];
+            for (int r = 0; r < 3; r++) {
+                float acc = D.model[r] * t.x;
+                acc = acc + D.model[4 + r] * t.y;
+                acc = acc + D.model[8 + r] * t.z;
+                tw[r] = acc;
+            }
+            xt[i] = make_float4(tw[0], tw[1], tw[2], t.w);
+        } else {
+            xt[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
+// Wave-uniform copy of a per-lane index when every active lane holds the same
+// value (then loads through it are scalar), else -1.
+__device__ __forceinline__ int wave_uniform(int d) {
+    int d0 = __builtin_amdgcn_readfirstlane(d);
+    return __ballot(d != d0) == 0ull ? d0 : -1;
+}
+
+__global__ void __launch_bounds__(256) k_transform(const DevView *__restrict__ views,
+                                                   const DevDraw *__restrict__ draws, int ndraws, uint32_t total,
+                                                   float4 *__restrict__ xp, float4 *__restrict__ xa,
+                                                   float4 *__restrict__ xt) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         int d = find_last_le(draws, ndraws, i, &DevDraw::xv_base);
-        const DevDraw &D = draws[d];
-        const DevScene &S = *views[D.view].scene;
-        uint32_t s = D.vtx_base + (i - D.xv_base);
-        float x = S.px[s], y = S.py[s], z = S.pz[s];
-        float nx = S.nx[s], ny = S.ny[s], nz = S.nz[s];
-        float c[4], w[3], n[3];
-        for (int r = 0; r < 4; r++) {
-            float acc = D.mvp[r] * x;
-            acc = acc + D.mvp[4 + r] * y;
-            acc = acc + D.mvp[8 + r] * z;
-            acc = acc + D.mvp[12 + r] * 1.0f;
-            c[r] = acc;
-        }
-        for (int r = 0; r < 3; r++) {
-            float acc = D.model[r] * x;
-            acc = acc + D.model[4 + r] * y;
-            acc = acc + D.model[8 + r] * z;
-            acc = acc + D.model[12 + r] * 1.0f;
-            w[r] = acc;
-            float an = D.model[r] * nx;
-            an = an + D.model[4 + r] * ny;
-            an = an + D.model[8 + r] * nz;
-            n[r] = an;
-        }
-        xp[i] = make_float4(c[0], c[1], c[2], c[3]);
-        xa[2 * (size_t)i] = make_float4(w[0], w[1], w[2], n[0]);
-        xa[2 * (size_t)i + 1] = make_float4(n[1], n[2], S.tu[s], S.tv[s]);
-        if (xt) {
-            if ((D.flags & SR_DRAW_TANGENTS) && S.tangent) {
-                float4 t = reinterpret_cast<const float4 *>(S.tangent)[s];
-                float tw[3];
-                for (int r = 0; r < 3; r++) {
-                    float acc = D.model[r] * t.x;
-                    acc = acc + D.model[4 + r] * t.y;
-                    acc = acc + D.model[8 + r] * t.z;
-                    tw[r] = acc;
-                }
-                xt[i] = make_float4(tw[0], tw[1], tw[2], t.w);
-            } else {
-                xt[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
+        int du = wave_uniform(d);
+        if (du >= 0) {
+            const DevDraw &D = draws[du];
+            transform_vertex(D, *views[D.view].scene, i, xp, xa, xt);
+        } else {
+            const DevDraw &D = draws[d];
+            transform_vertex(D, *views[D.view].scene, i, xp, xa, xt);
         }
     }
 }
@@ -397,12 +418,52 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &excl, 
     return total;
 }
 
+// Each wave owns a contiguous segment of 64 x SCAN_ITEMS elements and reads
+// it in SCAN_ITEMS coalesced rows (element seg + k * 64 + lane); ex[k] gets
+// the exclusive prefix of its element within the segment.  Returns the
+// segment total.
+__device__ __forceinline__ uint32_t wave_scan_segment(const uint32_t *in, size_t seg, size_t n,
+                                                      uint32_t ex[SCAN_ITEMS]) {
+    const int lane = threadIdx.x & 63;
+    uint32_t v[SCAN_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        size_t i = seg + (size_t)k * 64 + (size_t)lane;
+        v[k] = i < n ? in[i] : 0u;
+    }
+    uint32_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint32_t inc = wave_incl_scan(v[k]);
+        ex[k] = carry + inc - v[k];
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    }
+    return carry;
+}
+
+__device__ __forceinline__ void wave_store_segment(uint32_t *out, uint32_t *out2, size_t seg, size_t n, uint32_t base,
+                                                   const uint32_t ex[SCAN_ITEMS]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        size_t i = seg + (size_t)k * 64 + (size_t)lane;
+        if (i < n) {
+            out[i] = base + ex[k];
+            if (out2) out2[i] = base + ex[k];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_partials(const uint32_t *in, uint32_t n, uint32_t *partials) {
     __shared__ uint32_t sh[17];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    size_t seg = (size_t)blockIdx.x * SCAN_TILE + (size_t)wave * (64 * SCAN_ITEMS);
     uint32_t s = 0;
-    for (int k = 0; k < SCAN_ITEMS; k++)
-        if (base + k < n) s += in[base + k];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        size_t i = seg + (size_t)k * 64 + (size_t)lane;
+        if (i < n) s += in[i];
+    }
     uint32_t ex;
     uint32_t total = block_excl_scan(s, ex, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
@@ -424,24 +485,16 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t 
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, uint32_t n, const uint32_t *partials,
                                                            uint32_t *out, uint32_t *out2) {
-    __shared__ uint32_t sh[17];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        v[k] = base + k < n ? in[base + k] : 0u;
-        s += v[k];
-    }
-    uint32_t ex;
-    block_excl_scan(s, ex, sh);
-    uint32_t acc = partials[blockIdx.x] + ex;
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        if (base + k < n) {
-            out[base + k] = acc;
-            if (out2) out2[base + k] = acc;
-        }
-        acc += v[k];
-    }
+    __shared__ uint32_t sh[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    size_t seg = (size_t)blockIdx.x * SCAN_TILE + (size_t)wave * (64 * SCAN_ITEMS);
+    uint32_t ex[SCAN_ITEMS];
+    uint32_t tot = wave_scan_segment(in, seg, n, ex);
+    if (lane == 0) sh[wave] = tot;
+    __syncthreads();
+    uint32_t base = partials[blockIdx.x];
+    for (int w = 0; w < wave; w++) base += sh[w];
+    wave_store_segment(out, out2, seg, n, base, ex);
 }
 
 // One workgroup scans arrays of up to SCAN_SINGLE_MAX elements in a single
@@ -451,29 +504,23 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, u
 
 __global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2,
                                                       uint32_t *total_out) {
-    __shared__ uint32_t sh[17];
+    __shared__ uint32_t sh[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t carry = 0;
     for (uint32_t c = 0; c < n; c += 1024u * SCAN_ITEMS) {
-        uint32_t base = c + threadIdx.x * SCAN_ITEMS;
-        uint32_t v[SCAN_ITEMS];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; k++) {
-            v[k] = base + k < n ? in[base + k] : 0u;
-            sum += v[k];
+        size_t seg = (size_t)c + (size_t)wave * (64 * SCAN_ITEMS);
+        uint32_t ex[SCAN_ITEMS];
+        uint32_t tot = wave_scan_segment(in, seg, n, ex);
+        if (lane == 0) sh[wave] = tot;
+        __syncthreads();
+        uint32_t base = carry, all = 0;
+        for (int w = 0; w < 16; w++) {
+            if (w < wave) base += sh[w];
+            all += sh[w];
         }
-        uint32_t ex;
-        uint32_t tot = block_excl_scan(sum, ex, sh);
-        uint32_t acc = carry + ex;
-#pragma unroll
-        for (int k = 0; k < SCAN_ITEMS; k++) {
-            if (base + k < n) {
-                out[base + k] = acc;
-                if (out2) out2[base + k] = acc;
-            }
-            acc += v[k];
-        }
-        carry += tot;
+        wave_store_segment(out, out2, seg, n, base, ex);
+        carry += all;
+        __syncthreads();
     }
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
@@ -531,11 +578,11 @@ __device__ __forceinline__ Rect bin_rect(const DevView &V, int mnx, int mny, int
 // k_setup: clip with attributes, snap, cull, bbox, setup records, bin counts
 // ---------------------------------------------------------------------------
 struct SetupOut {
-    uint4 *geo;
-    float4 *raster;
-    uint32_t *bin_count;
+    SR_GLOBAL uint4 *geo;
+    SR_GLOBAL float4 *raster;
+    SR_GLOBAL uint32_t *bin_count;
     uint32_t cap;
-    DevStatus *status;
+    SR_GLOBAL DevStatus *status;
     int T;
 };
 
@@ -644,12 +691,12 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
 
 // Sources of a set-up triangle's vertices.
 struct TriSource {
-    const DevView *views;
-    const DevDraw *draws;
-    const float4 *xp;         // transformed clip-space positions
-    const float4 *xa;         // transformed attributes (2 per vertex)
-    const float4 *xt;         // transformed tangents (or null)
-    const float *clip_buf;    // clipped polygons
+    const SR_GLOBAL DevView *views;
+    const SR_GLOBAL DevDraw *draws;
+    const SR_GLOBAL float4 *xp;         // transformed clip-space positions
+    const SR_GLOBAL float4 *xa;         // transformed attributes (2 per vertex)
+    const SR_GLOBAL float4 *xt;         // transformed tangents (or null)
+    const SR_GLOBAL float *clip_buf;    // clipped polygons
 };
 
 // The three vertices of the triangle whose raster record is R: the indexed
@@ -675,7 +722,8 @@ __device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R,
         for (int k = 0; k < 3; k++) {
             size_t i = (size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k];
             q[k][0] = T.xp[i]; q[k][1] = T.xa[2 * i]; q[k][2] = T.xa[2 * i + 1];
-            q[k][3] = tan ? T.xt[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            q[k][3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (tan) q[k][3] = T.xt[i];
         }
     }
 #pragma unroll
@@ -1374,26 +1422,26 @@ __device__ f3 skybox_pixel(const DevScene &S, const DevView &V, int qx, int qy, 
 // k_raster_shade: one workgroup per (view, tile)
 // ---------------------------------------------------------------------------
 struct RasterArgs {
-    const DevView *views;
+    const SR_GLOBAL DevView *views;
     int32_t num_views;
     uint32_t total_items;
-    const uint32_t *bin_start;
-    const uint32_t *bin_end;
-    const uint32_t *list;
+    const SR_GLOBAL uint32_t *bin_start;
+    const SR_GLOBAL uint32_t *bin_end;
+    const SR_GLOBAL uint32_t *list;
     uint32_t pair_cap;
-    const uint4 *geo;
-    const float4 *raster;
+    const SR_GLOBAL uint4 *geo;
+    const SR_GLOBAL float4 *raster;
     TriSource src;
-    const uint32_t *tri_base;
-    uint32_t *out_ids;
-    float *out_depth;
-    float *out_color;
+    const SR_GLOBAL uint32_t *tri_base;
+    SR_GLOBAL uint32_t *out_ids;
+    SR_GLOBAL float *out_depth;
+    SR_GLOBAL float *out_color;
     uint64_t color_plane;
-    uint32_t *out_rgba;
-    float *out_lum;
-    uint32_t *center_ids;         // 4 winners per 64px tile: its centre quad, on screen or not
-    float4 *center_color;         // 4 (colour, depth) per 64px tile for the translucent pass (or null)
-    DevStatus *status;
+    SR_GLOBAL uint32_t *out_rgba;
+    SR_GLOBAL float *out_lum;
+    SR_GLOBAL uint32_t *center_ids;         // 4 winners per 64px tile: its centre quad, on screen or not
+    SR_GLOBAL float4 *center_color;         // 4 (colour, depth) per 64px tile for the translucent pass (or null)
+    SR_GLOBAL DevStatus *status;
     int32_t persistent;
 };
 
@@ -2125,9 +2173,9 @@ __global__ void __launch_bounds__(256) k_tr_sort(const uint32_t *tr_start, const
 }
 
 struct TrArgs {
-    const uint32_t *start;
-    const uint32_t *end;
-    const uint32_t *list;
+    const SR_GLOBAL uint32_t *start;
+    const SR_GLOBAL uint32_t *end;
+    const SR_GLOBAL uint32_t *list;
     uint32_t cap;
 };
 
@@ -2335,8 +2383,9 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
                                &a.status->total_records, st);
     if (e != hipSuccess) return e;
     SetupOut so;
-    so.geo = a.rec_geo; so.raster = a.rec_raster;
-    so.bin_count = a.bin_count; so.cap = a.rec_cap; so.status = a.status; so.T = a.tile_size;
+    so.geo = (SR_GLOBAL uint4 *)a.rec_geo; so.raster = (SR_GLOBAL float4 *)a.rec_raster;
+    so.bin_count = (SR_GLOBAL uint32_t *)a.bin_count; so.cap = a.rec_cap; so.status = (SR_GLOBAL DevStatus *)a.status;
+    so.T = a.tile_size;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv, a.tri_base, so);
@@ -2349,9 +2398,12 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(a.rec_cap, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
                        a.bin_end, a.list, a.pair_cap, a.status, a.tile_size);
     TriSource src;
-    src.views = a.views; src.draws = a.draws; src.xp = a.xv; src.xa = a.xv + a.total_xv;
-    src.xt = a.any_tangents ? a.xt : nullptr;
-    src.clip_buf = a.clip_buf;
+    src.views = (const SR_GLOBAL DevView *)a.views;
+    src.draws = (const SR_GLOBAL DevDraw *)a.draws;
+    src.xp = (const SR_GLOBAL float4 *)a.xv;
+    src.xa = (const SR_GLOBAL float4 *)(a.xv + a.total_xv);
+    src.xt = (const SR_GLOBAL float4 *)(a.any_tangents ? a.xt : nullptr);
+    src.clip_buf = (const SR_GLOBAL float *)a.clip_buf;
     if (a.has_translucent) {
         uint32_t g = grid_for(a.rec_cap, 256, GRID_CAP);
         hipLaunchKernelGGL(k_tr_bins, dim3(g), dim3(256), 0, st, a.views, a.rec_geo, a.rec_cap, a.tr_count, nullptr,
@@ -2365,14 +2417,17 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     }
     if (a.events[2]) hipEventRecord(a.events[2], st);
     RasterArgs P;
-    P.views = a.views; P.num_views = a.num_views; P.total_items = a.total_items;
-    P.bin_start = a.bin_start; P.bin_end = a.bin_end; P.list = a.list; P.pair_cap = a.pair_cap;
-    P.geo = a.rec_geo; P.raster = a.rec_raster;
+    P.views = (const SR_GLOBAL DevView *)a.views; P.num_views = a.num_views; P.total_items = a.total_items;
+    P.bin_start = (const SR_GLOBAL uint32_t *)a.bin_start; P.bin_end = (const SR_GLOBAL uint32_t *)a.bin_end;
+    P.list = (const SR_GLOBAL uint32_t *)a.list; P.pair_cap = a.pair_cap;
+    P.geo = (const SR_GLOBAL uint4 *)a.rec_geo; P.raster = (const SR_GLOBAL float4 *)a.rec_raster;
     P.src = src;
-    P.tri_base = a.tri_base; P.out_ids = a.out_ids; P.out_depth = a.out_depth; P.out_color = a.out_color;
-    P.color_plane = a.color_plane; P.out_rgba = a.out_rgba; P.out_lum = a.out_lum; P.status = a.status;
-    P.center_ids = a.center_ids;
-    P.center_color = a.has_translucent ? a.center_color : nullptr;
+    P.tri_base = (const SR_GLOBAL uint32_t *)a.tri_base; P.out_ids = (SR_GLOBAL uint32_t *)a.out_ids;
+    P.out_depth = (SR_GLOBAL float *)a.out_depth; P.out_color = (SR_GLOBAL float *)a.out_color;
+    P.color_plane = a.color_plane; P.out_rgba = (SR_GLOBAL uint32_t *)a.out_rgba;
+    P.out_lum = (SR_GLOBAL float *)a.out_lum; P.status = (SR_GLOBAL DevStatus *)a.status;
+    P.center_ids = (SR_GLOBAL uint32_t *)a.center_ids;
+    P.center_color = (SR_GLOBAL float4 *)(a.has_translucent ? a.center_color : nullptr);
     P.persistent = a.persistent_blocks > 0;
     uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : a.total_items;
     switch (a.tile_size) {
@@ -2388,7 +2443,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     }
     if (a.has_translucent) {
         TrArgs Q;
-        Q.start = a.tr_start; Q.end = a.tr_end; Q.list = a.tr_list; Q.cap = a.tr_cap;
+        Q.start = (const SR_GLOBAL uint32_t *)a.tr_start; Q.end = (const SR_GLOBAL uint32_t *)a.tr_end;
+        Q.list = (const SR_GLOBAL uint32_t *)a.tr_list; Q.cap = a.tr_cap;
         hipLaunchKernelGGL(k_translucent, dim3(a.tr_slots * 16u), dim3(256), 0, st, P, Q);
     }
     if (a.events[3]) hipEventRecord(a.events[3], st);
