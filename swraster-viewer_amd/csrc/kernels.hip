@@ -256,6 +256,22 @@ __device__ __forceinline__ void wave_lds_sync() {
 // frustum make one, rejected ones none; those that cross a plane are queued
 // for k_clip, which writes their count.
 // ---------------------------------------------------------------------------
+// Frustum classification of input triangle t of draw D (-1 = inside, or no
+// clipping needed).
+__device__ __forceinline__ int classify_input_triangle(const DevDraw &D, const DevView *views, uint32_t t,
+                                                       const float4 *xp) {
+    if (!(D.flags & SR_DRAW_CLIP)) return -1;
+    const DevScene &S = *views[D.view].scene;
+    uint32_t lt = t - D.in_tri_base;
+    float c[3][4];
+    for (int k = 0; k < 3; k++) {
+        uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
+        float4 p = xp[D.xv_base + vi];
+        c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
+    }
+    return clip_classify(c);
+}
+
 __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const DevDraw *draws, int ndraws,
                                                     uint32_t total, const float4 *xp, uint32_t *counts,
                                                     uint32_t *queue, uint32_t qcap, DevStatus *status) {
@@ -266,19 +282,9 @@ __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const 
         bool q = false;
         if (t < total) {
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
-            const DevDraw &D = draws[d];
-            int cls = -1;
-            if (D.flags & SR_DRAW_CLIP) {
-                const DevScene &S = *views[D.view].scene;
-                uint32_t lt = t - D.in_tri_base;
-                float c[3][4];
-                for (int k = 0; k < 3; k++) {
-                    uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-                    float4 p = xp[D.xv_base + vi];
-                    c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
-                }
-                cls = clip_classify(c);
-            }
+            int du = wave_uniform(d);
+            int cls = du >= 0 ? classify_input_triangle(draws[du], views, t, xp)
+                              : classify_input_triangle(draws[d], views, t, xp);
             counts[t] = cls == -1 ? 1u : 0u;
             q = cls >= 0;
         }
@@ -699,6 +705,20 @@ struct TriSource {
     const SR_GLOBAL float *clip_buf;    // clipped polygons
 };
 
+// Transformed vertices of input triangle src0 of draw D.
+__device__ __forceinline__ void load_input_tri(const TriSource &T, const DevDraw &D, uint32_t src0, float4 q[3][4]) {
+    const DevScene &S = *T.views[D.view].scene;
+    uint32_t lt = src0 - D.in_tri_base;
+    bool tan = (D.flags & SR_DRAW_TANGENTS) && T.xt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        size_t i = (size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k];
+        q[k][0] = T.xp[i]; q[k][1] = T.xa[2 * i]; q[k][2] = T.xa[2 * i + 1];
+        q[k][3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (tan) q[k][3] = T.xt[i];
+    }
+}
+
 // The three vertices of the triangle whose raster record is R: the indexed
 // transformed vertices of an unclipped input triangle (src0 = input triangle,
 // src1 = draw), or vertices 0, f+1, f+2 of a clipped polygon.
@@ -714,17 +734,9 @@ __device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R,
             q[j][0] = p[0]; q[j][1] = p[1]; q[j][2] = p[2]; q[j][3] = p[3];
         }
     } else {
-        const DevDraw &D = T.draws[src1];
-        const DevScene &S = *T.views[D.view].scene;
-        uint32_t lt = src0 - D.in_tri_base;
-        bool tan = (D.flags & SR_DRAW_TANGENTS) && T.xt;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            size_t i = (size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k];
-            q[k][0] = T.xp[i]; q[k][1] = T.xa[2 * i]; q[k][2] = T.xa[2 * i + 1];
-            q[k][3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (tan) q[k][3] = T.xt[i];
-        }
+        int du = wave_uniform((int)src1);
+        if (du >= 0) load_input_tri(T, T.draws[du], src0, q);
+        else load_input_tri(T, T.draws[src1], src0, q);
     }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
@@ -887,6 +899,30 @@ __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view,
     }
 }
 
+// Setup of input triangle t of draw D (index d), unless it crosses a frustum
+// plane (k_clip_emit sets those up).
+__device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, const DevView *views, uint32_t t,
+                                                     const float4 *xp, uint32_t id, const SetupOut &O) {
+    const DevView &V = views[D.view];
+    const DevScene &S = *V.scene;
+    uint32_t lt = t - D.in_tri_base;
+    CV v[3];
+    for (int k = 0; k < 3; k++) {
+        uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
+        float4 q = xp[D.xv_base + vi];    // setup reads clip positions only
+        v[k].cx = q.x; v[k].cy = q.y; v[k].cz = q.z; v[k].cw = q.w;
+    }
+    int cls = -1;
+    if (D.flags & SR_DRAW_CLIP) {
+        float c[3][4];
+        for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
+        cls = clip_classify(c);
+    }
+    if (cls != -1) return Rect{0, 0, 0, 0};
+    return emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, S.materials[D.material].flags,
+                         !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
+}
+
 __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
                                                const float4 *xp, const uint32_t *tri_base,
                                                SetupOut O) {
@@ -897,29 +933,15 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
         int view = 0;
         if (t < total) {
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
-            const DevDraw &D = draws[d];
-            view = D.view;
-            const DevView &V = views[D.view];
-            const DevScene &S = *V.scene;
-            uint32_t lt = t - D.in_tri_base;
-            CV v[3];
-            for (int k = 0; k < 3; k++) {
-                uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-                float4 q = xp[D.xv_base + vi];    // setup reads clip positions only
-                v[k].cx = q.x; v[k].cy = q.y; v[k].cz = q.z; v[k].cw = q.w;
-            }
-            uint32_t matflags = S.materials[D.material].flags;
             uint32_t id = tri_base[t];
-            int cls = -1;
-            if (D.flags & SR_DRAW_CLIP) {
-                float c[3][4];
-                for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
-                cls = clip_classify(c);
+            int du = wave_uniform(d);
+            if (du >= 0) {
+                view = draws[du].view;
+                r = setup_input_triangle(draws[du], du, views, t, xp, id, O);
+            } else {
+                view = draws[d].view;
+                r = setup_input_triangle(draws[d], d, views, t, xp, id, O);
             }
-            // triangles that cross a plane are set up by k_clip_emit
-            if (cls == -1)
-                r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, matflags,
-                                  !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
         }
         bin_rect_count(views, view, r, O.bin_count);
     }
