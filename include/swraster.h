@@ -297,6 +297,27 @@ SR_API int sr_texture_mip_chain(const sr_texture_desc *tex, uint32_t *texels_out
                                 uint32_t offsets[32], uint32_t widths[32],
                                 uint32_t heights[32]);
 
+/* ---- glTF scene files (host, C++) ---------------------------------------- */
+/* load_gltf + Scene::from_gltf (src/main.rs:81-97, src/scene.rs:145-815): a
+ * .gltf (external or base64 data-URI buffers) or .glb file, every mesh,
+ * node (world transforms from the hierarchy) and material, PNG textures by
+ * URI relative to the file (the reference's TextureCache,
+ * src/texture.rs:866-1015).  Missing normals are computed as
+ * compute_smooth_normals (src/scene.rs:515-546), missing texture
+ * coordinates are zeros.  The result is host memory owned by the handle. */
+typedef struct sr_gltf sr_gltf;
+SR_API int sr_gltf_load(const char *path, sr_gltf **out);
+/* Views of the loaded arrays as a scene description (valid until
+ * sr_gltf_destroy); environment and voxels are NULL, the light is the
+ * reference's default sun. */
+SR_API int sr_gltf_scene_desc(sr_gltf *gltf, sr_scene_desc *desc);
+SR_API int sr_gltf_destroy(sr_gltf *gltf);
+/* PNG decoding as used for glTF textures (non-interlaced; every colour type
+ * and bit depth; tRNS): packed r<<24|g<<16|b<<8|a texels.  With texels NULL
+ * only the size is returned. */
+SR_API int sr_png_decode(const uint8_t *data, size_t size, uint32_t *width, uint32_t *height,
+                         uint32_t *texels, size_t capacity);
+
 /* ---- camera ------------------------------------------------------------- */
 /* RenderCamera::new (src/rendercamera.rs:28-63): yaw/pitch from the look-at
  * direction, near = 0.01 * far.  Matrices are computed by update_matrices. */

@@ -168,6 +168,11 @@ _SIGS = {
                                        C.POINTER(C.c_float)]),
     "sr_texture_mip_chain": (C.c_int, [C.POINTER(sr_texture_desc), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                        C.POINTER(C.c_uint32), C.c_uint32 * 32, C.c_uint32 * 32, C.c_uint32 * 32]),
+    "sr_gltf_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "sr_gltf_scene_desc": (C.c_int, [_P, C.POINTER(sr_scene_desc)]),
+    "sr_gltf_destroy": (C.c_int, [_P]),
+    "sr_png_decode": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint32), C.c_size_t]),
     "sr_camera_new": (C.c_int, [C.c_float * 3, C.c_float * 3, C.c_float, C.c_float, C.c_float, C.c_float,
                                 C.POINTER(sr_camera)]),
     "sr_camera_from_pose": (C.c_int, [C.c_float * 3, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -230,6 +235,18 @@ def check(code):
     if code != SR_OK:
         raise SwrasterError(code, lib().sr_last_error().decode(errors="replace"))
     return code
+
+
+def png_decode(data):
+    """Decode PNG bytes with the library's decoder (the one the glTF loader
+    uses): (height, width) array of packed r<<24|g<<16|b<<8|a texels."""
+    buf = bytes(data)
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().sr_png_decode(buf, len(buf), C.byref(w), C.byref(h), None, 0))
+    out = np.empty(w.value * h.value, np.uint32)
+    check(lib().sr_png_decode(buf, len(buf), C.byref(w), C.byref(h), out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              out.size))
+    return out.reshape(h.value, w.value)
 
 
 def device_count():
@@ -473,6 +490,63 @@ class SceneData:
         keep += [prims, meshes, nodes, mats, texs]
         self._keep = keep
         self.desc = d
+
+    _TEX_SLOTS = (("base_color", "base_color"), ("metallic_roughness", "metallic_roughness"),
+                  ("normal", "normal"), ("emissive", "emissive"), ("occlusion", "occlusion"),
+                  ("transmission_texture", "transmission_texture"))
+
+    @classmethod
+    def from_gltf(cls, path, name=None):
+        """Load a .gltf or .glb file with the library's C++ loader
+        (sr_gltf_load: load_gltf + Scene::from_gltf, src/main.rs:81-97,
+        src/scene.rs:145-815) and copy its arrays into a SceneData."""
+        h = _P()
+        check(lib().sr_gltf_load(os.fsencode(str(path)), C.byref(h)))
+        try:
+            d = sr_scene_desc()
+            check(lib().sr_gltf_scene_desc(h, C.byref(d)))
+
+            def arr(ptr, n, width):
+                if not ptr or n == 0:
+                    return None
+                return np.ctypeslib.as_array(ptr, (n * width,)).copy().reshape(n, width) if width > 1 else \
+                    np.ctypeslib.as_array(ptr, (n,)).copy()
+
+            prims = []
+            for i in range(d.num_primitives):
+                p = d.primitives[i]
+                nv = p.num_vertices
+                pos = arr(p.positions, nv, 3)
+                prims.append({"positions": pos if pos is not None else np.zeros((0, 3), np.float32),
+                              "normals": arr(p.normals, nv, 3) if nv else np.zeros((0, 3), np.float32),
+                              "texcoords": arr(p.texcoords, nv, 2), "tangents": arr(p.tangents, nv, 4),
+                              "indices": arr(p.indices, p.num_indices, 1) if p.num_indices else
+                              np.zeros(0, np.uint32), "material": int(p.material)})
+            meshes = [(d.meshes[i].first_primitive, d.meshes[i].num_primitives) for i in range(d.num_meshes)]
+            nodes = [(np.array(d.nodes[i].transform, np.float32).reshape(4, 4).T.copy(), int(d.nodes[i].mesh))
+                     for i in range(d.num_nodes)]
+            mats = []
+            for i in range(d.num_materials):
+                m = d.materials[i]
+                md = {"base_color": tuple(float(x) for x in m.base_color_factor), "metallic": m.metallic_factor,
+                      "roughness": m.roughness_factor, "emissive": tuple(float(x) for x in m.emissive_factor),
+                      "occlusion_strength": m.occlusion_strength, "alpha_cutoff": m.alpha_cutoff,
+                      "transmission": m.transmission, "alpha_mode": int(m.alpha_mode),
+                      "is_translucent": int(m.is_translucent)}
+                for key, field in cls._TEX_SLOTS:
+                    r = getattr(m, field)
+                    if r.texture >= 0:
+                        md[key + "_tex"] = (int(r.texture), int(r.wrap_s), int(r.wrap_t))
+                mats.append(md)
+            texs = []
+            for i in range(d.num_textures):
+                t = d.textures[i]
+                texs.append((int(t.width), int(t.height), int(t.type),
+                             np.ctypeslib.as_array(t.texels, (t.width * t.height,)).copy()))
+        finally:
+            lib().sr_gltf_destroy(h)
+        base = os.path.splitext(os.path.basename(str(path)))[0]
+        return cls(prims, meshes, nodes, mats, texs, name=name or base)
 
     def bounds(self):
         mn, mx, ce = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()

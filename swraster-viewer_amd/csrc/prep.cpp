@@ -643,6 +643,22 @@ static inline M from_quat(Q q) {
     return {{1.0f - (yy + zz), xy + wz, xz - wy, 0, xy - wz, 1.0f - (xx + zz), yz + wx, 0, xz + wy, yz - wx,
              1.0f - (xx + yy), 0, 0, 0, 0, 1}};
 }
+}  // namespace
+
+void mat4_mul(const float a[16], const float b[16], float out[16]) {
+    M A, B;
+    memcpy(A.m, a, 64);
+    memcpy(B.m, b, 64);
+    M r = mmul(A, B);
+    memcpy(out, r.m, 64);
+}
+
+void mat4_from_quat(const float q[4], float out[16]) {
+    M r = from_quat(Q{q[0], q[1], q[2], q[3]});
+    memcpy(out, r.m, 64);
+}
+
+namespace {
 static M inverse(const M &A) {
     const float *m = A.m;
     float m00 = m[0], m01 = m[1], m02 = m[2], m03 = m[3], m10 = m[4], m11 = m[5], m12 = m[6], m13 = m[7];

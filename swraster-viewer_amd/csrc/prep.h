@@ -20,5 +20,9 @@ int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<
 void camera_new(const float pos[3], const float look[3], float fov, float w, float h, float far_plane, sr_camera *c);
 void camera_update(sr_camera *c);
 void compute_tangents(const sr_primitive_desc &p, std::vector<float> &out);
+// Column-major 4x4 product a * b and rotation matrix of quaternion (x, y, z, w)
+// (glam Mat4::mul_mat4, Mat4::from_quat).
+void mat4_mul(const float a[16], const float b[16], float out[16]);
+void mat4_from_quat(const float q[4], float out[16]);
 
 }  // namespace sr

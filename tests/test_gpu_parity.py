@@ -372,6 +372,22 @@ def test_translucent_materials(sr, orc, scenes, env, size):
     assert changed > 20000
 
 
+def test_gltf_scene_renders_identically(sr, scenes, env, tmp_path):
+    """C1 written as glTF and read back by the C++ loader (sr_gltf_load)
+    renders bit-identically to the same scene given as arrays."""
+    scene, spec = scenes.make_c1(width=640, height=360)
+    loaded = sr.SceneData.from_gltf(scenes.write_gltf(scene, str(tmp_path / "c1.gltf")))
+    scene.prepare(env, cache_key=scene.name)
+    loaded.prepare(env, cache_key=scene.name)
+    cam = scenes.make_camera(spec)
+    a = render_gpu(sr, sr.Scene(scene), cam, 640, 360)
+    b = render_gpu(sr, sr.Scene(loaded), cam, 640, 360)
+    assert np.array_equal(a["ids"], b["ids"])
+    assert np.array_equal(bits(a["depth"]), bits(b["depth"]))
+    assert np.array_equal(bits(a["rgb"]), bits(b["rgb"]))
+    assert np.array_equal(a["px"], b["px"])
+
+
 def test_gpu_matches_committed_c1_frame(sr, scenes, env):
     """The GPU reproduces the committed oracle frame (tests/golden/) without
     running the oracle."""
