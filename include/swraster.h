@@ -285,6 +285,13 @@ SR_API int sr_environment_free(sr_environment *env);
  * (src/voxelgrid.rs:371-419).  scene->voxels is ignored. */
 SR_API int sr_prepare_voxel_grid(const sr_scene_desc *scene, int32_t grid_size,
                                  int32_t threads, sr_voxel_grid *out);
+/* The same bake on a HIP device (csrc/bake.hip), bit-identical to
+ * sr_prepare_voxel_grid: occupancy, dilation, sun rays against the
+ * triangles binned on the plane normal to the light (all rays are
+ * parallel), blur.  Rays meeting more translucent surfaces than the device
+ * keeps make it bake on the host instead. */
+SR_API int sr_prepare_voxel_grid_device(const sr_scene_desc *scene, int32_t grid_size,
+                                        int32_t device, sr_voxel_grid *out);
 SR_API int sr_voxel_grid_free(sr_voxel_grid *grid);
 /* Scene bounds over all node-transformed positions (src/scene.rs:333-351). */
 SR_API int sr_scene_desc_bounds(const sr_scene_desc *scene, float min[3], float max[3],

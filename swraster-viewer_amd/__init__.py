@@ -164,6 +164,8 @@ _SIGS = {
     "sr_environment_free": (C.c_int, [C.POINTER(sr_environment)]),
     "sr_prepare_voxel_grid": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32, C.POINTER(sr_voxel_grid)]),
     "sr_voxel_grid_free": (C.c_int, [C.POINTER(sr_voxel_grid)]),
+    "sr_prepare_voxel_grid_device": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32,
+                                               C.POINTER(sr_voxel_grid)]),
     "sr_scene_desc_bounds": (C.c_int, [C.POINTER(sr_scene_desc), C.c_float * 3, C.c_float * 3, C.c_float * 3,
                                        C.POINTER(C.c_float)]),
     "sr_texture_mip_chain": (C.c_int, [C.POINTER(sr_texture_desc), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
@@ -561,8 +563,9 @@ class SceneData:
         self.desc.voxels = C.pointer(voxels.desc)
         return self
 
-    def prepare(self, environment=None, grid=128, threads=0, cache_key=None):
-        """Attach the IBL environment and build (or load) the voxel grid."""
+    def prepare(self, environment=None, grid=128, threads=0, cache_key=None, device=None):
+        """Attach the IBL environment and build (or load) the voxel grid; with
+        a device ordinal the voxel bake runs on that GPU (identical result)."""
         environment = environment or Environment.load(threads=threads)
         path = None
         if cache_key:
@@ -571,7 +574,10 @@ class SceneData:
                 z = np.load(path)
                 return self.attach(environment, VoxelGrid(z["dims"], z["wmin"], z["wmax"], z["light"]))
         g = sr_voxel_grid()
-        check(lib().sr_prepare_voxel_grid(C.byref(self.desc), grid, threads, C.byref(g)))
+        if device is None:
+            check(lib().sr_prepare_voxel_grid(C.byref(self.desc), grid, threads, C.byref(g)))
+        else:
+            check(lib().sr_prepare_voxel_grid_device(C.byref(self.desc), grid, int(device), C.byref(g)))
         n = g.dims[0] * g.dims[1] * g.dims[2]
         light = np.ctypeslib.as_array(g.light_intensity, shape=(n,)).copy()
         dims, wmin, wmax = list(g.dims), list(g.world_min), list(g.world_max)

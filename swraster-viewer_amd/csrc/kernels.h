@@ -61,6 +61,11 @@ struct FrameArgs {
 // first launch error.
 hipError_t launch_frame(const FrameArgs &a, hipStream_t stream);
 
+// Exclusive scan of n u32 (out, and out2 if not null); the total goes to
+// *total (device).  partials holds (n + 4095) / 4096 + 1 words.
+hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
+                       uint32_t *total, hipStream_t stream);
+
 // Exposure + tonemap + quantize + pack of one view's linear colour planes.
 hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
                        hipStream_t stream);

@@ -531,8 +531,8 @@ __global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
-static hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
-                              uint32_t *total, hipStream_t st) {
+hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
+                       uint32_t *total, hipStream_t st) {
     if (n <= SCAN_SINGLE_MAX) {
         hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, in, n, out, out2, total);
         return hipGetLastError();

@@ -459,44 +459,94 @@ static bool ray_tri(F3 o, F3 d, const Tri &t, float tmin, float tmax, float &tou
 }
 }  // namespace
 
-int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<float> &light, float wmin[3],
-                     float wmax[3]) {
-    float center[3], diag;
-    scene_bounds(d, wmin, wmax, center, &diag);
-    F3 lo = f3(wmin[0], wmin[1], wmin[2]), hi = f3(wmax[0], wmax[1], wmax[2]);
-    F3 ws = hi - lo;
-    F3 vs = f3(ws.x / (float)grid, ws.y / (float)grid, ws.z / (float)grid);
-    size_t nvox = (size_t)grid * grid * grid;
-
-    // RayTracer::new: world triangles in node order, skipping degenerate ones
-    Bvh bvh;
+// World-space triangles of every node in node order, primitive order and
+// triangle order, skipping degenerate ones (RayTracer::new,
+// src/raytracer.rs:172-223).  Parallel over chunks; the order is kept.
+void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<WorldTri> &out) {
+    struct Job { int node, prim; uint32_t first, count; };
+    std::vector<Job> jobs;
+    const uint32_t CHUNK = 1u << 16;
     for (int ni = 0; ni < d->num_nodes; ni++) {
         const sr_node_desc &n = d->nodes[ni];
         if (n.mesh < 0) continue;
         const sr_mesh_desc &m = d->meshes[n.mesh];
         for (int k = 0; k < m.num_primitives; k++) {
-            const sr_primitive_desc &p = d->primitives[m.first_primitive + k];
-            for (uint32_t t = 0; t + 2 < p.num_indices; t += 3) {
-                float a[4], b[4], c[4];
-                mat_point(n.transform, p.positions + 3 * p.indices[t], a);
-                mat_point(n.transform, p.positions + 3 * p.indices[t + 1], b);
-                mat_point(n.transform, p.positions + 3 * p.indices[t + 2], c);
-                Tri tr = {f3(a[0], a[1], a[2]), f3(b[0], b[1], b[2]), f3(c[0], c[1], c[2]), p.material};
-                F3 fn = cross(tr.p1 - tr.p0, tr.p2 - tr.p0);
-                if (dot(fn, fn) <= 1.0e-12f) continue;
-                bvh.tris.push_back(tr);
-                const float eps = 1.0e-5f;
-                Box bx = {fmin3(fmin3(tr.p0, tr.p1), tr.p2) - f3(eps, eps, eps),
-                          fmax3(fmax3(tr.p0, tr.p1), tr.p2) + f3(eps, eps, eps)};
-                bvh.tbox.push_back(bx);
-            }
+            int pi = m.first_primitive + k;
+            uint32_t nt = d->primitives[pi].num_indices / 3;
+            for (uint32_t f = 0; f < nt; f += CHUNK) jobs.push_back({ni, pi, f, std::min(CHUNK, nt - f)});
+        }
+    }
+    std::vector<std::vector<WorldTri>> parts(jobs.size());
+    parallel_for(jobs.size(), threads, [&](size_t j) {
+        const Job &jb = jobs[j];
+        const sr_node_desc &n = d->nodes[jb.node];
+        const sr_primitive_desc &p = d->primitives[jb.prim];
+        std::vector<WorldTri> &o = parts[j];
+        o.reserve(jb.count);
+        for (uint32_t t = jb.first; t < jb.first + jb.count; t++) {
+            float a[4], b[4], c[4];
+            mat_point(n.transform, p.positions + 3 * p.indices[3 * t], a);
+            mat_point(n.transform, p.positions + 3 * p.indices[3 * t + 1], b);
+            mat_point(n.transform, p.positions + 3 * p.indices[3 * t + 2], c);
+            F3 p0 = f3(a[0], a[1], a[2]), p1 = f3(b[0], b[1], b[2]), p2 = f3(c[0], c[1], c[2]);
+            F3 fn = cross(p1 - p0, p2 - p0);
+            if (dot(fn, fn) <= 1.0e-12f) continue;
+            o.push_back({{p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z}, p.material});
+        }
+    });
+    out.clear();
+    for (auto &pt : parts) out.insert(out.end(), pt.begin(), pt.end());
+}
+
+// Grid placement and sun ray constants of the voxel bake (src/gi.rs:267-290,
+// src/voxelgrid.rs:154-161).
+void voxel_frame(const sr_scene_desc *d, int grid, VoxelFrame &f) {
+    float center[3], diag;
+    scene_bounds(d, f.wmin, f.wmax, center, &diag);
+    F3 lo = f3(f.wmin[0], f.wmin[1], f.wmin[2]), hi = f3(f.wmax[0], f.wmax[1], f.wmax[2]);
+    F3 ws = hi - lo;
+    F3 vs = f3(ws.x / (float)grid, ws.y / (float)grid, ws.z / (float)grid);
+    F3 ldir = normalize(f3(d->light.direction[0], d->light.direction[1], d->light.direction[2]));
+    F3 cmin = lo + vs * 0.5f;
+    f.grid = grid;
+    f.lo[0] = lo.x; f.lo[1] = lo.y; f.lo[2] = lo.z;
+    f.vs[0] = vs.x; f.vs[1] = vs.y; f.vs[2] = vs.z;
+    f.ldir[0] = ldir.x; f.ldir[1] = ldir.y; f.ldir[2] = ldir.z;
+    f.cmin[0] = cmin.x; f.cmin[1] = cmin.y; f.cmin[2] = cmin.z;
+    f.bias = len(vs) * 3.0f;
+    f.area_ref = std::fmax(std::fmin(vs.x, std::fmin(vs.y, vs.z)), 1.0e-6f);
+    f.area_ref = f.area_ref * f.area_ref;
+}
+
+int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<float> &light, float wmin[3],
+                     float wmax[3]) {
+    VoxelFrame fr;
+    voxel_frame(d, grid, fr);
+    memcpy(wmin, fr.wmin, 12);
+    memcpy(wmax, fr.wmax, 12);
+    F3 lo = f3(fr.lo[0], fr.lo[1], fr.lo[2]);
+    F3 vs = f3(fr.vs[0], fr.vs[1], fr.vs[2]);
+    size_t nvox = (size_t)grid * grid * grid;
+
+    Bvh bvh;
+    {
+        std::vector<WorldTri> wt;
+        collect_world_triangles(d, threads, wt);
+        bvh.tris.reserve(wt.size());
+        bvh.tbox.reserve(wt.size());
+        for (const WorldTri &w : wt) {
+            Tri tr = {f3(w.p[0], w.p[1], w.p[2]), f3(w.p[3], w.p[4], w.p[5]), f3(w.p[6], w.p[7], w.p[8]), w.material};
+            bvh.tris.push_back(tr);
+            const float eps = 1.0e-5f;
+            Box bx = {fmin3(fmin3(tr.p0, tr.p1), tr.p2) - f3(eps, eps, eps),
+                      fmax3(fmax3(tr.p0, tr.p1), tr.p2) + f3(eps, eps, eps)};
+            bvh.tbox.push_back(bx);
         }
     }
 
     // build_active_voxel_mask (src/gi.rs:151-265): occupancy by surface sampling
     std::vector<uint8_t> occupied(nvox, 0);
-    float mve = std::fmax(std::fmin(vs.x, std::fmin(vs.y, vs.z)), 1.0e-6f);
-    float area_ref = mve * mve;
+    float area_ref = fr.area_ref;
     for (const Tri &t : bvh.tris) {
         F3 e1 = t.p1 - t.p0, e2 = t.p2 - t.p0;
         float area = 0.5f * len(cross(e1, e2));
@@ -543,15 +593,16 @@ int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<
     }
 
     // compute_sun_visibility (src/gi.rs:267-314)
-    F3 ldir = normalize(f3(d->light.direction[0], d->light.direction[1], d->light.direction[2]));
-    F3 cmin = lo + vs * 0.5f;
-    float bias = len(vs) * 3.0f;
+    F3 ldir = f3(fr.ldir[0], fr.ldir[1], fr.ldir[2]);
+    F3 cmin = f3(fr.cmin[0], fr.cmin[1], fr.cmin[2]);
+    float bias = fr.bias;
     std::vector<float> vis(nvox, 1.0f);
     F3 inv = f3(1.0f / ldir.x, 1.0f / ldir.y, 1.0f / ldir.z);
     parallel_for((size_t)grid * grid, threads, [&](size_t zy) {
         int z = (int)(zy / grid), y = (int)(zy % grid);
         std::vector<int> stack;
-        std::vector<std::pair<float, int>> hits;
+        struct Hit { float t; int tri, material; };
+        std::vector<Hit> hits;
         for (int x = 0; x < grid; x++) {
             size_t idx = ((size_t)z * grid + y) * grid + x;
             if (!active[idx]) continue;
@@ -570,7 +621,7 @@ int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<
                         for (int k = nd.first; k < nd.first + nd.count; k++) {
                             const Tri &t = bvh.tris[bvh.order[k]];
                             float tt;
-                            if (ray_tri(o, ldir, t, 1.0e-4f, INFINITY, tt)) hits.push_back({tt, t.material});
+                            if (ray_tri(o, ldir, t, 1.0e-4f, INFINITY, tt)) hits.push_back({tt, bvh.order[k], t.material});
                         }
                     } else {
                         stack.push_back(nd.left);
@@ -578,11 +629,13 @@ int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<
                     }
                 }
             }
-            std::stable_sort(hits.begin(), hits.end(),
-                             [](const std::pair<float, int> &a, const std::pair<float, int> &b) { return a.first < b.first; });
+            // by distance; equal distances by triangle order, so the result
+            // does not depend on the traversal order
+            std::sort(hits.begin(), hits.end(),
+                      [](const Hit &a, const Hit &b) { return a.t < b.t || (a.t == b.t && a.tri < b.tri); });
             float tr = 1.0f;
             for (auto &h : hits) {
-                int m = (h.second >= 0 && h.second < (int)translucent.size()) ? h.second : 0;
+                int m = (h.material >= 0 && h.material < (int)translucent.size()) ? h.material : 0;
                 if (!translucent[m]) { tr = 0.0f; break; }
                 tr *= transmission[m];
                 if (tr <= 0.0001f) { tr = 0.0f; break; }

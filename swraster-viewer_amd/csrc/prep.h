@@ -17,6 +17,27 @@ void compute_specular_cubemap(const uint32_t *faces, uint32_t size, uint32_t sam
 void scene_bounds(const sr_scene_desc *d, float mn[3], float mx[3], float center[3], float *diag);
 int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<float> &light, float wmin[3],
                      float wmax[3]);
+
+// Shared by the host bake and the device bake (bake.hip).
+struct WorldTri {
+    float p[9];          // p0, p1, p2 in world space
+    int32_t material;
+};
+struct VoxelFrame {
+    int grid;
+    float wmin[3], wmax[3];
+    float lo[3], vs[3];  // grid origin and voxel size
+    float cmin[3];       // centre of voxel (0, 0, 0)
+    float ldir[3];       // normalized sun direction
+    float bias;          // ray origin offset along ldir
+    float area_ref;      // occupancy sampling reference area
+};
+void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<WorldTri> &out);
+void voxel_frame(const sr_scene_desc *d, int grid, VoxelFrame &f);
+// The bake on a HIP device (bake.hip).  fallback: a ray met more translucent
+// surfaces than the device keeps per ray; the caller bakes on the host.
+int bake_voxels_device(const sr_scene_desc *d, int grid, int device, std::vector<float> &light, VoxelFrame &fr,
+                       bool &fallback);
 void camera_new(const float pos[3], const float look[3], float fov, float w, float h, float far_plane, sr_camera *c);
 void camera_update(sr_camera *c);
 void compute_tangents(const sr_primitive_desc &p, std::vector<float> &out);

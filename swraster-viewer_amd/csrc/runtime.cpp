@@ -494,6 +494,36 @@ SR_API int sr_prepare_voxel_grid(const sr_scene_desc *scene, int32_t grid, int32
     return SR_OK;
 }
 
+SR_API int sr_prepare_voxel_grid_device(const sr_scene_desc *scene, int32_t grid, int32_t device,
+                                        sr_voxel_grid *out) {
+    if (!scene || !out || grid <= 0 || grid > 1024) return fail(SR_ERR_INVALID_ARGUMENT, "bad voxel grid arguments");
+    int32_t ndev = 0;
+    sr_device_count(&ndev);
+    if (device < 0 || device >= ndev) return fail(SR_ERR_INVALID_ARGUMENT, "no HIP device %d (have %d)", device, ndev);
+    memset(out, 0, sizeof(*out));
+    std::vector<float> light;
+    VoxelFrame fr;
+    bool fallback = false;
+    int r = bake_voxels_device(scene, grid, device, light, fr, fallback);
+    if (r) return r;
+    if (fallback) {
+        r = build_voxel_grid(scene, grid, 0, light, out->world_min, out->world_max);
+        if (r) return r;
+    } else {
+        memcpy(out->world_min, fr.wmin, 12);
+        memcpy(out->world_max, fr.wmax, 12);
+    }
+    float *lp = (float *)malloc(light.size() * 4);
+    if (!lp) return fail(SR_ERR_OUT_OF_MEMORY, "voxel grid");
+    memcpy(lp, light.data(), light.size() * 4);
+    out->dims[0] = out->dims[1] = out->dims[2] = grid;
+    out->light_intensity = lp;
+    out->sh_scale = 0.25f;
+    out->sky_visibility = 1.0f;
+    out->sh4 = nullptr;
+    return SR_OK;
+}
+
 SR_API int sr_voxel_grid_free(sr_voxel_grid *g) {
     if (!g) return SR_OK;
     free((void *)g->light_intensity);
