@@ -278,6 +278,12 @@ SR_API int sr_device_count(int32_t *count);
  * sr_environment_free. */
 SR_API int sr_prepare_environment(const uint32_t *cubemap_faces, uint32_t face_size,
                                   int32_t threads, sr_environment *out);
+/* The same inputs computed on a HIP device (csrc/envmap.hip): the BRDF LUT
+ * bit-identical, the specular cubemap within 1 LSB per channel and the SH
+ * within float rounding of the host version (the sRGB decode's power is
+ * correctly rounded here; the host's libm powf is not always). */
+SR_API int sr_prepare_environment_device(const uint32_t *cubemap_faces, uint32_t face_size,
+                                         int32_t device, sr_environment *out);
 SR_API int sr_environment_free(sr_environment *env);
 /* Builds the 128^3 voxel grid over the scene bounds: occupancy and active
  * mask (src/gi.rs:151-265), sun visibility by ray casting

@@ -162,6 +162,8 @@ _SIGS = {
     "sr_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
     "sr_prepare_environment": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32, C.c_int32, C.POINTER(sr_environment)]),
     "sr_environment_free": (C.c_int, [C.POINTER(sr_environment)]),
+    "sr_prepare_environment_device": (C.c_int, [C.POINTER(C.c_uint32), C.c_uint32, C.c_int32,
+                                                C.POINTER(sr_environment)]),
     "sr_prepare_voxel_grid": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32, C.POINTER(sr_voxel_grid)]),
     "sr_voxel_grid_free": (C.c_int, [C.POINTER(sr_voxel_grid)]),
     "sr_prepare_voxel_grid_device": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32,
@@ -345,15 +347,20 @@ class Environment:
                 self.desc.irradiance_sh[i][k] = float(self.sh[i, k])
 
     @classmethod
-    def load(cls, threads=0, cache=True):
-        path = os.path.join(CACHE_DIR, "environment_v1.npz")
+    def load(cls, threads=0, cache=True, device=None):
+        """The skybox's IBL inputs, computed on the host (threads) or on a
+        HIP device (device ordinal), cached on disk per variant."""
+        path = os.path.join(CACHE_DIR, "environment_v1.npz" if device is None else "environment_device_v1.npz")
         faces = cubemap_faces()
         if cache and os.path.exists(path):
             z = np.load(path)
             if np.array_equal(z["cubemap"], faces):
                 return cls(z["cubemap"], z["specular"], int(z["mips"]), z["lut"], z["sh"])
         env = sr_environment()
-        check(lib().sr_prepare_environment(_uptr(faces), faces.shape[-1], threads, C.byref(env)))
+        if device is None:
+            check(lib().sr_prepare_environment(_uptr(faces), faces.shape[-1], threads, C.byref(env)))
+        else:
+            check(lib().sr_prepare_environment_device(_uptr(faces), faces.shape[-1], int(device), C.byref(env)))
         size = env.cubemap_size
         spec = np.ctypeslib.as_array(env.specular, shape=(env.specular_mips * 6 * size * size,)).copy()
         lut = np.ctypeslib.as_array(env.brdf_lut, shape=(env.brdf_lut_size ** 2,)).copy()

@@ -38,6 +38,9 @@ void voxel_frame(const sr_scene_desc *d, int grid, VoxelFrame &f);
 // surfaces than the device keeps per ray; the caller bakes on the host.
 int bake_voxels_device(const sr_scene_desc *d, int grid, int device, std::vector<float> &light, VoxelFrame &fr,
                        bool &fallback);
+// IBL inputs on a HIP device (envmap.hip).
+int prepare_environment_device(const uint32_t *faces, uint32_t size, int device, std::vector<uint32_t> &spec,
+                               uint32_t &mips, std::vector<uint32_t> &lut, uint32_t lut_size, float sh[4][3]);
 void camera_new(const float pos[3], const float look[3], float fov, float w, float h, float far_plane, sr_camera *c);
 void camera_update(sr_camera *c);
 void compute_tangents(const sr_primitive_desc &p, std::vector<float> &out);

@@ -469,6 +469,40 @@ SR_API int sr_prepare_environment(const uint32_t *faces, uint32_t size, int32_t 
     return SR_OK;
 }
 
+SR_API int sr_prepare_environment_device(const uint32_t *faces, uint32_t size, int32_t device, sr_environment *out) {
+    if (!faces || !out || size == 0 || (size & (size - 1)))
+        return fail(SR_ERR_INVALID_ARGUMENT, "cubemap faces must be a power-of-two size");
+    int32_t ndev = 0;
+    sr_device_count(&ndev);
+    if (device < 0 || device >= ndev) return fail(SR_ERR_INVALID_ARGUMENT, "no HIP device %d (have %d)", device, ndev);
+    memset(out, 0, sizeof(*out));
+    std::vector<uint32_t> spec, lut;
+    uint32_t mips = 0;
+    int r = prepare_environment_device(faces, size, device, spec, mips, lut, 128, out->irradiance_sh);
+    if (r) return r;
+    size_t face = (size_t)size * size;
+    uint32_t *cube = (uint32_t *)malloc(face * 6 * 4);
+    uint32_t *sp = (uint32_t *)malloc(spec.size() * 4);
+    uint32_t *lp = (uint32_t *)malloc(lut.size() * 4);
+    if (!cube || !sp || !lp) {
+        free(cube);
+        free(sp);
+        free(lp);
+        return fail(SR_ERR_OUT_OF_MEMORY, "environment");
+    }
+    memcpy(cube, faces, face * 6 * 4);
+    memcpy(sp, spec.data(), spec.size() * 4);
+    memcpy(lp, lut.data(), lut.size() * 4);
+    out->cubemap_size = size;
+    out->cubemap = cube;
+    out->specular_size = size;
+    out->specular_mips = mips;
+    out->specular = sp;
+    out->brdf_lut_size = 128;
+    out->brdf_lut = lp;
+    return SR_OK;
+}
+
 SR_API int sr_environment_free(sr_environment *env) {
     if (!env) return SR_OK;
     free((void *)env->cubemap);

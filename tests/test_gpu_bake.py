@@ -53,3 +53,40 @@ def test_device_bake_translucent_shadows(sr, scenes):
     assert np.array_equal(dev.view(np.uint32), host.view(np.uint32))
     # partial shadow values exist (neither 0 nor 1 before the blur)
     assert np.any((host > 0.01) & (host < 0.99))
+
+
+def _environment(sr, faces, device):
+    env = sr.sr_environment()
+    t0 = time.perf_counter()
+    if device is None:
+        sr.check(sr.lib().sr_prepare_environment(sr._uptr(faces), faces.shape[-1], 0, sr.C.byref(env)))
+    else:
+        sr.check(sr.lib().sr_prepare_environment_device(sr._uptr(faces), faces.shape[-1], device, sr.C.byref(env)))
+    dt = time.perf_counter() - t0
+    size = env.cubemap_size
+    spec = np.ctypeslib.as_array(env.specular, shape=(env.specular_mips * 6 * size * size,)).copy()
+    lut = np.ctypeslib.as_array(env.brdf_lut, shape=(env.brdf_lut_size ** 2,)).copy()
+    sh = np.array([[env.irradiance_sh[i][k] for k in range(3)] for i in range(4)], np.float32)
+    sr.lib().sr_environment_free(sr.C.byref(env))
+    return spec, lut, sh, dt
+
+
+def test_device_environment_matches_host(sr):
+    """GGX-prefiltered specular cubemap, BRDF LUT and irradiance SH of the
+    skybox (src/texture.rs:199-420) on the device against the host: the LUT
+    bit-identical, the specular cubemap within 1 LSB per channel, the SH
+    within float rounding."""
+    faces = sr.cubemap_faces()
+    _environment(sr, faces[:, :16, :16].copy(), 0)               # first-use set-up outside the timing
+    spec_d, lut_d, sh_d, t_dev = _environment(sr, faces, 0)
+    spec_h, lut_h, sh_h, t_host = _environment(sr, faces, None)
+    assert np.array_equal(lut_d, lut_h)
+    a, b = sr.unpack_rgba(spec_d).astype(int), sr.unpack_rgba(spec_h).astype(int)
+    diff = np.abs(a - b).max(-1)
+    rel = np.abs(sh_d.astype(np.float64) - sh_h) / np.maximum(np.abs(sh_h), 1e-12)
+    print(f"[environment] device {t_dev * 1e3:.0f} ms, host {t_host * 1e3:.0f} ms ({t_host / t_dev:.0f}x); "
+          f"specular texels differing {(diff > 0).sum()} of {diff.size} (max {diff.max()} LSB); "
+          f"SH max rel diff {rel.max():.2e}")
+    assert diff.max() <= 1
+    assert (diff > 0).mean() < 1e-3
+    assert rel.max() < 1e-5
