@@ -179,3 +179,28 @@ def test_power_of_two_reciprocal_division():
     for k in (0, 1, 5, 11, 12, 20):
         d = np.float32(2.0 ** k)
         assert np.array_equal((x / d).view(np.uint32), (x * np.float32(2.0 ** -k)).view(np.uint32))
+
+
+def test_srgb_decode_power_is_correctly_rounded():
+    """The device IBL precompute evaluates the sRGB decode's x^2.4 in double
+    and rounds once.  Over every float input the decode can take
+    (x in ((0.04045 + 0.055) / 1.055, 1]) that equals x^2.4 computed in
+    80-bit extended precision and rounded to float, and no exact value lies
+    within 2.5e-16 (relative) of a float rounding boundary, so any double
+    power accurate to about one ulp gives the same float."""
+    lo = np.float32((np.float32(0.04045) + np.float32(0.055)) / np.float32(1.055))
+    b0, b1 = int(lo.view(np.uint32)) - 16, int(np.float32(1.0).view(np.uint32))
+    e24 = np.longdouble(np.float32(2.4))
+    assert np.finfo(np.longdouble).nmant >= 63, "needs x87 extended precision"
+    mismatches, closest = 0, 1.0
+    for s in range(b0, b1 + 1, 1 << 22):
+        x = np.arange(s, min(s + (1 << 22), b1 + 1), dtype=np.uint32).view(np.float32)
+        d = np.power(x.astype(np.float64), np.float64(np.float32(2.4))).astype(np.float32)
+        e = np.power(x.astype(np.longdouble), e24)
+        f = e.astype(np.float32)
+        mismatches += int((f != d).sum())
+        toward = np.where(e > f.astype(np.longdouble), np.float32(np.inf), np.float32(-np.inf)).astype(np.float32)
+        mid = (f.astype(np.longdouble) + np.nextafter(f, toward).astype(np.longdouble)) / 2
+        closest = min(closest, float((np.abs(e - mid) / e).min()))
+    assert mismatches == 0
+    assert closest > 2.5e-16
