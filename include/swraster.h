@@ -298,6 +298,18 @@ SR_API int sr_prepare_voxel_grid(const sr_scene_desc *scene, int32_t grid_size,
  * keeps make it bake on the host instead. */
 SR_API int sr_prepare_voxel_grid_device(const sr_scene_desc *scene, int32_t grid_size,
                                         int32_t device, sr_voxel_grid *out);
+/* Voxel grid with the GI bake (the --gi path of load_scene,
+ * src/main.rs:238-282: build_active_voxel_mask, compute_sun_visibility,
+ * bake_voxel_gi, src/gi.rs:151-412): per active voxel 64 primary rays with
+ * a one-bounce surface estimate, projected to 4-coefficient SH, floored and
+ * blurred; inactive voxels keep the scene SH.  Fills sh4 (16 floats per
+ * voxel) and light_intensity.  device >= 0 bakes on that HIP device, -1 on
+ * `threads` host threads (0 = all); both give identical grids.  The scene
+ * needs its environment.  The reference's brotli .gi cache file is not
+ * read or written.  SR_ERR_CAPACITY: a ray met more than 32 translucent
+ * surfaces.  Free with sr_voxel_grid_free. */
+SR_API int sr_prepare_gi(const sr_scene_desc *scene, int32_t grid_size, int32_t device,
+                         int32_t threads, sr_voxel_grid *out);
 SR_API int sr_voxel_grid_free(sr_voxel_grid *grid);
 /* Scene bounds over all node-transformed positions (src/scene.rs:333-351). */
 SR_API int sr_scene_desc_bounds(const sr_scene_desc *scene, float min[3], float max[3],

@@ -166,6 +166,8 @@ _SIGS = {
                                                 C.POINTER(sr_environment)]),
     "sr_prepare_voxel_grid": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32, C.POINTER(sr_voxel_grid)]),
     "sr_voxel_grid_free": (C.c_int, [C.POINTER(sr_voxel_grid)]),
+    "sr_prepare_gi": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32, C.c_int32,
+                                C.POINTER(sr_voxel_grid)]),
     "sr_prepare_voxel_grid_device": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32,
                                                C.POINTER(sr_voxel_grid)]),
     "sr_scene_desc_bounds": (C.c_int, [C.POINTER(sr_scene_desc), C.c_float * 3, C.c_float * 3, C.c_float * 3,
@@ -374,8 +376,13 @@ class Environment:
 
 
 class VoxelGrid:
-    def __init__(self, dims, wmin, wmax, light, sh_scale=0.25, sky=1.0):
+    """Voxel lighting grid: per-voxel sun visibility and, after a GI bake,
+    16 SH floats per voxel (sh4, x fastest); without sh4 every voxel uses
+    sh_scale times the scene SH."""
+
+    def __init__(self, dims, wmin, wmax, light, sh_scale=0.25, sky=1.0, sh4=None):
         self.light = np.ascontiguousarray(light, np.float32).ravel()
+        self.sh4 = None if sh4 is None else np.ascontiguousarray(sh4, np.float32).ravel()
         self.desc = sr_voxel_grid()
         for k in range(3):
             self.desc.dims[k] = int(dims[k])
@@ -384,7 +391,7 @@ class VoxelGrid:
         self.desc.light_intensity = _fptr(self.light)
         self.desc.sh_scale = sh_scale
         self.desc.sky_visibility = sky
-        self.desc.sh4 = None
+        self.desc.sh4 = None if self.sh4 is None else _fptr(self.sh4)
         self.dims = tuple(int(d) for d in dims)
         self.wmin = np.array(wmin, np.float32)
         self.wmax = np.array(wmax, np.float32)
@@ -570,17 +577,34 @@ class SceneData:
         self.desc.voxels = C.pointer(voxels.desc)
         return self
 
-    def prepare(self, environment=None, grid=128, threads=0, cache_key=None, device=None):
+    def prepare(self, environment=None, grid=128, threads=0, cache_key=None, device=None, gi=False):
         """Attach the IBL environment and build (or load) the voxel grid; with
-        a device ordinal the voxel bake runs on that GPU (identical result)."""
+        a device ordinal the bakes run on that GPU (identical result).  gi
+        adds the voxel GI bake (the reference's --gi, src/main.rs:238-282)."""
         environment = environment or Environment.load(threads=threads)
         path = None
         if cache_key:
-            path = os.path.join(CACHE_DIR, f"voxels_{cache_key}_{grid}.npz")
+            path = os.path.join(CACHE_DIR, f"voxels_{cache_key}_{grid}{'_gi' if gi else ''}.npz")
             if os.path.exists(path):
                 z = np.load(path)
-                return self.attach(environment, VoxelGrid(z["dims"], z["wmin"], z["wmax"], z["light"]))
+                sh4 = z["sh4"] if gi else None
+                return self.attach(environment, VoxelGrid(z["dims"], z["wmin"], z["wmax"], z["light"],
+                                                          sh_scale=1.0 if gi else 0.25, sh4=sh4))
         g = sr_voxel_grid()
+        if gi:
+            self.environment = environment
+            self.desc.environment = C.pointer(environment.desc)
+            check(lib().sr_prepare_gi(C.byref(self.desc), grid, -1 if device is None else int(device),
+                                      threads, C.byref(g)))
+            n = g.dims[0] * g.dims[1] * g.dims[2]
+            light = np.ctypeslib.as_array(g.light_intensity, shape=(n,)).copy()
+            sh4 = np.ctypeslib.as_array(g.sh4, shape=(n * 16,)).copy()
+            dims, wmin, wmax = list(g.dims), list(g.world_min), list(g.world_max)
+            lib().sr_voxel_grid_free(C.byref(g))
+            if path:
+                os.makedirs(CACHE_DIR, exist_ok=True)
+                np.savez(path, dims=dims, wmin=wmin, wmax=wmax, light=light, sh4=sh4)
+            return self.attach(environment, VoxelGrid(dims, wmin, wmax, light, sh_scale=1.0, sh4=sh4))
         if device is None:
             check(lib().sr_prepare_voxel_grid(C.byref(self.desc), grid, threads, C.byref(g)))
         else:

@@ -263,7 +263,7 @@ inline uint32_t blocks(size_t n) { return (uint32_t)std::min<size_t>(std::max<si
     } while (0)
 
 int bake_voxels_device(const sr_scene_desc *d, int grid, int device, std::vector<float> &light, VoxelFrame &fr,
-                       bool &fallback) {
+                       bool &fallback, std::vector<uint8_t> *active_out) {
     fallback = false;
     voxel_frame(d, grid, fr);
     std::vector<WorldTri> tris;
@@ -370,6 +370,10 @@ int bake_voxels_device(const sr_scene_desc *d, int grid, int device, std::vector
     }
     light.resize(nvox);
     BAKE_TRY(hipMemcpy(light.data(), B.light, nvox * 4, hipMemcpyDeviceToHost));
+    if (active_out) {
+        active_out->resize(nvox);
+        BAKE_TRY(hipMemcpy(active_out->data(), B.active, nvox, hipMemcpyDeviceToHost));
+    }
     return SR_OK;
 }
 

@@ -462,7 +462,8 @@ static bool ray_tri(F3 o, F3 d, const Tri &t, float tmin, float tmax, float &tou
 // World-space triangles of every node in node order, primitive order and
 // triangle order, skipping degenerate ones (RayTracer::new,
 // src/raytracer.rs:172-223).  Parallel over chunks; the order is kept.
-void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<WorldTri> &out) {
+void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<WorldTri> &out,
+                             std::vector<float> *uvs) {
     struct Job { int node, prim; uint32_t first, count; };
     std::vector<Job> jobs;
     const uint32_t CHUNK = 1u << 16;
@@ -477,6 +478,7 @@ void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<Wo
         }
     }
     std::vector<std::vector<WorldTri>> parts(jobs.size());
+    std::vector<std::vector<float>> uparts(uvs ? jobs.size() : 0);
     parallel_for(jobs.size(), threads, [&](size_t j) {
         const Job &jb = jobs[j];
         const sr_node_desc &n = d->nodes[jb.node];
@@ -492,10 +494,20 @@ void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<Wo
             F3 fn = cross(p1 - p0, p2 - p0);
             if (dot(fn, fn) <= 1.0e-12f) continue;
             o.push_back({{p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z}, p.material});
+            if (uvs)
+                for (int k = 0; k < 3; k++) {
+                    uint32_t v = p.indices[3 * t + k];
+                    uparts[j].push_back(p.texcoords ? p.texcoords[2 * v] : 0.0f);
+                    uparts[j].push_back(p.texcoords ? p.texcoords[2 * v + 1] : 0.0f);
+                }
         }
     });
     out.clear();
     for (auto &pt : parts) out.insert(out.end(), pt.begin(), pt.end());
+    if (uvs) {
+        uvs->clear();
+        for (auto &pt : uparts) uvs->insert(uvs->end(), pt.begin(), pt.end());
+    }
 }
 
 // Grid placement and sun ray constants of the voxel bake (src/gi.rs:267-290,
@@ -519,7 +531,7 @@ void voxel_frame(const sr_scene_desc *d, int grid, VoxelFrame &f) {
 }
 
 int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<float> &light, float wmin[3],
-                     float wmax[3]) {
+                     float wmax[3], std::vector<uint8_t> *active_out) {
     VoxelFrame fr;
     voxel_frame(d, grid, fr);
     memcpy(wmin, fr.wmin, 12);
@@ -591,6 +603,7 @@ int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<
         translucent[i] = d->materials[i].is_translucent != 0;
         transmission[i] = d->materials[i].transmission;
     }
+    if (active_out) *active_out = active;
 
     // compute_sun_visibility (src/gi.rs:267-314)
     F3 ldir = f3(fr.ldir[0], fr.ldir[1], fr.ldir[2]);

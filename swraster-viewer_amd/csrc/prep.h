@@ -15,8 +15,9 @@ void compute_irradiance_sh4(const uint32_t *faces, uint32_t size, float sh[4][3]
 void compute_specular_cubemap(const uint32_t *faces, uint32_t size, uint32_t sample_count, std::vector<uint32_t> &out,
                               uint32_t &num_mips, int threads);
 void scene_bounds(const sr_scene_desc *d, float mn[3], float mx[3], float center[3], float *diag);
+// active_out (optional): the bake's active-voxel mask (occupancy dilated by 2).
 int build_voxel_grid(const sr_scene_desc *d, int grid, int threads, std::vector<float> &light, float wmin[3],
-                     float wmax[3]);
+                     float wmax[3], std::vector<uint8_t> *active_out = nullptr);
 
 // Shared by the host bake and the device bake (bake.hip).
 struct WorldTri {
@@ -32,12 +33,14 @@ struct VoxelFrame {
     float bias;          // ray origin offset along ldir
     float area_ref;      // occupancy sampling reference area
 };
-void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<WorldTri> &out);
+// uvs (optional): the three vertices' texture coordinates of each triangle.
+void collect_world_triangles(const sr_scene_desc *d, int threads, std::vector<WorldTri> &out,
+                             std::vector<float> *uvs = nullptr);
 void voxel_frame(const sr_scene_desc *d, int grid, VoxelFrame &f);
 // The bake on a HIP device (bake.hip).  fallback: a ray met more translucent
 // surfaces than the device keeps per ray; the caller bakes on the host.
 int bake_voxels_device(const sr_scene_desc *d, int grid, int device, std::vector<float> &light, VoxelFrame &fr,
-                       bool &fallback);
+                       bool &fallback, std::vector<uint8_t> *active_out = nullptr);
 // IBL inputs on a HIP device (envmap.hip).
 int prepare_environment_device(const uint32_t *faces, uint32_t size, int device, std::vector<uint32_t> &spec,
                                uint32_t &mips, std::vector<uint32_t> &lut, uint32_t lut_size, float sh[4][3]);
