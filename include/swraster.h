@@ -310,6 +310,25 @@ SR_API int sr_prepare_voxel_grid_device(const sr_scene_desc *scene, int32_t grid
  * surfaces.  Free with sr_voxel_grid_free. */
 SR_API int sr_prepare_gi(const sr_scene_desc *scene, int32_t grid_size, int32_t device,
                          int32_t threads, sr_voxel_grid *out);
+/* Cache files of the reference, same layout: a 20-byte little-endian header
+ * (magic, version, three u32) then a brotli stream (quality 6, window 22) of
+ * the little-endian payload.  Brotli is the system's libbrotlienc.so.1 /
+ * libbrotlidec.so.1, opened at run time (SR_ERR_UNSUPPORTED without them).
+ * Loads return SR_ERR_NOT_FOUND for a missing file or one the reference
+ * would not use (other magic, version, dimensions or payload size) and
+ * SR_ERR_INVALID_ARGUMENT for a corrupt stream; either way the caller bakes
+ * again, as the reference does.
+ * .gi (save_gi_cache / try_load_gi_cache, src/gi.rs:30-117): "VGI0", 8,
+ * dims; 16 floats per voxel (4 x rgbw, x fastest).  The load fills
+ * sh4[dims product * 16]. */
+SR_API int sr_gi_cache_save(const char *path, const sr_voxel_grid *grid);
+SR_API int sr_gi_cache_load(const char *path, const int32_t dims[3], float *sh4);
+/* .ggx (save_ / try_load_prefiltered_specular_cubemap_cache,
+ * src/texture.rs:426-552): "GGX0", 1, width, height, mips; every mip of
+ * the six faces at full face size.  The load fills specular[(1 +
+ * floor(log2(face_size))) * 6 * face_size^2]. */
+SR_API int sr_specular_cache_save(const char *path, const sr_environment *env);
+SR_API int sr_specular_cache_load(const char *path, uint32_t face_size, uint32_t *specular);
 SR_API int sr_voxel_grid_free(sr_voxel_grid *grid);
 /* Scene bounds over all node-transformed positions (src/scene.rs:333-351). */
 SR_API int sr_scene_desc_bounds(const sr_scene_desc *scene, float min[3], float max[3],

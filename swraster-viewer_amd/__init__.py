@@ -168,6 +168,10 @@ _SIGS = {
     "sr_voxel_grid_free": (C.c_int, [C.POINTER(sr_voxel_grid)]),
     "sr_prepare_gi": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32, C.c_int32,
                                 C.POINTER(sr_voxel_grid)]),
+    "sr_gi_cache_save": (C.c_int, [C.c_char_p, C.POINTER(sr_voxel_grid)]),
+    "sr_gi_cache_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
+    "sr_specular_cache_save": (C.c_int, [C.c_char_p, C.POINTER(sr_environment)]),
+    "sr_specular_cache_load": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "sr_prepare_voxel_grid_device": (C.c_int, [C.POINTER(sr_scene_desc), C.c_int32, C.c_int32,
                                                C.POINTER(sr_voxel_grid)]),
     "sr_scene_desc_bounds": (C.c_int, [C.POINTER(sr_scene_desc), C.c_float * 3, C.c_float * 3, C.c_float * 3,
@@ -375,6 +379,22 @@ class Environment:
         return cls(faces, spec, mips, lut, sh)
 
 
+def save_specular_cache(path, env):
+    """Write env's GGX-prefiltered cubemap as the reference's .ggx file
+    (src/texture.rs:520-552)."""
+    check(lib().sr_specular_cache_save(os.fsencode(path), C.byref(env.desc)))
+
+
+def load_specular_cache(path, face_size):
+    """The specular mip chain from a .ggx file (src/texture.rs:426-518), or
+    None when the file is missing or does not fit face_size."""
+    mips = 1 + int(face_size).bit_length() - 1
+    spec = np.empty(mips * 6 * face_size * face_size, np.uint32)
+    if lib().sr_specular_cache_load(os.fsencode(path), face_size, _uptr(spec)) != 0:
+        return None
+    return spec, mips
+
+
 class VoxelGrid:
     """Voxel lighting grid: per-voxel sun visibility and, after a GI bake,
     16 SH floats per voxel (sh4, x fastest); without sh4 every voxel uses
@@ -577,10 +597,13 @@ class SceneData:
         self.desc.voxels = C.pointer(voxels.desc)
         return self
 
-    def prepare(self, environment=None, grid=128, threads=0, cache_key=None, device=None, gi=False):
+    def prepare(self, environment=None, grid=128, threads=0, cache_key=None, device=None, gi=False,
+                gi_cache=None):
         """Attach the IBL environment and build (or load) the voxel grid; with
         a device ordinal the bakes run on that GPU (identical result).  gi
-        adds the voxel GI bake (the reference's --gi, src/main.rs:238-282)."""
+        adds the voxel GI bake (the reference's --gi, src/main.rs:238-282);
+        gi_cache names its .gi file (the reference uses the glTF path with
+        the extension .gi): loaded when it fits, else written after the bake."""
         environment = environment or Environment.load(threads=threads)
         path = None
         if cache_key:
@@ -591,6 +614,15 @@ class SceneData:
                 return self.attach(environment, VoxelGrid(z["dims"], z["wmin"], z["wmax"], z["light"],
                                                           sh_scale=1.0 if gi else 0.25, sh4=sh4))
         g = sr_voxel_grid()
+        if gi and gi_cache and os.path.exists(gi_cache):
+            # the reference's --gi load path (src/main.rs:247-276): sun
+            # visibility first, then the .gi cache if it fits the grid
+            sun = self.prepare(environment, grid=grid, threads=threads, device=device).voxels
+            sh4 = np.empty(int(np.prod(sun.dims)) * 16, np.float32)
+            dims = (C.c_int32 * 3)(*sun.dims)
+            if lib().sr_gi_cache_load(os.fsencode(gi_cache), dims, _fptr(sh4)) == 0:
+                return self.attach(environment, VoxelGrid(sun.dims, sun.wmin, sun.wmax, sun.light,
+                                                          sh_scale=1.0, sh4=sh4))
         if gi:
             self.environment = environment
             self.desc.environment = C.pointer(environment.desc)
@@ -600,6 +632,8 @@ class SceneData:
             light = np.ctypeslib.as_array(g.light_intensity, shape=(n,)).copy()
             sh4 = np.ctypeslib.as_array(g.sh4, shape=(n * 16,)).copy()
             dims, wmin, wmax = list(g.dims), list(g.world_min), list(g.world_max)
+            if gi_cache:
+                check(lib().sr_gi_cache_save(os.fsencode(gi_cache), C.byref(g)))
             lib().sr_voxel_grid_free(C.byref(g))
             if path:
                 os.makedirs(CACHE_DIR, exist_ok=True)
