@@ -391,6 +391,14 @@ SR_API int sr_renderer_render_scene(sr_renderer *renderer, const sr_scene *scene
 /* Renderer::update_auto_exposure: trimmed mean of the per-64px-tile centre
  * luminances, exponential adaptation with tau = 1 s. */
 SR_API int sr_renderer_update_auto_exposure(sr_renderer *renderer, float delta_time);
+/* The same step on the device (src/renderer.rs:258-290 run by one
+ * workgroup), queued on the renderer's stream with no host round trip; the
+ * blits that follow read the new exposure on the device, and
+ * sr_renderer_exposure copies it back.  The device's log2f / expf / powf may
+ * differ from the host libm's by an ulp, so the exposure agrees with
+ * sr_renderer_update_auto_exposure to a few ulps (RGBA8 within 1 LSB).
+ * Above 8192 reference tiles it runs the host step instead. */
+SR_API int sr_renderer_update_auto_exposure_device(sr_renderer *renderer, float delta_time);
 /* Renderer::blit_to_buffer: exposure, tonemap, quantize, pack into a host
  * buffer of width*height u32 owned by the caller.  Rows are written in
  * pairs; an odd last row is left untouched as in the reference. */

@@ -195,6 +195,7 @@ _SIGS = {
     "sr_renderer_destroy": (C.c_int, [_P]),
     "sr_renderer_render_scene": (C.c_int, [_P, _P, C.POINTER(sr_camera)]),
     "sr_renderer_update_auto_exposure": (C.c_int, [_P, C.c_float]),
+"sr_renderer_update_auto_exposure_device": (C.c_int, [_P, C.c_float]),
     "sr_renderer_blit_to_buffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t]),
     "sr_renderer_blit_to_device": (C.c_int, [_P, _P]),
     "sr_renderer_exposure": (C.c_float, [_P]),
@@ -697,9 +698,16 @@ class Renderer:
     def update_auto_exposure(self, delta_time):
         check(lib().sr_renderer_update_auto_exposure(self.handle, float(delta_time)))
 
+    def update_auto_exposure_device(self, delta_time):
+        """The exposure step on the device (no host round trip)."""
+        check(lib().sr_renderer_update_auto_exposure_device(self.handle, float(delta_time)))
+
     @property
     def exposure(self):
         return lib().sr_renderer_exposure(self.handle)
+
+    def set_exposure(self, exposure):
+        check(lib().sr_renderer_set_exposure(self.handle, float(exposure)))
 
     def blit_to_buffer(self, buffer=None):
         if buffer is None:

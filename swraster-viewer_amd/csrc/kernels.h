@@ -68,6 +68,11 @@ hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *
 
 // Exposure + tonemap + quantize + pack of one view's linear colour planes.
 hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
-                       hipStream_t stream);
+                       hipStream_t st, const float *exposure_dev = nullptr);
+// Most 64-px reference tiles the device exposure step sorts (8K UHD has 8160).
+#define SR_EXPOSURE_MAX 8192
+// state = {ev, target, exposure} on the device (k_exposure).
+hipError_t launch_exposure(const float *lum, uint32_t n, float dt, bool use_host, float ev, float target, float *state,
+                           hipStream_t st);
 
 }  // namespace sr

@@ -2361,7 +2361,8 @@ __global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
 // k_blit: exposure, tonemap, quantize, pack (src/renderer.rs:293-355)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_blit(const float *color, uint64_t plane, int W, int H, float exposure,
-                                              uint32_t *out) {
+                                              const float *exposure_dev, uint32_t *out) {
+    if (exposure_dev) exposure = exposure_dev[2];
     uint64_t n = (uint64_t)W * (uint64_t)(H & ~1);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float k = 0.2f;
@@ -2371,6 +2372,78 @@ __global__ void __launch_bounds__(256) k_blit(const float *color, uint64_t plane
             c[ch] = (x / (x + k)) * (1.0f + k);
         }
         out[i] = (sat_u8(c[0] * 255.0f) << 24) | (sat_u8(c[1] * 255.0f) << 16) | (sat_u8(c[2] * 255.0f) << 8) | 0xFFu;
+    }
+}
+
+// Renderer::update_auto_exposure (src/renderer.rs:258-290) on the device: one
+// workgroup sorts the tiles' log luminance in LDS (bitonic, on the order of
+// f32::total_cmp), one lane sums the trimmed middle in sorted order and
+// steps the exposure.  state = {ev, target, exposure}; with use_host the
+// step starts from (host_ev, host_target) instead of the stored state.
+// The device's log2f / expf / powf may differ from the host libm's by an
+// ulp, so the exposure agrees with the host step to a few ulps.
+__global__ void __launch_bounds__(1024) k_exposure(const float *lum, uint32_t n, float dt, int use_host, float host_ev,
+                                                   float host_target, float *state) {
+    __shared__ int32_t key[SR_EXPOSURE_MAX];
+    float ev = use_host ? host_ev : state[0];
+    float target = use_host ? host_target : state[1];
+    float dtc = maxps(dt, 0.0f);
+    float tau = maxps(1.0f, 1e-4f);
+    float alpha = 1.0f - expf(-(dtc / tau));
+    if (alpha == 0.0f) {    // a zero step: the exposure is 2^ev whatever the target
+        if (threadIdx.x == 0) {
+            state[0] = ev;
+            state[1] = target;
+            state[2] = powf(2.0f, ev);
+        }
+        return;
+    }
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        int32_t k = 0x7FFFFFFF;
+        if (i < n) {
+            float sv = log2f(maxps(lum[i], 1e-4f));
+            int32_t b = __float_as_int(sv);
+            k = b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+                uint32_t l = i ^ j;
+                if (l > i) {
+                    int32_t a = key[i], b = key[l];
+                    bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    if (threadIdx.x == 0) {
+        uint32_t trim = sat_u32(floorf((float)n * 0.10f));
+        trim = min(trim, (n - 1) / 2);
+        float sum = 0.0f;
+        for (uint32_t i = trim; i < n - trim; i++) {
+            int32_t k = key[i];
+            sum += __int_as_float(k ^ (int32_t)(((uint32_t)(k >> 31)) >> 1));
+        }
+        float mean = sum / (float)(n - 2 * trim);
+        float y = minps(maxps(0.45f, 0.0f), 1.0f);
+        float kk = 0.2f;
+        float denom = maxps(1.0f + kk - y, 1e-6f);
+        float meter = maxps((y * kk) / denom, 1e-4f);
+        float tev = log2f(meter) - mean;
+        float t = minps(maxps(powf(2.0f, tev), 0.05f), 32.0f);
+        float nev = ev + (log2f(t) - ev) * alpha;
+        state[0] = nev;
+        state[1] = t;
+        state[2] = powf(2.0f, nev);
     }
 }
 
@@ -2474,9 +2547,17 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
-                       hipStream_t st) {
+                       hipStream_t st, const float *exposure_dev) {
     uint64_t n = (uint64_t)W * (uint64_t)(H & ~1);
-    hipLaunchKernelGGL(k_blit, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, st, color, plane, W, H, exposure, out);
+    hipLaunchKernelGGL(k_blit, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, st, color, plane, W, H, exposure,
+                       exposure_dev, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_exposure(const float *lum, uint32_t n, float dt, bool use_host, float ev, float target, float *state,
+                           hipStream_t st) {
+    if (n == 0 || n > SR_EXPOSURE_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_exposure, dim3(1), dim3(1024), 0, st, lum, n, dt, use_host ? 1 : 0, ev, target, state);
     return hipGetLastError();
 }
 

@@ -28,6 +28,8 @@ struct sr_renderer {
     int device = 0;
     Pipeline pipe;
     DevBuf ids, depth, color, rgba, lum;
+    DevBuf expo;                 // {ev, target, exposure} of the device exposure step
+    bool expo_on_device = false; // the device state is newer than the host fields
     int ref_tiles = 0;
     float auto_exposure = DEFAULT_EXPOSURE, auto_exposure_target = DEFAULT_EXPOSURE, auto_exposure_ev = 1.0f;
     sr_frame_stats stats;
@@ -76,6 +78,7 @@ SR_API int sr_renderer_create(int32_t width, int32_t height, const sr_renderer_o
     if (!rc) rc = r->color.ensure(np * 12);
     if (!rc) rc = r->rgba.ensure(np * 4);
     if (!rc) rc = r->lum.ensure((size_t)r->ref_tiles * 4);
+    if (!rc) rc = r->expo.ensure(16);
     if (rc) {
         r->pipe.destroy();
         delete r;
@@ -97,6 +100,7 @@ SR_API int sr_renderer_destroy(sr_renderer *r) {
     r->color.release();
     r->rgba.release();
     r->lum.release();
+    r->expo.release();
     delete r;
     return SR_OK;
 }
@@ -131,9 +135,25 @@ SR_API int sr_renderer_render_scene(sr_renderer *r, const sr_scene *scene, const
     return SR_OK;
 }
 
+// Brings the host's exposure fields up to date with a device exposure step.
+static int sync_exposure(sr_renderer *r) {
+    if (!r->expo_on_device) return SR_OK;
+    hipSetDevice(r->device);
+    float st[4];
+    HIP_TRY(hipMemcpyAsync(st, r->expo.ptr, 12, hipMemcpyDeviceToHost, r->pipe.stream));
+    HIP_TRY(hipStreamSynchronize(r->pipe.stream));
+    r->auto_exposure_ev = st[0];
+    r->auto_exposure_target = st[1];
+    r->auto_exposure = st[2];
+    r->expo_on_device = false;
+    return SR_OK;
+}
+
 // Renderer::update_auto_exposure (src/renderer.rs:258-290)
 SR_API int sr_renderer_update_auto_exposure(sr_renderer *r, float dt) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    int src = sync_exposure(r);
+    if (src) return src;
     size_t n = (size_t)r->ref_tiles;
     if (n == 0) return SR_OK;
     float dtc = std::fmax(dt, 0.0f);
@@ -172,12 +192,29 @@ SR_API int sr_renderer_update_auto_exposure(sr_renderer *r, float dt) {
     return SR_OK;
 }
 
-SR_API float sr_renderer_exposure(const sr_renderer *r) { return r ? r->auto_exposure : 0.0f; }
+// The same step on the device, queued on the renderer's stream without a
+// host round trip; later blits read its exposure there.
+SR_API int sr_renderer_update_auto_exposure_device(sr_renderer *r, float dt) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (r->ref_tiles > SR_EXPOSURE_MAX) return sr_renderer_update_auto_exposure(r, dt);
+    hipSetDevice(r->device);
+    HIP_TRY(launch_exposure((const float *)r->lum.ptr, (uint32_t)r->ref_tiles, dt, !r->expo_on_device,
+                            r->auto_exposure_ev, r->auto_exposure_target, (float *)r->expo.ptr, r->pipe.stream));
+    r->expo_on_device = true;
+    return SR_OK;
+}
+
+SR_API float sr_renderer_exposure(const sr_renderer *r) {
+    if (!r) return 0.0f;
+    if (sync_exposure(const_cast<sr_renderer *>(r))) return 0.0f;
+    return r->auto_exposure;
+}
 
 SR_API int sr_renderer_set_exposure(sr_renderer *r, float exposure) {
     if (!r || !(exposure > 0.0f)) return fail(SR_ERR_INVALID_ARGUMENT, "exposure must be positive");
     r->auto_exposure = exposure;
     r->auto_exposure_ev = std::log2(exposure);
+    r->expo_on_device = false;
     return SR_OK;
 }
 
@@ -186,7 +223,8 @@ SR_API int sr_renderer_blit_to_device(sr_renderer *r, void *device_pixels) {
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
     hipSetDevice(r->device);
     HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure,
-                        (uint32_t *)device_pixels, r->pipe.stream));
+                        (uint32_t *)device_pixels, r->pipe.stream,
+                        r->expo_on_device ? (const float *)r->expo.ptr : nullptr));
     HIP_TRY(hipStreamSynchronize(r->pipe.stream));
     return SR_OK;
 }
@@ -198,7 +236,8 @@ SR_API int sr_renderer_blit_to_buffer(sr_renderer *r, uint32_t *pixels, size_t w
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
     hipSetDevice(r->device);
     HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure,
-                        (uint32_t *)r->rgba.ptr, r->pipe.stream));
+                        (uint32_t *)r->rgba.ptr, r->pipe.stream,
+                        r->expo_on_device ? (const float *)r->expo.ptr : nullptr));
     size_t rows = (size_t)(r->H & ~1);
     HIP_TRY(hipMemcpyAsync(pixels, r->rgba.ptr, rows * (size_t)r->W * 4, hipMemcpyDeviceToHost, r->pipe.stream));
     HIP_TRY(hipStreamSynchronize(r->pipe.stream));
