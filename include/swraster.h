@@ -206,6 +206,9 @@ typedef struct sr_frame_stats {
     float ms_total;
     float ms_raster;               /* coverage, depth and visibility buffer */
     float ms_shade;                /* visibility-buffer shading */
+    uint32_t depth_rejection;      /* 1: the raster rejected entries behind the
+                                      tile's stored depth (chosen from the
+                                      previous frame's overdraw; same result) */
 } sr_frame_stats;
 
 /* ---- batched views and render server ------------------------------------ */

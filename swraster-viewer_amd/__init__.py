@@ -119,7 +119,7 @@ class sr_frame_stats(C.Structure):
                 ("binned_pairs", C.c_uint64), ("tiles", C.c_uint32), ("draws", C.c_uint32),
                 ("ms_setup", C.c_float), ("ms_bin", C.c_float), ("ms_raster_shade", C.c_float),
                 ("ms_blit", C.c_float), ("ms_total", C.c_float), ("ms_raster", C.c_float),
-                ("ms_shade", C.c_float)]
+                ("ms_shade", C.c_float), ("depth_rejection", C.c_uint32)]
 
 
 class sr_view(C.Structure):

@@ -54,6 +54,7 @@ struct FrameArgs {
     uint32_t tr_cap;
     float4 *center_color;         // 4 (colour, depth) per 64px tile
     int32_t persistent_blocks;    // 0 = one block per item
+    int32_t hiz;                  // 16-px raster with depth rejection between task rounds
     hipEvent_t events[5];         // optional stage markers: start, setup done, bins done, shade done, raster done
 };
 

@@ -171,6 +171,28 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
     return -1;
 }
 
+// The frame's counters and status cleared (by k_transform's threads, or by
+// k_frame_begin when there are no vertices).
+struct FrameBegin {
+    DevStatus *status;
+    uint32_t *bin_count, *sentinel, *tr_count;
+    uint32_t items, tr_slots;
+};
+
+__device__ __forceinline__ void frame_begin(const FrameBegin &F) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        uint32_t *w = reinterpret_cast<uint32_t *>(F.status);
+        for (uint32_t k = 0; k < (uint32_t)(sizeof(DevStatus) / 4); k++) w[k] = 0u;
+        *F.sentinel = 0u;
+    }
+    for (uint32_t k = i; k < F.items; k += gridDim.x * blockDim.x) F.bin_count[k] = 0u;
+    if (F.tr_count)
+        for (uint32_t k = i; k < F.tr_slots; k += gridDim.x * blockDim.x) F.tr_count[k] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_frame_begin(FrameBegin F) { frame_begin(F); }
+
 // ---------------------------------------------------------------------------
 // k_transform: one thread per (draw, vertex)
 // ---------------------------------------------------------------------------
@@ -230,7 +252,8 @@ __device__ __forceinline__ int wave_uniform(int d) {
 __global__ void __launch_bounds__(256) k_transform(const DevView *__restrict__ views,
                                                    const DevDraw *__restrict__ draws, int ndraws, uint32_t total,
                                                    float4 *__restrict__ xp, float4 *__restrict__ xa,
-                                                   float4 *__restrict__ xt) {
+                                                   float4 *__restrict__ xt, FrameBegin F) {
+    frame_begin(F);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
         int d = find_last_le(draws, ndraws, i, &DevDraw::xv_base);
         int du = wave_uniform(d);
@@ -542,21 +565,6 @@ hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2);
     return hipGetLastError();
-}
-
-// Frame start in one launch: clear the status, zero the bin counts and set
-// the zero sentinel after the per-triangle counts.
-__global__ void __launch_bounds__(256) k_frame_begin(DevStatus *status, uint32_t *bin_count, uint32_t items,
-                                                     uint32_t *sentinel, uint32_t *tr_count, uint32_t tr_slots) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        uint32_t *w = reinterpret_cast<uint32_t *>(status);
-        for (uint32_t k = 0; k < (uint32_t)(sizeof(DevStatus) / 4); k++) w[k] = 0u;
-        *sentinel = 0u;
-    }
-    for (uint32_t k = i; k < items; k += gridDim.x * blockDim.x) bin_count[k] = 0u;
-    if (tr_count)
-        for (uint32_t k = i; k < tr_slots; k += gridDim.x * blockDim.x) tr_count[k] = 0u;
 }
 
 
@@ -1740,6 +1748,32 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
 #undef SF
 }
 
+// A depth key no fragment of the triangle goes below, or 0 (no bound).  The
+// depth ((za + b1 zda) + b2 zdb) / ((oa + b1 oda) + b2 odb) is a ratio of
+// affine functions of the barycentrics, so over the triangle it lies between
+// its values at the vertices when the denominator keeps its sign.  Covered
+// samples have b1, b2 >= 0 and b1 + b2 <= 1 to within 4e-7 (the edge values
+// and 1/area rounded once each) and are evaluated to a few ulps; with the
+// vertices' denominators within a factor 16 of each other the depth's
+// derivative in b is below 48 times the largest vertex depth, so a margin of
+// 1e-4 of that depth covers both with room to spare.
+__device__ __forceinline__ uint32_t tri_zmin_key(float za, float zda, float zdb, float oa, float oda, float odb) {
+    float d0 = oa, d1 = oa + oda, d2 = oa + odb;
+    float dmin = fminf(d0, fminf(d1, d2)), dmax = fmaxf(d0, fmaxf(d1, d2));
+    if (!(dmin > 0.0f) || !(dmax <= 16.0f * dmin)) return 0u;
+    float z0 = za / d0, z1 = (za + zda) / d1, z2 = (za + zdb) / d2;
+    float zmin = fminf(z0, fminf(z1, z2));
+    float zabs = fmaxf(fabsf(z0), fmaxf(fabsf(z1), fabsf(z2)));
+    float lo = zmin - (zabs * 1e-4f + 1e-30f);
+    if (!(lo == lo) || !(zabs < 1e30f)) return 0u;
+    return depth_key(lo);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    return v;
+}
+
 __device__ __forceinline__ int find_view(const RasterArgs &P, uint32_t item) {
     int lo = 0, hi = P.num_views - 1;
     while (lo < hi) {
@@ -1759,7 +1793,7 @@ __device__ __forceinline__ bool centre_tile(int tx0, int ty0) { return ((tx0 & 6
 // entry into an LDS depth buffer, then the visibility buffer (triangle id per
 // pixel, or none).
 // ---------------------------------------------------------------------------
-template <int T, int NT>
+template <int T, int NT, bool HIZ>
 __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long long *keys, uint32_t *stage) {
     constexpr int NW = NT / 64;               // waves
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1779,6 +1813,8 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
     // quad) tasks that all NT threads then walk, so one-pixel and full-tile
     // triangles, and tiles with few or many entries, keep every lane busy.
     __shared__ int wtot[NW];
+    __shared__ uint32_t wzmax[NW];
+    __shared__ uint32_t zmin_s[HIZ ? NT : 1];
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
     if (end > P.pair_cap) end = P.pair_cap;
@@ -1807,8 +1843,35 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
             total += wtot[w];
         }
         stage[F_INCL * NT + tid] = (uint32_t)incl;
+        // HIZ: with many rounds of quad tasks (high overdraw), the tile's
+        // farthest stored depth, taken between rounds, rejects the quads of
+        // entries that lie wholly behind it (they could not win a pixel).
+        uint32_t zmax = 0xFFFFFFFFu;
+        bool hiz = false;
+        if constexpr (HIZ) {
+            hiz = total > 4 * NT || c0 > start;
+            if (hiz) {
+#define SFE(f) __uint_as_float(stage[(f) * NT + tid])
+                zmin_s[tid] =
+                    nq > 0 ? tri_zmin_key(SFE(F_ZA), SFE(F_ZDA), SFE(F_ZDB), SFE(F_OA), SFE(F_ODA), SFE(F_ODB)) : 0u;
+#undef SFE
+            }
+        }
         __syncthreads();
         for (int k0 = 0; k0 < total; k0 += NT) {
+            if constexpr (HIZ) {
+                if (hiz && (k0 > 0 || c0 > start)) {
+                    __syncthreads();
+                    uint32_t m = 0u;
+                    for (int i = tid; i < T * T; i += NT) m = max(m, (uint32_t)(keys[i] >> 32));
+                    m = wave_max_u32(m);
+                    if (lane == 0) wzmax[wave] = m;
+                    __syncthreads();
+                    zmax = 0u;
+#pragma unroll
+                    for (int w = 0; w < NW; w++) zmax = max(zmax, wzmax[w]);
+                }
+            }
             int k = k0 + tid;
             if (k < total) {
                 // owner = number of entries whose inclusive count is <= k
@@ -1817,7 +1880,9 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
                 for (int step = NT / 2; step > 0; step >>= 1)
                     if ((int)stage[F_INCL * NT + o + step - 1] <= k) o += step;
                 int before = o ? (int)stage[F_INCL * NT + o - 1] : 0;
-                walk_quad<T, NT>(S, P, stage, o, k - before, tx0, ty0, keys);
+                bool go = true;
+                if constexpr (HIZ) go = !hiz || zmin_s[o] <= zmax;
+                if (go) walk_quad<T, NT>(S, P, stage, o, k - before, tx0, ty0, keys);
             }
         }
         __syncthreads();
@@ -1838,12 +1903,12 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
     }
 }
 
-template <int T, int NT>
+template <int T, int NT, bool HIZ = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32 ? 4 : 5))) k_raster(RasterArgs P) {
     __shared__ unsigned long long keys[T * T];
     __shared__ uint32_t stage[Stage<T>::COUNT * NT];
     if (!P.persistent) {
-        raster_item<T, NT>(P, blockIdx.x, keys, stage);
+        raster_item<T, NT, HIZ>(P, blockIdx.x, keys, stage);
         return;
     }
     __shared__ uint32_t next;
@@ -1853,7 +1918,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32
         uint32_t item = next;
         __syncthreads();
         if (item >= P.total_items) break;
-        raster_item<T, NT>(P, item, keys, stage);
+        raster_item<T, NT, HIZ>(P, item, keys, stage);
         __syncthreads();
     }
 }
@@ -2460,12 +2525,18 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
 hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const uint32_t GRID_CAP = 256 * 16;
     if (a.events[0]) hipEventRecord(a.events[0], st);
+    FrameBegin fb;
+    fb.status = a.status;
+    fb.bin_count = a.bin_count;
+    fb.items = a.total_items;
+    fb.sentinel = a.tri_counts + a.total_in;
+    fb.tr_count = a.has_translucent ? a.tr_count : nullptr;
+    fb.tr_slots = a.tr_slots;
     if (a.total_xv > 0)
         hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr);
-    hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, a.status,
-                       a.bin_count, a.total_items, a.tri_counts + a.total_in,
-                       a.has_translucent ? a.tr_count : nullptr, a.tr_slots);
+                           a.num_draws, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr, fb);
+    else
+        hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, fb);
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv, a.tri_counts, a.clip_queue, a.clip_cap, a.status);
@@ -2528,7 +2599,10 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     switch (a.tile_size) {
         case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, st, P); break;
         case 32: hipLaunchKernelGGL((k_raster<32, 256>), dim3(grid), dim3(256), 0, st, P); break;
-        default: hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, st, P); break;
+        default:
+            if (a.hiz) hipLaunchKernelGGL((k_raster<16, 256, true>), dim3(grid), dim3(256), 0, st, P);
+            else hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, st, P);
+            break;
     }
     if (a.events[4]) hipEventRecord(a.events[4], st);
     switch (a.tile_size) {

@@ -49,7 +49,10 @@ struct sr_batch {
 
 static float ev_ms(hipEvent_t a, hipEvent_t b) {
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();    // an unrecorded stage event is not an error of a later launch
+        return 0.0f;
+    }
     return ms;
 }
 
@@ -126,6 +129,7 @@ SR_API int sr_renderer_render_scene(sr_renderer *r, const sr_scene *scene, const
     r->stats.binned_pairs = r->pipe.last_pairs;
     r->stats.tiles = items;
     r->stats.draws = (uint32_t)draws.size();
+    r->stats.depth_rejection = r->pipe.last_hiz ? 1u : 0u;
     r->stats.ms_setup = ev_ms(r->pipe.ev[0], r->pipe.ev[1]);
     r->stats.ms_bin = ev_ms(r->pipe.ev[1], r->pipe.ev[2]);
     r->stats.ms_raster_shade = ev_ms(r->pipe.ev[2], r->pipe.ev[3]);

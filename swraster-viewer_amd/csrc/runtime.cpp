@@ -205,7 +205,7 @@ void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
-    DevBuf *bufs[] = {&views, &draws, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
+    DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
                       &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf, &tr_count,
                       &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color};
     for (DevBuf *b : bufs) b->release();
@@ -234,7 +234,7 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
     k.n[4] = a.total_items; k.n[5] = (uint64_t)a.tile_size; k.n[6] = (uint64_t)a.any_tangents; k.n[7] = a.rec_cap;
     k.n[8] = a.pair_cap; k.n[9] = a.clip_cap; k.n[10] = (uint64_t)a.persistent_blocks; k.n[11] = vbytes;
     k.n[12] = dbytes; k.n[13] = a.color_plane; k.n[14] = (uint64_t)a.has_translucent;
-    k.n[15] = ((uint64_t)a.tr_cap << 32) | a.tr_slots;
+    k.n[15] = ((uint64_t)a.tr_cap << 32) | a.tr_slots | ((uint64_t)a.hiz << 31);
     const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
                           a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
@@ -245,9 +245,7 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
     if (!graph_exec || memcmp(&k, &graph_key, sizeof(k)) != 0) {
         drop_graph();
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipMemcpyAsync((void *)a.views, stage.ptr, vbytes, hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess && a.num_draws > 0)
-            e = hipMemcpyAsync((void *)a.draws, (char *)stage.ptr + vbytes, dbytes, hipMemcpyHostToDevice, stream);
+        hipError_t e = hipMemcpyAsync((void *)a.views, stage.ptr, vbytes + dbytes, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = launch_frame(a, stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(status_host.ptr, a.status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream);
@@ -273,13 +271,15 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
                   uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks, bool sync,
                   bool use_graph) {
     HIP_TRY(hipSetDevice(device));
-    size_t vbytes = hv.size() * sizeof(DevView), dbytes = std::max<size_t>(hd.size(), 1) * sizeof(DevDraw);
+    // the view and draw tables share one device buffer (draws at a 256-byte
+    // aligned offset) so one copy uploads both
+    size_t vbytes = (hv.size() * sizeof(DevView) + 255) & ~(size_t)255;
+    size_t dbytes = std::max<size_t>(hd.size(), 1) * sizeof(DevDraw);
     int r;
     if ((r = stage.ensure(vbytes + dbytes))) return r;
-    memcpy(stage.ptr, hv.data(), vbytes);
+    memcpy(stage.ptr, hv.data(), hv.size() * sizeof(DevView));
     if (!hd.empty()) memcpy((char *)stage.ptr + vbytes, hd.data(), hd.size() * sizeof(DevDraw));
-    if ((r = views.ensure(vbytes))) return r;
-    if ((r = draws.ensure(dbytes))) return r;
+    if ((r = views.ensure(vbytes + dbytes))) return r;
     if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 48))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
     if ((r = tri_counts.ensure(((size_t)total_in + 1) * 4))) return r;
@@ -317,7 +317,7 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         memset(&a, 0, sizeof(a));
         a.views = (const DevView *)views.ptr;
         a.num_views = (int)hv.size();
-        a.draws = (const DevDraw *)draws.ptr;
+        a.draws = (const DevDraw *)((char *)views.ptr + vbytes);
         a.num_draws = (int)hd.size();
         a.total_xv = total_xv;
         a.total_in = total_in;
@@ -360,13 +360,14 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
             a.center_color = (float4 *)center_color.ptr;
         }
         a.persistent_blocks = persistent_blocks;
+        // depth rejection pays where triangles span many tiles and tiles
+        // hold many of them (high overdraw); judged from the last frame
+        a.hiz = tile_size == 16 && last_pairs > 32ull * total_items && last_pairs > 4ull * last_records;
+        last_hiz = a.hiz != 0;
         if (use_graph) {
-            if ((r = launch_graph(a, vbytes, hd.empty() ? 0 : hd.size() * sizeof(DevDraw)))) return r;
+            if ((r = launch_graph(a, vbytes, dbytes))) return r;
         } else {
-            HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, vbytes, hipMemcpyHostToDevice, stream));
-            if (!hd.empty())
-                HIP_TRY(hipMemcpyAsync(draws.ptr, (char *)stage.ptr + vbytes, hd.size() * sizeof(DevDraw),
-                                       hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, vbytes + dbytes, hipMemcpyHostToDevice, stream));
             for (int i = 0; i < 5; i++) a.events[i] = ev[i];
             HIP_TRY(launch_frame(a, stream));
             HIP_TRY(hipMemcpyAsync(status_host.ptr, status.ptr, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));

@@ -58,12 +58,13 @@ struct Pipeline {
     int tile_size = 16;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    DevBuf views, draws, xv, xt, tri_counts, tri_base, bin_count, bin_start, bin_end, partials, list;
+    DevBuf views, xv, xt, tri_counts, tri_base, bin_count, bin_start, bin_end, partials, list;
     DevBuf rec_geo, rec_raster, status, center, clip_queue, clip_buf;
     DevBuf tr_count, tr_start, tr_end, tr_list, tr_scratch, center_color;
     PinBuf stage, status_host;
     uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0, tr_cap = 0;
     uint32_t last_records = 0, last_pairs = 0;
+    bool last_hiz = false;            // the last frame's raster used depth rejection
     struct GraphKey {
         uint64_t n[16];
         uint64_t p[32];

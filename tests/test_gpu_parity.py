@@ -165,6 +165,67 @@ def test_c3_parity(sr, orc, scenes, env):
     compare(sr, g, o, "c3")
 
 
+def _two_frames(sr, gscene, cam, w, h):
+    """The same frame twice on one renderer: the second takes the raster
+    variant chosen from the first frame's statistics."""
+    r = sr.Renderer(w, h)
+    out = []
+    for _ in range(2):
+        r.render_scene(gscene, cam)
+        ids, depth = r.read_vbuffer()
+        out.append((ids, depth, r.stats()["depth_rejection"]))
+    r.close()
+    return out
+
+
+def test_c3_depth_rejection_parity(sr, orc, scenes, env):
+    """C3's overdraw selects the raster that rejects entries behind the
+    tile's stored depth; its visibility buffer is the oracle's."""
+    scene, spec = scenes.make_c3()
+    scene.prepare(env, cache_key=scene.name)
+    gscene = sr.Scene(scene)
+    cam = scenes.make_camera(spec)
+    w, h = spec["width"], spec["height"]
+    (i1, d1, h1), (i2, d2, h2) = _two_frames(sr, gscene, cam, w, h)
+    # (the first frame may already take it: a frame that outgrows its
+    # buffers is rendered again with the statistics it gathered)
+    assert h2 == 1
+    o = render_oracle(orc, scene, cam, w, h, threads=16)
+    for ids, depth in ((i1, d1), (i2, d2)):
+        assert np.array_equal(ids, o["ids"]) and np.array_equal(bits(depth), bits(o["depth"]))
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_depth_rejection_tilted_overdraw(sr, orc, scenes, seed):
+    """Large overlapping triangles whose vertices lie at different depths
+    (1 to 100, so the rejection bound's margin matters; some with a depth
+    ratio above the bound's limit of 16): the depth-rejecting raster matches
+    the oracle bit for bit."""
+    from util import screen_triangle_scene
+    W, H = 640, 360
+    rng = np.random.default_rng(seed)
+    n = 3000
+    c = rng.uniform([0, 0], [W * 16, H * 16], (n, 1, 2))
+    ang = rng.uniform(0, 2 * np.pi, (n, 3))
+    rad = rng.uniform(60, 250, (n, 3)) * 16
+    tris = np.round(c + np.stack([np.cos(ang), np.sin(ang)], -1) * rad[..., None])
+    tris = np.clip(tris, -2000, (W + 100) * 16)
+    a = (tris[:, 1, 0] - tris[:, 0, 0]) * (tris[:, 2, 1] - tris[:, 0, 1]) - \
+        (tris[:, 2, 0] - tris[:, 0, 0]) * (tris[:, 1, 1] - tris[:, 0, 1])
+    tris[a > 0] = tris[a > 0][:, ::-1]
+    depths = rng.uniform(1.0, 100.0, (n, 3))
+    scene, cam = screen_triangle_scene(sr, tris, depths, W, H)
+    from util import attach_dummy
+    attach_dummy(sr, scene)
+    gscene = sr.Scene(scene)
+    camera = scenes.make_camera(cam)
+    (i1, d1, h1), (i2, d2, h2) = _two_frames(sr, gscene, camera, W, H)
+    assert h2 == 1, "the overdraw did not select the depth-rejecting raster"
+    o = render_oracle(orc, scene, camera, W, H)
+    for ids, depth in ((i1, d1), (i2, d2)):
+        assert np.array_equal(ids, o["ids"]) and np.array_equal(bits(depth), bits(o["depth"]))
+
+
 def test_determinism(sr, scenes, c1):
     scene, spec, gscene = c1
     cam = scenes.make_camera(spec)
