@@ -1662,7 +1662,7 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
 // depth enter the 64-bit minimum.
 template <int T, int SW>
 __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P, const uint32_t *st, int o, int local,
-                                          int tx0, int ty0, unsigned long long *keys) {
+                                          int tx0, int ty0, unsigned long long *keys, uint32_t zbound) {
     using SG = Stage<T>;
     constexpr int NR = SG::NR;
 #define SF(f) __uint_as_float(st[(f) * SW + o])
@@ -1678,6 +1678,14 @@ __device__ __forceinline__ void walk_quad(const DevScene &S, const RasterArgs &P
     int col = (int)((st[(SG::LO + row / 8) * SW + o] >> (4 * (row & 7))) & 15u) + (local - before);
     uint32_t q0 = st[F_Q0 * SW + o];
     int qpx = (int)(q0 & 0xFFFFu) + 2 * col, qpy = (int)(q0 >> 16) + 2 * row;
+    if (zbound) {
+        unsigned long long *kz = keys + (qpy - ty0) * T + (qpx - tx0);
+        auto depth_of = [](unsigned long long *k) {
+            return (uint32_t)(__hip_atomic_load(k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> 32);
+        };
+        uint32_t qmax = max(max(depth_of(kz), depth_of(kz + 1)), max(depth_of(kz + T), depth_of(kz + T + 1)));
+        if (zbound > qmax) return;
+    }
 
     uint32_t org = st[F_ORG * SW + o], flags = st[F_FLAGS * SW + o];
     int ox0 = (int)(org & 0xFFFFu), oy0 = (int)(org >> 16);
@@ -1769,11 +1777,6 @@ __device__ __forceinline__ uint32_t tri_zmin_key(float za, float zda, float zdb,
     return depth_key(lo);
 }
 
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
-    return v;
-}
-
 __device__ __forceinline__ int find_view(const RasterArgs &P, uint32_t item) {
     int lo = 0, hi = P.num_views - 1;
     while (lo < hi) {
@@ -1813,7 +1816,6 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
     // quad) tasks that all NT threads then walk, so one-pixel and full-tile
     // triangles, and tiles with few or many entries, keep every lane busy.
     __shared__ int wtot[NW];
-    __shared__ uint32_t wzmax[NW];
     __shared__ uint32_t zmin_s[HIZ ? NT : 1];
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
@@ -1843,35 +1845,16 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
             total += wtot[w];
         }
         stage[F_INCL * NT + tid] = (uint32_t)incl;
-        // HIZ: with many rounds of quad tasks (high overdraw), the tile's
-        // farthest stored depth, taken between rounds, rejects the quads of
-        // entries that lie wholly behind it (they could not win a pixel).
-        uint32_t zmax = 0xFFFFFFFFu;
-        bool hiz = false;
+        // HIZ: a quad task is dropped when its entry's depth bound lies
+        // behind all four of the quad's stored depths (the entry could not
+        // win any of them; a stale read only keeps more work).
         if constexpr (HIZ) {
-            hiz = total > 4 * NT || c0 > start;
-            if (hiz) {
 #define SFE(f) __uint_as_float(stage[(f) * NT + tid])
-                zmin_s[tid] =
-                    nq > 0 ? tri_zmin_key(SFE(F_ZA), SFE(F_ZDA), SFE(F_ZDB), SFE(F_OA), SFE(F_ODA), SFE(F_ODB)) : 0u;
+            zmin_s[tid] = nq > 0 ? tri_zmin_key(SFE(F_ZA), SFE(F_ZDA), SFE(F_ZDB), SFE(F_OA), SFE(F_ODA), SFE(F_ODB)) : 0u;
 #undef SFE
-            }
         }
         __syncthreads();
         for (int k0 = 0; k0 < total; k0 += NT) {
-            if constexpr (HIZ) {
-                if (hiz && (k0 > 0 || c0 > start)) {
-                    __syncthreads();
-                    uint32_t m = 0u;
-                    for (int i = tid; i < T * T; i += NT) m = max(m, (uint32_t)(keys[i] >> 32));
-                    m = wave_max_u32(m);
-                    if (lane == 0) wzmax[wave] = m;
-                    __syncthreads();
-                    zmax = 0u;
-#pragma unroll
-                    for (int w = 0; w < NW; w++) zmax = max(zmax, wzmax[w]);
-                }
-            }
             int k = k0 + tid;
             if (k < total) {
                 // owner = number of entries whose inclusive count is <= k
@@ -1880,9 +1863,9 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
                 for (int step = NT / 2; step > 0; step >>= 1)
                     if ((int)stage[F_INCL * NT + o + step - 1] <= k) o += step;
                 int before = o ? (int)stage[F_INCL * NT + o - 1] : 0;
-                bool go = true;
-                if constexpr (HIZ) go = !hiz || zmin_s[o] <= zmax;
-                if (go) walk_quad<T, NT>(S, P, stage, o, k - before, tx0, ty0, keys);
+                uint32_t zb = 0u;
+                if constexpr (HIZ) zb = zmin_s[o];
+                walk_quad<T, NT>(S, P, stage, o, k - before, tx0, ty0, keys, zb);
             }
         }
         __syncthreads();
