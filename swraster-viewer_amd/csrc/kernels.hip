@@ -2004,13 +2004,13 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
     }
 }
 
-// Six waves per SIMD (80 VGPRs and a 28-byte spill) hide more of the shading
-// chain's dependent memory latency than four waves without a spill
-// (measured: 0.90x the time at 1080p, 0.85x at 2160p).  The 32-px variant
-// loops over four pixels per thread and would spill far more, so it keeps
-// four.
+// Five waves per SIMD (up to 96 VGPRs, no spill) hide more of the shading
+// chain's dependent memory latency than four (measured 0.90x the time at
+// 1080p, 0.85x at 2160p), and beat six, whose 80 VGPRs spill 56 bytes per
+// thread to scratch (0.98x the time at 1080p, equal at 2160p).  The 32-px
+// variant loops over four pixels per thread and keeps four waves.
 template <int T, int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32 ? 4 : 6))) k_shade(RasterArgs P) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32 ? 4 : 5))) k_shade(RasterArgs P) {
     shade_item<T, NT>(P, blockIdx.x);
 }
 
