@@ -295,22 +295,33 @@ __device__ __forceinline__ int classify_input_triangle(const DevDraw &D, const D
     return clip_classify(c);
 }
 
+// chunk[c] gets the number of triangles of chunk c (input triangles 256c ..
+// 256c + 255) that lie inside the frustum; k_clip adds the fans of the
+// clipped ones, and k_scan_top turns the sums into each chunk's first id.
 __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const DevDraw *draws, int ndraws,
                                                     uint32_t total, const float4 *xp, uint32_t *counts,
-                                                    uint32_t *queue, uint32_t qcap, DevStatus *status) {
+                                                    uint32_t *chunk, uint32_t *queue, uint32_t qcap,
+                                                    DevStatus *status) {
+    __shared__ uint32_t wsum[4];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t t = base + threadIdx.x;
-        bool q = false;
+        bool q = false, in = false;
         if (t < total) {
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
             int cls = du >= 0 ? classify_input_triangle(draws[du], views, t, xp)
                               : classify_input_triangle(draws[d], views, t, xp);
-            counts[t] = cls == -1 ? 1u : 0u;
+            in = cls == -1;
+            counts[t] = in ? 1u : 0u;
             q = cls >= 0;
         }
+        uint32_t nin = (uint32_t)__popcll(__ballot(in));
+        __syncthreads();                 // the previous chunk's sums are read
+        if (lane == 0) wsum[wave] = nin;
+        __syncthreads();
+        if (threadIdx.x == 0) chunk[base >> 8] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
         uint64_t m = __ballot(q);
         if (m) {
             int leader = __ffsll((unsigned long long)m) - 1;
@@ -342,7 +353,7 @@ __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const 
 
 __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDraw *draws, int ndraws,
                                               const float4 *xp, const float4 *xa, const float4 *xt, uint32_t *counts,
-                                              const uint32_t *queue, uint32_t qcap, float *buf,
+                                              uint32_t *chunk, const uint32_t *queue, uint32_t qcap, float *buf,
                                               DevStatus *status) {
     __shared__ float sh[16][2][CLIP_MAXV][16];
     const int g = threadIdx.x >> 4, c = threadIdx.x & 15;
@@ -404,7 +415,11 @@ __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDra
         }
         if (len < 3) len = 0;
         for (int i = 0; i < len; i++) buf[((size_t)e * CLIP_MAXV + i) * 16 + c] = P[i][c];
-        if (c == 0) counts[t] = len ? (uint32_t)(len - 2) : 0u;
+        if (c == 0) {
+            uint32_t fan = len ? (uint32_t)(len - 2) : 0u;
+            counts[t] = fan;
+            if (fan) atomicAdd(&chunk[t >> 8], fan);
+        }
         wave_lds_sync();
     }
 }
@@ -498,7 +513,8 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_partials(const uint32_t *in
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t nparts, uint32_t *total_out) {
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t nparts, uint32_t *total_out,
+                                                  uint32_t *total_out2) {
     __shared__ uint32_t sh[17];
     uint32_t carry = 0;
     for (uint32_t c = 0; c < nparts; c += 1024) {
@@ -509,7 +525,10 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t 
         if (i < nparts) partials[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0 && total_out) *total_out = carry;
+    if (threadIdx.x == 0) {
+        if (total_out) *total_out = carry;
+        if (total_out2) *total_out2 = carry;
+    }
 }
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, uint32_t n, const uint32_t *partials,
@@ -562,7 +581,7 @@ hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *
     }
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total, nullptr);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2);
     return hipGetLastError();
 }
@@ -931,17 +950,24 @@ __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, co
                          !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
 }
 
+// The post-clip id of input triangle t is its chunk's first id plus the
+// fan counts before it in the chunk (a workgroup scan); it is stored for
+// k_clip_emit and the per-view offsets.
 __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
-                                               const float4 *xp, const uint32_t *tri_base,
-                                               SetupOut O) {
+                                               const float4 *xp, const uint32_t *counts, const uint32_t *chunk,
+                                               uint32_t *tri_base, SetupOut O) {
+    __shared__ uint32_t sh[17];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t t = base + threadIdx.x;
+        uint32_t ex;
+        block_excl_scan(t < total ? counts[t] : 0u, ex, sh);
+        uint32_t id = chunk[base >> 8] + ex;
         Rect r = {0, 0, 0, 0};
         int view = 0;
         if (t < total) {
+            tri_base[t] = id;
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
-            uint32_t id = tri_base[t];
             int du = wave_uniform(d);
             if (du >= 0) {
                 view = draws[du].view;
@@ -2522,14 +2548,17 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, fb);
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.clip_queue, a.clip_cap, a.status);
-        hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv, a.xv + a.total_xv,
-                           a.any_tangents ? a.xt : nullptr, a.tri_counts, a.clip_queue, a.clip_cap, a.clip_buf,
+                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
                            a.status);
+        hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv, a.xv + a.total_xv,
+                           a.any_tangents ? a.xt : nullptr, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
+                           a.clip_buf, a.status);
     }
-    // tri_counts[total_in] is a zero sentinel so tri_base[total_in] = total
-    hipError_t e = launch_scan(a.tri_counts, a.total_in + 1, a.tri_base, nullptr, a.scan_partials,
-                               &a.status->total_records, st);
+    // first post-clip id of every 256-triangle chunk; the total is also
+    // tri_base[total_in], the end of the last view's ids
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, a.tri_chunks, (a.total_in + 255u) / 256u,
+                       &a.status->total_records, a.tri_base + a.total_in);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     SetupOut so;
     so.geo = (SR_GLOBAL uint4 *)a.rec_geo; so.raster = (SR_GLOBAL float4 *)a.rec_raster;
@@ -2537,7 +2566,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     so.T = a.tile_size;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.tri_base, so);
+                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.tri_chunks, a.tri_base, so);
         hipLaunchKernelGGL(k_clip_emit, dim3(256), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.clip_queue,
                            a.clip_cap, a.clip_buf, a.tri_counts, a.tri_base, a.status, so);
     }

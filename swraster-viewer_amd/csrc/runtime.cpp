@@ -205,7 +205,7 @@ void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
-    DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
+    DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_chunks, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
                       &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf, &tr_count,
                       &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color};
     for (DevBuf *b : bufs) b->release();
@@ -235,7 +235,7 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
     k.n[8] = a.pair_cap; k.n[9] = a.clip_cap; k.n[10] = (uint64_t)a.persistent_blocks; k.n[11] = vbytes;
     k.n[12] = dbytes; k.n[13] = a.color_plane; k.n[14] = (uint64_t)a.has_translucent;
     k.n[15] = ((uint64_t)a.tr_cap << 32) | a.tr_slots | ((uint64_t)a.hiz << 31);
-    const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
+    const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_chunks, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
                           a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
                           status_host.ptr, a.tr_count, a.tr_start, a.tr_end, a.tr_list, a.tr_scratch,
@@ -283,6 +283,7 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 48))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
     if ((r = tri_counts.ensure(((size_t)total_in + 1) * 4))) return r;
+    if ((r = tri_chunks.ensure(((size_t)total_in / 256 + 2) * 4))) return r;
     if ((r = tri_base.ensure(((size_t)total_in + 1) * 4))) return r;
     if ((r = bin_count.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_start.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
@@ -327,6 +328,7 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.xv = (float4 *)xv.ptr;
         a.xt = any_tangents ? (float4 *)xt.ptr : nullptr;
         a.tri_counts = (uint32_t *)tri_counts.ptr;
+        a.tri_chunks = (uint32_t *)tri_chunks.ptr;
         a.tri_base = (uint32_t *)tri_base.ptr;
         a.bin_count = (uint32_t *)bin_count.ptr;
         a.bin_start = (uint32_t *)bin_start.ptr;

@@ -58,7 +58,7 @@ struct Pipeline {
     int tile_size = 16;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    DevBuf views, xv, xt, tri_counts, tri_base, bin_count, bin_start, bin_end, partials, list;
+    DevBuf views, xv, xt, tri_counts, tri_chunks, tri_base, bin_count, bin_start, bin_end, partials, list;
     DevBuf rec_geo, rec_raster, status, center, clip_queue, clip_buf;
     DevBuf tr_count, tr_start, tr_end, tr_list, tr_scratch, center_color;
     PinBuf stage, status_host;
