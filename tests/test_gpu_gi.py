@@ -76,3 +76,27 @@ def test_gi_lit_frame_parity(sr, orc, scenes, env):
     changed = int((plain["px"] != g["px"]).sum())
     print(f"[gi] {changed} pixels change with GI", flush=True)
     assert changed > 10000
+
+
+def test_device_gi_full_grid_properties(sr, scenes, env):
+    """The reference's 128^3 grid on C1 (device only; the host bake takes
+    seconds): its sun visibility is the sun bake's grid exactly, every
+    coefficient is finite, SH0 is non-negative (baked terms are clamped, and
+    the scene's SH0 is positive; the linear terms keep the scene's signs), the
+    sky visibility is a mean of fractions, and SH0's luminance stays at or
+    above 3/4 of the floor (0.005 of the scene SH): the floor's blend gives
+    cur + (floor - cur)^2 / floor >= 3/4 floor, and the blur averages."""
+    scene, _ = scenes.make_c1()
+    light, sh4 = _gi(sr, scene, env, 128, device=0)
+    d = sr.sr_voxel_grid()
+    sr.check(sr.lib().sr_prepare_voxel_grid_device(sr.C.byref(scene.desc), 128, 0, sr.C.byref(d)))
+    sun = np.ctypeslib.as_array(d.light_intensity, shape=(128 ** 3,)).copy().reshape(128, 128, 128)
+    sr.lib().sr_voxel_grid_free(sr.C.byref(d))
+    assert np.array_equal(light.view(np.uint32), sun.view(np.uint32))
+    assert np.all(np.isfinite(sh4))
+    assert np.all(sh4[..., 0, :3] >= 0.0)
+    sky = sh4[..., 1, 3]
+    assert np.all((sky >= 0.0) & (sky <= 1.0))
+    floor = np.asarray(env.sh, np.float32)[0] * np.float32(0.005)
+    lum = lambda c: 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
+    assert lum(sh4[..., 0, :3]).min() >= 0.75 * lum(floor) * (1 - 1e-5)
