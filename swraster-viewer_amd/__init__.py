@@ -15,6 +15,7 @@ is no CPU fallback: without the built library every entry point raises.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -633,8 +634,10 @@ class SceneData:
             light = np.ctypeslib.as_array(g.light_intensity, shape=(n,)).copy()
             sh4 = np.ctypeslib.as_array(g.sh4, shape=(n * 16,)).copy()
             dims, wmin, wmax = list(g.dims), list(g.world_min), list(g.world_max)
-            if gi_cache:
-                check(lib().sr_gi_cache_save(os.fsencode(gi_cache), C.byref(g)))
+            if gi_cache and lib().sr_gi_cache_save(os.fsencode(gi_cache), C.byref(g)) != 0:
+                # as the reference (src/main.rs:258-262): report and go on
+                print(f"Failed to write GI cache {gi_cache}: {lib().sr_last_error().decode(errors='replace')}",
+                      file=sys.stderr)
             lib().sr_voxel_grid_free(C.byref(g))
             if path:
                 os.makedirs(CACHE_DIR, exist_ok=True)
