@@ -967,6 +967,7 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
         int view = 0;
         if (t < total) {
             tri_base[t] = id;
+            if (t == total - 1) tri_base[total] = id + counts[t];
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
             if (du >= 0) {
@@ -2556,10 +2557,18 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
                            a.clip_buf, a.status);
     }
     // first post-clip id of every 256-triangle chunk; the total is also
-    // tri_base[total_in], the end of the last view's ids
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, a.tri_chunks, (a.total_in + 255u) / 256u,
-                       &a.status->total_records, a.tri_base + a.total_in);
-    hipError_t e = hipGetLastError();
+    // tri_base[total_in], the end of the last view's ids (k_setup writes it
+    // too).  One workgroup up to SCAN_SINGLE_MAX chunks, else the
+    // three-pass scan in place.
+    uint32_t nchunks = (a.total_in + 255u) / 256u;
+    hipError_t e;
+    if (nchunks <= SCAN_SINGLE_MAX) {
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, a.tri_chunks, nchunks, &a.status->total_records,
+                           a.tri_base + a.total_in);
+        e = hipGetLastError();
+    } else {
+        e = launch_scan(a.tri_chunks, nchunks, a.tri_chunks, nullptr, a.scan_partials, &a.status->total_records, st);
+    }
     if (e != hipSuccess) return e;
     SetupOut so;
     so.geo = (SR_GLOBAL uint4 *)a.rec_geo; so.raster = (SR_GLOBAL float4 *)a.rec_raster;
