@@ -295,6 +295,10 @@ __device__ __forceinline__ int classify_input_triangle(const DevDraw &D, const D
     return clip_classify(c);
 }
 
+// counts[t]: 1 inside the frustum, 0 outside, fan size | SR_COUNT_CLIPPED
+// for a triangle that crosses it (written by k_clip).
+#define SR_COUNT_CLIPPED 0x80000000u
+
 // chunk[c] gets the number of triangles of chunk c (input triangles 256c ..
 // 256c + 255) that lie inside the frustum; k_clip adds the fans of the
 // clipped ones, and k_scan_top turns the sums into each chunk's first id.
@@ -417,7 +421,7 @@ __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDra
         for (int i = 0; i < len; i++) buf[((size_t)e * CLIP_MAXV + i) * 16 + c] = P[i][c];
         if (c == 0) {
             uint32_t fan = len ? (uint32_t)(len - 2) : 0u;
-            counts[t] = fan;
+            counts[t] = fan | SR_COUNT_CLIPPED;
             if (fan) atomicAdd(&chunk[t >> 8], fan);
         }
         wave_lds_sync();
@@ -926,8 +930,8 @@ __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view,
     }
 }
 
-// Setup of input triangle t of draw D (index d), unless it crosses a frustum
-// plane (k_clip_emit sets those up).
+// Setup of input triangle t of draw D (index d), which k_clip_count found
+// inside the frustum (k_clip_emit sets up the fans of those that cross it).
 __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, const DevView *views, uint32_t t,
                                                      const float4 *xp, uint32_t id, const SetupOut &O) {
     const DevView &V = views[D.view];
@@ -939,13 +943,6 @@ __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, co
         float4 q = xp[D.xv_base + vi];    // setup reads clip positions only
         v[k].cx = q.x; v[k].cy = q.y; v[k].cz = q.z; v[k].cw = q.w;
     }
-    int cls = -1;
-    if (D.flags & SR_DRAW_CLIP) {
-        float c[3][4];
-        for (int k = 0; k < 3; k++) { c[k][0] = v[k].cx; c[k][1] = v[k].cy; c[k][2] = v[k].cz; c[k][3] = v[k].cw; }
-        cls = clip_classify(c);
-    }
-    if (cls != -1) return Rect{0, 0, 0, 0};
     return emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, S.materials[D.material].flags,
                          !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
 }
@@ -960,22 +957,22 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t t = base + threadIdx.x;
-        uint32_t ex;
-        block_excl_scan(t < total ? counts[t] : 0u, ex, sh);
+        uint32_t ex, c = t < total ? counts[t] : 0u;
+        block_excl_scan(c & ~SR_COUNT_CLIPPED, ex, sh);
         uint32_t id = chunk[base >> 8] + ex;
         Rect r = {0, 0, 0, 0};
         int view = 0;
         if (t < total) {
             tri_base[t] = id;
-            if (t == total - 1) tri_base[total] = id + counts[t];
+            if (t == total - 1) tri_base[total] = id + (c & ~SR_COUNT_CLIPPED);
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
             if (du >= 0) {
                 view = draws[du].view;
-                r = setup_input_triangle(draws[du], du, views, t, xp, id, O);
+                if (c == 1u) r = setup_input_triangle(draws[du], du, views, t, xp, id, O);
             } else {
                 view = draws[d].view;
-                r = setup_input_triangle(draws[d], d, views, t, xp, id, O);
+                if (c == 1u) r = setup_input_triangle(draws[d], d, views, t, xp, id, O);
             }
         }
         bin_rect_count(views, view, r, O.bin_count);
@@ -1000,7 +997,7 @@ __global__ void __launch_bounds__(256) k_clip_emit(const DevView *views, const D
         if (k < total) {
             uint32_t e = k / (uint32_t)(CLIP_MAXV - 2), f = k % (uint32_t)(CLIP_MAXV - 2);
             uint32_t t = queue[e];
-            if (f < counts[t]) {
+            if (f < (counts[t] & ~SR_COUNT_CLIPPED)) {
                 int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
                 const DevDraw &D = draws[d];
                 view = D.view;
