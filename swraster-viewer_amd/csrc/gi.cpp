@@ -85,7 +85,7 @@ struct Builder {
         nd.first = first;
         nd.count = count;
         nodes.push_back(nd);
-        if (count <= 4 || depth > 60) return id;
+        if (count <= 4 || depth >= GI_STACK - 4) return id;   // depth-first stacks hold depth + 1 entries
         float ext[3] = {chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]};
         int axis = ext[0] > ext[1] ? (ext[0] > ext[2] ? 0 : 2) : (ext[1] > ext[2] ? 1 : 2);
         auto key = [&](int t) { return box[6 * (size_t)t + axis] + box[6 * (size_t)t + 3 + axis]; };
@@ -260,9 +260,11 @@ int gi_bake_host(const GiHost &h, const std::vector<uint8_t> &active, const std:
         float rad[GI_SAMPLES * 3];
         uint32_t sky_hits = 0;
         bool ov = false;
+        GiLocalStack st;
         for (int s = 0; s < GI_SAMPLES; s++) {
             bool sky = false;
-            G3 r = gi_primary_ray(S, c, g3(dirs[3 * s], dirs[3 * s + 1], dirs[3 * s + 2]), h.primary_bias, h.params, sky, ov);
+            G3 r = gi_primary_ray(S, c, g3(dirs[3 * s], dirs[3 * s + 1], dirs[3 * s + 2]), h.primary_bias, h.params, sky,
+                                  ov, st);
             rad[3 * s] = r.x; rad[3 * s + 1] = r.y; rad[3 * s + 2] = r.z;
             sky_hits += sky ? 1u : 0u;
         }

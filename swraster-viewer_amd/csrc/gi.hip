@@ -52,10 +52,20 @@ __global__ void k_gi_init(GiDev G, size_t nvox) {
     }
 }
 
+// Traversal stack of one lane in LDS, entries strided by the wave width so
+// that the lanes of a wave touch consecutive words.
+struct GiLdsStack {
+    int32_t *base;
+    int n;
+    __device__ void push(int32_t v) { base[64 * n++] = v; }
+    __device__ int32_t pop() { return base[64 * --n]; }
+};
+
 // One wave per active voxel, four per workgroup.
 __global__ __launch_bounds__(256) void k_gi_voxels(GiDev G, uint32_t first) {
     __shared__ float rad[4][GI_SAMPLES * 3];
     __shared__ uint32_t sky[4][GI_SAMPLES];
+    __shared__ int32_t stacks[4][GI_STACK * 64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t a = first + blockIdx.x * 4 + w;
     const bool live = a < G.nactive;
@@ -70,7 +80,8 @@ __global__ __launch_bounds__(256) void k_gi_voxels(GiDev G, uint32_t first) {
         G3 c = g3(G.cmin[0] + (float)x * G.vs[0], G.cmin[1] + (float)y * G.vs[1], G.cmin[2] + (float)z * G.vs[2]);
         const float *d = G.dirs + ((size_t)set * GI_SAMPLES + lane) * 3;
         bool hit_sky = false, over = false;
-        G3 r = gi_primary_ray(G.S, c, g3(d[0], d[1], d[2]), G.primary_bias, G.P, hit_sky, over);
+        GiLdsStack st{&stacks[w][lane], 0};
+        G3 r = gi_primary_ray(G.S, c, g3(d[0], d[1], d[2]), G.primary_bias, G.P, hit_sky, over, st);
         rad[w][3 * lane] = r.x;
         rad[w][3 * lane + 1] = r.y;
         rad[w][3 * lane + 2] = r.z;

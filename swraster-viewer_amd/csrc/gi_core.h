@@ -61,8 +61,17 @@ struct GiScene {
 
 #define GI_SAMPLES 64
 #define GI_SETS 9
-#define GI_STACK 72
+#define GI_STACK 48          // traversal stack entries; the BVH is at most 44 levels deep
 #define GI_HIT_CAP 32
+
+// Traversal stack in a private array (host).  The device keeps its stacks
+// in LDS with the same interface and capacity, so both walk identically.
+struct GiLocalStack {
+    int32_t s[GI_STACK];
+    int n = 0;
+    GI_HD void push(int32_t v) { s[n++] = v; }
+    GI_HD int32_t pop() { return s[--n]; }
+};
 
 struct G3 { float x, y, z; };
 GI_HD G3 g3(float x, float y, float z) { return {x, y, z}; }
@@ -134,17 +143,17 @@ struct GiHit {
 
 // RayTracer::trace_nearest (src/raytracer.rs:135-172): depth-first, right
 // child first; boxes beyond the current nearest distance are skipped.
-GI_HD bool gi_trace_nearest(const GiScene &S, G3 o, G3 d, float tmin, GiHit &hit) {
+template <class Stack>
+GI_HD bool gi_trace_nearest(const GiScene &S, G3 o, G3 d, float tmin, GiHit &hit, Stack &st) {
     if (S.nnodes == 0) return false;
     G3 inv = g3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    int32_t stack[GI_STACK];
-    int sp = 0;
-    stack[sp++] = 0;
+    st.n = 0;
+    st.push(0);
     float best = INFINITY;
     int32_t best_tri = -1;
     float bu = 0.0f, bv = 0.0f;
-    while (sp > 0) {
-        const GiNode &n = S.nodes[stack[--sp]];
+    while (st.n > 0) {
+        const GiNode &n = S.nodes[st.pop()];
         if (!gi_hit_box(n, o, inv, best)) continue;
         if (n.left < 0) {
             for (int32_t k = n.first; k < n.first + n.count; k++) {
@@ -158,9 +167,9 @@ GI_HD bool gi_trace_nearest(const GiScene &S, G3 o, G3 d, float tmin, GiHit &hit
                     bv = v;
                 }
             }
-        } else if (sp + 2 <= GI_STACK) {
-            stack[sp++] = n.left;
-            stack[sp++] = n.right;
+        } else if (st.n + 2 <= GI_STACK) {
+            st.push(n.left);
+            st.push(n.right);
         }
     }
     if (best_tri < 0) return false;
@@ -173,17 +182,17 @@ GI_HD bool gi_trace_nearest(const GiScene &S, G3 o, G3 d, float tmin, GiHit &hit
 
 // RayTracer::trace_transmittance (src/raytracer.rs:174-207).  over is set if
 // more than GI_HIT_CAP translucent surfaces lie on the ray.
-GI_HD float gi_trace_transmittance(const GiScene &S, G3 o, G3 d, float tmin, bool &over) {
+template <class Stack>
+GI_HD float gi_trace_transmittance(const GiScene &S, G3 o, G3 d, float tmin, bool &over, Stack &st) {
     if (S.nnodes == 0) return 1.0f;
     G3 inv = g3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    int32_t stack[GI_STACK];
-    int sp = 0;
-    stack[sp++] = 0;
+    st.n = 0;
+    st.push(0);
     float ht[GI_HIT_CAP];
     int32_t hi[GI_HIT_CAP];
     int nh = 0;
-    while (sp > 0) {
-        const GiNode &n = S.nodes[stack[--sp]];
+    while (st.n > 0) {
+        const GiNode &n = S.nodes[st.pop()];
         if (!gi_hit_box(n, o, inv, INFINITY)) continue;
         if (n.left < 0) {
             for (int32_t k = n.first; k < n.first + n.count; k++) {
@@ -206,9 +215,9 @@ GI_HD float gi_trace_transmittance(const GiScene &S, G3 o, G3 d, float tmin, boo
                 ht[j] = t;
                 hi[j] = ti;
             }
-        } else if (sp + 2 <= GI_STACK) {
-            stack[sp++] = n.left;
-            stack[sp++] = n.right;
+        } else if (st.n + 2 <= GI_STACK) {
+            st.push(n.left);
+            st.push(n.right);
         }
     }
     float tr = 1.0f;
@@ -284,7 +293,9 @@ struct GiRayParams {
 };
 
 // sky_visibility_hemisphere4 (src/gi.rs:549-570)
-GI_HD float gi_sky_visibility(const GiScene &S, G3 origin, G3 normal, float dir_bias, float tmin, bool &over) {
+template <class Stack>
+GI_HD float gi_sky_visibility(const GiScene &S, G3 origin, G3 normal, float dir_bias, float tmin, bool &over,
+                              Stack &st) {
     G3 up = fabsf(normal.z) < 0.999f ? g3(0.0f, 0.0f, 1.0f) : g3(1.0f, 0.0f, 0.0f);
     G3 tangent = gnormalize(gcross(normal, up));
     G3 bitangent = gcross(normal, tangent);
@@ -294,13 +305,15 @@ GI_HD float gi_sky_visibility(const GiScene &S, G3 origin, G3 normal, float dir_
     for (int k = 0; k < 4; k++) {
         G3 dir = gnormalize(gadd(gadd(gscale(tangent, dl[k][0]), gscale(bitangent, dl[k][1])), gscale(normal, dl[k][2])));
         G3 o = gadd(origin, gscale(dir, dir_bias));
-        vis += gi_trace_transmittance(S, o, dir, tmin, over);
+        vis += gi_trace_transmittance(S, o, dir, tmin, over, st);
     }
     return vis * 0.25f;
 }
 
 // estimate_surface_bounce_radiance (src/gi.rs:450-547)
-GI_HD G3 gi_bounce(const GiScene &S, const GiHit &hit, G3 o, G3 incoming, const GiRayParams &P, bool &over) {
+template <class Stack>
+GI_HD G3 gi_bounce(const GiScene &S, const GiHit &hit, G3 o, G3 incoming, const GiRayParams &P, bool &over,
+                   Stack &st) {
     const GiTri &tri = S.tris[hit.tri];
     int32_t mi = (tri.material >= 0 && tri.material < S.nmat) ? tri.material : 0;
     const GiMat &M = S.mats[mi];
@@ -325,10 +338,10 @@ GI_HD G3 gi_bounce(const GiScene &S, const GiHit &hit, G3 o, G3 incoming, const 
     G3 e_sun = g3(0.0f, 0.0f, 0.0f);
     if (ndl > 0.0f) {
         G3 so = gadd(shading_origin, gscale(L, dir_bias));
-        float vis = gi_trace_transmittance(S, so, L, P.tmin, over);
+        float vis = gi_trace_transmittance(S, so, L, P.tmin, over, st);
         e_sun = gscale(g3(S.light_color[0], S.light_color[1], S.light_color[2]), ndl * vis);
     }
-    float sky = gi_sky_visibility(S, shading_origin, normal, dir_bias, P.tmin, over);
+    float sky = gi_sky_visibility(S, shading_origin, normal, dir_bias, P.tmin, over, st);
     G3 irr;
     irr.x = ((S.sh[0][0] + S.sh[1][0] * normal.y) + S.sh[2][0] * normal.z) + S.sh[3][0] * normal.x;
     irr.y = ((S.sh[0][1] + S.sh[1][1] * normal.y) + S.sh[2][1] * normal.z) + S.sh[3][1] * normal.x;
@@ -343,13 +356,14 @@ GI_HD G3 gi_bounce(const GiScene &S, const GiHit &hit, G3 o, G3 incoming, const 
 
 // Incident radiance along primary direction dir from the voxel centre
 // (one iteration of the ray loop of bake_voxel_gi, src/gi.rs:372-388).
+template <class Stack>
 GI_HD G3 gi_primary_ray(const GiScene &S, G3 center, G3 dir, float primary_bias, const GiRayParams &P, bool &sky,
-                        bool &over) {
+                        bool &over, Stack &st) {
     G3 o = gadd(center, gscale(dir, primary_bias));
     GiHit hit;
-    if (gi_trace_nearest(S, o, dir, P.tmin, hit)) {
+    if (gi_trace_nearest(S, o, dir, P.tmin, hit, st)) {
         sky = false;
-        return gi_bounce(S, hit, o, dir, P, over);
+        return gi_bounce(S, hit, o, dir, P, over, st);
     }
     sky = true;
     return gi_sky_radiance(S, dir);
