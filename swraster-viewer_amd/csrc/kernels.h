@@ -21,7 +21,7 @@ struct FrameArgs {
     // transient buffers
     float4 *xv;                   // total_xv clip positions, then 2 * total_xv attribute float4
     float4 *xt;                   // tangent float4 per transformed vertex (or null)
-    uint32_t *tri_counts;         // total_in (+1)
+    uint32_t *tri_counts;         // total_in
     uint32_t *tri_chunks;         // (total_in + 255) / 256 fan-count sums, then first ids
     uint32_t *tri_base;           // total_in (+1)
     uint32_t *bin_count;          // total_items

@@ -175,7 +175,7 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
 // k_frame_begin when there are no vertices).
 struct FrameBegin {
     DevStatus *status;
-    uint32_t *bin_count, *sentinel, *tr_count;
+    uint32_t *bin_count, *tr_count;
     uint32_t items, tr_slots;
 };
 
@@ -184,7 +184,6 @@ __device__ __forceinline__ void frame_begin(const FrameBegin &F) {
     if (i == 0) {
         uint32_t *w = reinterpret_cast<uint32_t *>(F.status);
         for (uint32_t k = 0; k < (uint32_t)(sizeof(DevStatus) / 4); k++) w[k] = 0u;
-        *F.sentinel = 0u;
     }
     for (uint32_t k = i; k < F.items; k += gridDim.x * blockDim.x) F.bin_count[k] = 0u;
     if (F.tr_count)
@@ -2537,7 +2536,6 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     fb.status = a.status;
     fb.bin_count = a.bin_count;
     fb.items = a.total_items;
-    fb.sentinel = a.tri_counts + a.total_in;
     fb.tr_count = a.has_translucent ? a.tr_count : nullptr;
     fb.tr_slots = a.tr_slots;
     if (a.total_xv > 0)
