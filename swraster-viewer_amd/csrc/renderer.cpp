@@ -40,6 +40,7 @@ struct sr_batch {
     int device = 0;
     Pipeline pipe;
     DevBuf ids, depth, rgba, lum;
+    DevBuf color;                // colour planes, only for batches with translucent primitives
     std::vector<uint64_t> fb_offset;
     std::vector<int> width, height;
     sr_batch_stats stats;
@@ -337,6 +338,7 @@ SR_API int sr_batch_destroy(sr_batch *b) {
     b->depth.release();
     b->rgba.release();
     b->lum.release();
+    b->color.release();
     delete b;
     return SR_OK;
 }
@@ -358,7 +360,7 @@ SR_API int sr_batch_render(sr_batch *b, const sr_view *views, int32_t n) {
             return fail(SR_ERR_INVALID_ARGUMENT, "view %d is invalid", i);
         if (v.scene->device != b->device) return fail(SR_ERR_INVALID_ARGUMENT, "view %d scene on another device", i);
         uint32_t tri0 = in_tri;
-        int rc = build_draws(v.scene, v.camera, i, in_tri, xv, draws, false);
+        int rc = build_draws(v.scene, v.camera, i, in_tri, xv, draws, true);
         if (rc) return rc;
         fill_view(hv[i], v.scene, v.camera, v.width, v.height, T, items, tri0, fb, lum,
                   v.exposure > 0.0f ? v.exposure : DEFAULT_EXPOSURE, 1u);
@@ -370,15 +372,30 @@ SR_API int sr_batch_render(sr_batch *b, const sr_view *views, int32_t n) {
         fb += (uint64_t)v.width * v.height;
         tangents = tangents || v.scene->any_tangents;
     }
+    // Translucent primitives blend over the opaque colour before the
+    // tonemap, so such a batch writes colour planes and blits each view
+    // afterwards instead of fusing the tonemap into the shading.
+    bool has_tr = false;
+    for (const DevDraw &d : draws) has_tr = has_tr || !(d.flags & SR_DRAW_OPAQUE);
+    if (has_tr)
+        for (DevView &v : hv) v.flags &= ~1u;
     hipSetDevice(b->device);
     int rc;
     if ((rc = b->ids.ensure(fb * 4))) return rc;
     if ((rc = b->depth.ensure(fb * 4))) return rc;
     if ((rc = b->rgba.ensure(fb * 4))) return rc;
     if ((rc = b->lum.ensure((size_t)lum * 4))) return rc;
-    rc = b->pipe.run(hv, draws, xv, in_tri, items, tangents, (uint32_t *)b->ids.ptr, (float *)b->depth.ptr, nullptr, 0,
-                     (uint32_t *)b->rgba.ptr, (float *)b->lum.ptr, b->persistent, true, b->use_graph);
+    if (has_tr && (rc = b->color.ensure(fb * 12))) return rc;
+    rc = b->pipe.run(hv, draws, xv, in_tri, items, tangents, (uint32_t *)b->ids.ptr, (float *)b->depth.ptr,
+                     has_tr ? (float *)b->color.ptr : nullptr, has_tr ? fb : 0, (uint32_t *)b->rgba.ptr,
+                     (float *)b->lum.ptr, b->persistent, true, b->use_graph);
     if (rc) return rc;
+    if (has_tr) {
+        for (int i = 0; i < n; i++)
+            HIP_TRY(launch_blit((const float *)b->color.ptr + b->fb_offset[i], fb, b->width[i], b->height[i],
+                                hv[i].exposure, (uint32_t *)b->rgba.ptr + b->fb_offset[i], b->pipe.stream));
+        HIP_TRY(hipStreamSynchronize(b->pipe.stream));
+    }
     b->stats.input_triangles = in_tri;
     b->stats.setup_triangles = b->pipe.last_records;
     b->stats.binned_pairs = b->pipe.last_pairs;

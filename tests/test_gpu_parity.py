@@ -252,6 +252,34 @@ def test_batch_matches_plain(sr, scenes, c1):
         assert np.array_equal(b.read_pixels(i), p["px"])
 
 
+def test_batch_with_translucent_views_matches_plain(sr, scenes, env, c1):
+    """A batch that mixes translucent and opaque scenes at several sizes
+    (colour planes, back-to-front blending per view, then one blit per view)
+    gives each view the plain renderer's visibility buffer and pixels."""
+    _, spec1, gc1 = c1
+    tscene, tspec = _translucent_c1(sr, scenes, 640, 360)
+    tscene.prepare(env, cache_key=None, grid=32)
+    gt = sr.Scene(tscene)
+    views, plain = [], []
+    for i, (g, spec, w, h) in enumerate([(gt, tspec, 640, 360), (gc1, spec1, 320, 240), (gt, tspec, 256, 200),
+                                         (gt, tspec, 640, 336)]):
+        s = dict(spec)
+        s["yaw"] = spec.get("yaw", 0.0) + 0.05 * i
+        cam = scenes.make_camera(s, w, h)
+        views.append((g, cam, w, h, 2.0))
+        plain.append(render_gpu(sr, g, cam, w, h))
+    for use_graph in (False, True):
+        b = sr.Batch()
+        b.configure(use_graph=use_graph)
+        for _ in range(2 if use_graph else 1):
+            b.render(views)
+            for i, p in enumerate(plain):
+                ids, depth = b.read_vbuffer(i)
+                assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"])), f"view {i}"
+                assert np.array_equal(b.read_pixels(i), p["px"]), f"view {i} pixels"
+        b.close()
+
+
 def test_batch_graph_replays_match_eager(sr, scenes, c1):
     """The batch's launch sequence captured as a hipGraph, replayed three
     times with eager frames and synchronisations in between: every replay
@@ -379,18 +407,10 @@ def _world_quad(x0, x1, y0, y1, z):
             "indices": np.array([0, 2, 1, 0, 3, 2], np.uint32)}
 
 
-@pytest.mark.parametrize("size", [(640, 360), (640, 336)])
-def test_translucent_materials(sr, orc, scenes, env, size):
-    """Translucent primitives (src/tilerasterizer.rs:92-101, src/shader.rs:
-    66-76): per 64-px tile lists sorted back to front (src/bumpqueue.rs:
-    155-200) and blended over the opaque frame.  C1 with a third of the
-    spheres translucent (one through a transmission texture), two
-    camera-facing quads at one depth that overlap (tied sort keys, so the
-    quicksort's own order decides the blend), and a band wider than the
-    frustum (clipped; a width of 640 = 10 x 64 lists it a second time in the
-    first tile of the next row).  336 rows leave the last tile row's centre
-    quads off screen."""
-    W, H = size
+def _translucent_c1(sr, scenes, W, H):
+    """C1 with a third of the spheres translucent (one through a
+    transmission texture), two overlapping camera-facing glass quads at one
+    depth and a glass band wider than the frustum."""
     scene, spec = scenes.make_c1(width=W, height=H)
     for i, m in enumerate(scene.materials):
         if i % 3 == 1:
@@ -414,6 +434,22 @@ def test_translucent_materials(sr, orc, scenes, env, size):
     add(_world_quad(-0.5, 2.0, -1.0, 1.5, 8.0), glass((0.2, 0.4, 0.9), 0.5))
     add(_world_quad(-40.0, 40.0, -0.75, 0.25, 6.0), glass((0.7, 0.7, 0.3), 0.35))
     scene.rebuild()
+    return scene, spec
+
+
+@pytest.mark.parametrize("size", [(640, 360), (640, 336)])
+def test_translucent_materials(sr, orc, scenes, env, size):
+    """Translucent primitives (src/tilerasterizer.rs:92-101, src/shader.rs:
+    66-76): per 64-px tile lists sorted back to front (src/bumpqueue.rs:
+    155-200) and blended over the opaque frame.  C1 with a third of the
+    spheres translucent (one through a transmission texture), two
+    camera-facing quads at one depth that overlap (tied sort keys, so the
+    quicksort's own order decides the blend), and a band wider than the
+    frustum (clipped; a width of 640 = 10 x 64 lists it a second time in the
+    first tile of the next row).  336 rows leave the last tile row's centre
+    quads off screen."""
+    W, H = size
+    scene, spec = _translucent_c1(sr, scenes, W, H)
     scene.prepare(env, cache_key=None, grid=32)
     cam = scenes.make_camera(spec)
     g = render_gpu(sr, sr.Scene(scene), cam, W, H)
