@@ -90,6 +90,7 @@ struct sr_scene {
     std::vector<sr::HostMesh> meshes;
     std::vector<sr::HostPrim> prims;
     bool any_tangents = false;
+    bool any_translucent = false;     // a node draws a translucent primitive
     uint64_t triangles = 0, vertices = 0, device_bytes = 0;
 };
 

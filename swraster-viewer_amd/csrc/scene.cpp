@@ -338,6 +338,7 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
         n.bs_world.r = 0.0f;
         if (n.mesh >= 0)
             for (int pi : s->meshes[n.mesh].prims) grow(n.bs_world, xform_sphere(n.transform, s->prims[pi].bs));
+        if (n.mesh >= 0 && !s->meshes[n.mesh].translucent.empty()) s->any_translucent = true;
     }
     *out = s;
     return SR_OK;

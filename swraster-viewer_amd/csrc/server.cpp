@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "common.h"
+#include "kernels.h"
 #include "runtime.h"
 #include "swraster.h"
 
@@ -114,6 +115,10 @@ struct Slot {
     bool tangents = false;
     uint32_t *ids = nullptr, *rgba = nullptr;
     float *depth = nullptr, *lum = nullptr;
+    float *color = nullptr;         // colour planes when the batch has translucent primitives
+    uint64_t plane = 0;             // pixels of the batch (colour plane stride)
+    std::vector<uint64_t> px_off;
+    std::vector<int> width, height;
 };
 
 }  // namespace
@@ -172,10 +177,23 @@ void sr_server::complete(const std::vector<uint64_t> &ids, int rc) {
     }
 }
 
+// Visibility buffer, RGBA8 and tile luminance of a request; a batch with a
+// translucent scene also holds colour planes for all its views.
 static uint64_t fb_bytes(const sr_request &r) {
     uint64_t px = (uint64_t)r.width * r.height;
     uint64_t ref = (uint64_t)((r.width + 63) / 64) * ((r.height + 63) / 64);
     return px * 12 + ref * 4;
+}
+static uint64_t color_bytes(const sr_request &r) { return (uint64_t)r.width * r.height * 12; }
+
+// Blend results to RGBA8: one blit per view from the batch's colour planes.
+static int blit_views(Slot &s) {
+    for (size_t i = 0; i < s.hv.size(); i++) {
+        hipError_t e = launch_blit(s.color + s.px_off[i], s.plane, s.width[i], s.height[i], s.hv[i].exposure,
+                                   s.rgba + s.px_off[i], s.pipe.stream);
+        if (e != hipSuccess) return fail(SR_ERR_HIP, "blit: %s", hipGetErrorString(e));
+    }
+    return SR_OK;
 }
 
 static uint64_t bin_estimate(const sr_scene *s, const sr_request &r) {
@@ -187,23 +205,39 @@ static uint64_t bin_estimate(const sr_scene *s, const sr_request &r) {
 bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     // (locked) take requests FIFO while they fit the view limit and the budget
     std::vector<uint64_t> take;
-    uint64_t bytes = 0, bins = 0;
+    uint64_t bytes = 0, bins = 0, cbytes = 0;
+    bool tr = false;
     for (uint64_t id : pending) {
         const Request &q = reqs[id];
         uint64_t b = fb_bytes(q.r), e = bin_estimate(scenes[q.r.scene], q.r);
+        bool qtr = tr || scenes[q.r.scene]->any_translucent;
+        uint64_t c = qtr ? cbytes + color_bytes(q.r) : 0;
         if ((int)take.size() >= max_views) break;
-        if (!take.empty() && arena.used + bytes + b + bins + e > budget) break;
+        if (!take.empty() && arena.used + bytes + b + c + bins + e > budget) break;
         take.push_back(id);
         bytes += b;
         bins += e;
+        cbytes += color_bytes(q.r);
+        tr = qtr;
     }
     if (take.empty()) return false;
+    bool with_color = false;       // the region holds colour planes
+    auto batch_bytes = [&]() {
+        uint64_t b = 0, c = 0;
+        with_color = false;
+        for (uint64_t id : take) {
+            b += fb_bytes(reqs[id].r);
+            c += color_bytes(reqs[id].r);
+            with_color = with_color || scenes[reqs[id].r.scene]->any_translucent;
+        }
+        return b + (with_color ? c : 0);
+    };
+    bytes = batch_bytes();
     uint64_t off = 0;
     while (!arena.alloc(bytes, off)) {
         if (take.size() == 1) return false;     // wait for releases
         take.pop_back();
-        bytes = 0;
-        for (uint64_t id : take) bytes += fb_bytes(reqs[id].r);
+        bytes = batch_bytes();
     }
     for (size_t i = 0; i < take.size(); i++) pending.pop_front();
     int rg;
@@ -243,7 +277,7 @@ bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
     for (int i = 0; i < n; i++) {
         uint32_t tri0 = s.in_tri, xv0 = s.xv;
         size_t nd = s.draws.size();
-        int rc = build_draws(rs[i], rq[i].camera, i, s.in_tri, s.xv, s.draws, false);
+        int rc = build_draws(rs[i], rq[i].camera, i, s.in_tri, s.xv, s.draws, true);
         if (rc) {                      // the view renders nothing; the request reports rc
             st[i] = rc;
             s.draws.resize(nd);
@@ -258,12 +292,29 @@ bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
         px_total += (uint64_t)rq[i].width * rq[i].height;
         s.tangents = s.tangents || rs[i]->any_tangents;
     }
+    // translucent primitives blend over the opaque colour before the
+    // tonemap: such a batch writes colour planes and blits each view after
+    bool has_tr = false;
+    for (const DevDraw &d : s.draws) has_tr = has_tr || !(d.flags & SR_DRAW_OPAQUE);
+    if (has_tr)
+        for (DevView &v : s.hv) v.flags &= ~1u;
     s.ids = (uint32_t *)base;
     s.depth = (float *)(base + px_total * 4);
     s.rgba = (uint32_t *)(base + px_total * 8);
     s.lum = (float *)(base + px_total * 12);
-    int rc = s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth, nullptr, 0, s.rgba, s.lum,
-                        persistent, false);
+    s.color = has_tr ? (float *)(base + px_total * 12 + (uint64_t)lum_total * 4) : nullptr;
+    s.plane = px_total;
+    s.px_off = px_off;
+    s.width.resize(n);
+    s.height.resize(n);
+    for (int i = 0; i < n; i++) {
+        s.width[i] = rq[i].width;
+        s.height[i] = rq[i].height;
+    }
+    int rc = has_tr && !with_color ? fail(SR_ERR_UNSUPPORTED, "translucent draws in a batch without colour planes")
+                                   : s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth,
+                                                s.color, has_tr ? px_total : 0, s.rgba, s.lum, persistent, false);
+    if (rc == SR_OK && has_tr) rc = blit_views(s);
     if (rc == SR_OK && hipEventRecord(s.done, s.pipe.stream) != hipSuccess) rc = fail(SR_ERR_HIP, "event record");
     if (rc != SR_OK) hipStreamSynchronize(s.pipe.stream);
     lk.lock();
@@ -291,11 +342,16 @@ void sr_server::retire(Slot &s, std::unique_lock<std::mutex> &lk) {
     // synchronously (the pipeline grows its buffers and repeats the frame)
     const DevStatus &st = *(const DevStatus *)s.pipe.status_host.ptr;
     int rc = SR_OK;
+    bool tr_over = s.color && (st.overflow_tr || st.total_tr_pairs > s.pipe.tr_cap);
     if (st.overflow_records || st.overflow_pairs || st.overflow_clip || st.total_records > s.pipe.rec_cap ||
-        st.total_pairs > s.pipe.pair_cap || st.clip_overflow) {
+        st.total_pairs > s.pipe.pair_cap || st.clip_overflow || tr_over) {
         lk.unlock();
-        rc = s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth, nullptr, 0, s.rgba,
-                        s.lum, persistent, true);
+        rc = s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth, s.color,
+                        s.color ? s.plane : 0, s.rgba, s.lum, persistent, true);
+        if (rc == SR_OK && s.color) {
+            rc = blit_views(s);
+            if (rc == SR_OK && hipStreamSynchronize(s.pipe.stream) != hipSuccess) rc = fail(SR_ERR_HIP, "sync");
+        }
         lk.lock();
     }
     complete(s.reqs, rc);
@@ -403,9 +459,6 @@ SR_API int sr_server_destroy(sr_server *s) {
 SR_API int sr_server_add_scene(sr_server *s, const sr_scene *scene, int32_t *index) {
     if (!s || !scene || !index) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
     if (scene->device != s->device) return fail(SR_ERR_INVALID_ARGUMENT, "scene on another device");
-    for (const HostMesh &m : scene->meshes)
-        if (!m.translucent.empty())
-            return fail(SR_ERR_UNSUPPORTED, "translucent primitives are not served by the GPU path yet");
     std::lock_guard<std::mutex> g(s->mu);
     *index = (int32_t)s->scenes.size();
     s->scenes.push_back(scene);

@@ -371,6 +371,40 @@ def test_server_same_alone_and_in_the_mix(sr, scenes, c1, env):
     srv.close()
 
 
+def test_server_translucent_requests(sr, scenes, c1, env):
+    """Requests on a translucent scene mixed with opaque ones in the server's
+    batches (colour planes, per-view blending and blits) equal the plain
+    renderer's frames."""
+    _, spec, gscene = c1
+    tscene, tspec = _translucent_c1(sr, scenes, 640, 360)
+    tscene.prepare(env, cache_key=None, grid=32)
+    gt = sr.Scene(tscene)
+    srv = sr.Server(admission_bytes=256 << 20, max_batch_views=4, streams=2)
+    idx = [srv.add_scene(gscene), srv.add_scene(gt)]
+    reqs = []
+    for i, (w, h) in enumerate([(640, 360), (320, 240), (256, 144), (640, 336), (160, 90), (333, 202)]):
+        sp = dict(tspec if i % 2 == 0 else spec)
+        sp["yaw"] = sp["yaw"] + 0.03 * i
+        reqs.append((1 if i % 2 == 0 else 0, scenes.make_camera(sp, w, h), w, h))
+    rids = [srv.submit(idx[k], cam, w, h, tag=i) for i, (k, cam, w, h) in enumerate(reqs)]
+    srv.drain()
+    done = srv.poll(64)
+    assert all(d["status"] == 0 for d in done), [d["status"] for d in done]
+    for i, rid in enumerate(rids):
+        (ids, depth), px = srv.read_vbuffer(rid), srv.read_pixels(rid)
+        k, cam, w, h = reqs[i]
+        r = sr.Renderer(w, h)
+        r.render_scene([gscene, gt][k], cam)
+        pids, pdepth = r.read_vbuffer()
+        r.update_auto_exposure(0.0)
+        ppx = r.blit_to_buffer()
+        r.close()
+        assert np.array_equal(ids, pids) and np.array_equal(bits(depth), bits(pdepth)), f"request {i}: vbuffer"
+        assert np.array_equal(px[: h & ~1], ppx[: h & ~1]), f"request {i}: pixels"
+        srv.release(rid)
+    srv.close()
+
+
 def test_alpha_tested_materials(sr, orc, scenes, env):
     """MASK materials (src/shader.rs:40-61, :311-329): the base texture's
     nearest-sampled alpha against the cutoff decides coverage; alpha from
