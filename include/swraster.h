@@ -409,6 +409,13 @@ SR_API int sr_renderer_blit_to_buffer(sr_renderer *renderer, uint32_t *pixels,
                                       size_t width, size_t height);
 /* The same blit into device memory (width*height u32); nothing crosses PCIe. */
 SR_API int sr_renderer_blit_to_device(sr_renderer *renderer, void *device_pixels);
+/* App::render_frame's presentation copy (src/main.rs:574-590: each packed
+ * pixel unpacked into the bytes r, g, b, a of the pixels frame) fused into the
+ * blit: frame receives width*height*4 bytes in that order.  As with the blit,
+ * rows come in pairs, so an odd last row is not written. */
+SR_API int sr_renderer_present(sr_renderer *renderer, uint8_t *frame, size_t width, size_t height);
+/* The same into device memory (width*height*4 bytes). */
+SR_API int sr_renderer_present_to_device(sr_renderer *renderer, void *device_frame);
 SR_API float sr_renderer_exposure(const sr_renderer *renderer);
 SR_API int sr_renderer_set_exposure(sr_renderer *renderer, float exposure);
 /* Visibility buffer of the last frame: canonical triangle id (0xFFFFFFFF =

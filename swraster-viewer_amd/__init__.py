@@ -199,6 +199,8 @@ _SIGS = {
 "sr_renderer_update_auto_exposure_device": (C.c_int, [_P, C.c_float]),
     "sr_renderer_blit_to_buffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t]),
     "sr_renderer_blit_to_device": (C.c_int, [_P, _P]),
+    "sr_renderer_present": (C.c_int, [_P, C.POINTER(C.c_uint8), C.c_size_t, C.c_size_t]),
+    "sr_renderer_present_to_device": (C.c_int, [_P, _P]),
     "sr_renderer_exposure": (C.c_float, [_P]),
     "sr_renderer_set_exposure": (C.c_int, [_P, C.c_float]),
     "sr_renderer_read_vbuffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
@@ -717,6 +719,23 @@ class Renderer:
             buffer = np.zeros((self.height, self.width), np.uint32)
         check(lib().sr_renderer_blit_to_buffer(self.handle, _uptr(buffer), self.width, self.height))
         return buffer
+
+    def present(self, frame=None):
+        """The presented frame as (height, width, 4) bytes r, g, b, a: the
+        blit and App::render_frame's unpack into the pixels frame
+        (src/main.rs:574-590) in one device pass.  Rows come in pairs as in
+        the blit, so an odd last row keeps what the frame held."""
+        if frame is None:
+            frame = np.zeros((self.height, self.width, 4), np.uint8)
+        if frame.shape != (self.height, self.width, 4) or frame.dtype != np.uint8 or not frame.flags.c_contiguous:
+            raise ValueError(f"frame must be a contiguous ({self.height}, {self.width}, 4) uint8 array")
+        check(lib().sr_renderer_present(self.handle, frame.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                        self.width, self.height))
+        return frame
+
+    def present_to_device(self, device_ptr):
+        """The presented frame into device memory (width*height*4 bytes)."""
+        check(lib().sr_renderer_present_to_device(self.handle, C.c_void_p(int(device_ptr))))
 
     def read_vbuffer(self):
         ids = np.empty((self.height, self.width), np.uint32)

@@ -68,9 +68,10 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t stream);
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
                        uint32_t *total, hipStream_t stream);
 
-// Exposure + tonemap + quantize + pack of one view's linear colour planes.
+// Exposure + tonemap + quantize + pack of one view's linear colour planes;
+// frame_bytes stores each pixel as the bytes r, g, b, a.
 hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
-                       hipStream_t st, const float *exposure_dev = nullptr);
+                       hipStream_t st, const float *exposure_dev = nullptr, bool frame_bytes = false);
 // Most 64-px reference tiles the device exposure step sorts (8K UHD has 8160).
 #define SR_EXPOSURE_MAX 8192
 // state = {ev, target, exposure} on the device (k_exposure).

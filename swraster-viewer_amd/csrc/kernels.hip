@@ -2430,10 +2430,13 @@ __global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
 }
 
 // ---------------------------------------------------------------------------
-// k_blit: exposure, tonemap, quantize, pack (src/renderer.rs:293-355)
+// k_blit: exposure, tonemap, quantize, pack (src/renderer.rs:293-355).  With
+// frame_bytes the word is stored byte-swapped, so memory holds r, g, b, a per
+// pixel: the presented frame (rgba8_unpack_u8 into the pixel bytes,
+// src/main.rs:574-590) without a host pass.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_blit(const float *color, uint64_t plane, int W, int H, float exposure,
-                                              const float *exposure_dev, uint32_t *out) {
+                                              const float *exposure_dev, uint32_t *out, int frame_bytes) {
     if (exposure_dev) exposure = exposure_dev[2];
     uint64_t n = (uint64_t)W * (uint64_t)(H & ~1);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -2443,7 +2446,8 @@ __global__ void __launch_bounds__(256) k_blit(const float *color, uint64_t plane
             float x = color[ch * plane + i] * exposure;
             c[ch] = (x / (x + k)) * (1.0f + k);
         }
-        out[i] = (sat_u8(c[0] * 255.0f) << 24) | (sat_u8(c[1] * 255.0f) << 16) | (sat_u8(c[2] * 255.0f) << 8) | 0xFFu;
+        uint32_t v = (sat_u8(c[0] * 255.0f) << 24) | (sat_u8(c[1] * 255.0f) << 16) | (sat_u8(c[2] * 255.0f) << 8) | 0xFFu;
+        out[i] = frame_bytes ? __builtin_bswap32(v) : v;
     }
 }
 
@@ -2638,10 +2642,10 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_blit(const float *color, uint64_t plane, int32_t W, int32_t H, float exposure, uint32_t *out,
-                       hipStream_t st, const float *exposure_dev) {
+                       hipStream_t st, const float *exposure_dev, bool frame_bytes) {
     uint64_t n = (uint64_t)W * (uint64_t)(H & ~1);
     hipLaunchKernelGGL(k_blit, dim3(grid_for(n, 256, 256 * 32)), dim3(256), 0, st, color, plane, W, H, exposure,
-                       exposure_dev, out);
+                       exposure_dev, out, frame_bytes ? 1 : 0);
     return hipGetLastError();
 }
 

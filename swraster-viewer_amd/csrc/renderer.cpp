@@ -249,6 +249,38 @@ SR_API int sr_renderer_blit_to_buffer(sr_renderer *r, uint32_t *pixels, size_t w
     return SR_OK;
 }
 
+// The presented frame: the blit with each pixel stored as the bytes r, g, b,
+// a (App::render_frame's copy into the pixels frame, src/main.rs:574-590).
+static int present(sr_renderer *r, uint32_t *dst) {
+    HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure, dst,
+                        r->pipe.stream, r->expo_on_device ? (const float *)r->expo.ptr : nullptr, true));
+    return SR_OK;
+}
+
+SR_API int sr_renderer_present(sr_renderer *r, uint8_t *frame, size_t width, size_t height) {
+    if (!r || !frame) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if ((int)width != r->W || (int)height != r->H)
+        return fail(SR_ERR_INVALID_ARGUMENT, "frame is %zux%zu, renderer is %dx%d", width, height, r->W, r->H);
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    int rc = present(r, (uint32_t *)r->rgba.ptr);
+    if (rc) return rc;
+    size_t rows = (size_t)(r->H & ~1);
+    HIP_TRY(hipMemcpyAsync(frame, r->rgba.ptr, rows * (size_t)r->W * 4, hipMemcpyDeviceToHost, r->pipe.stream));
+    HIP_TRY(hipStreamSynchronize(r->pipe.stream));
+    return SR_OK;
+}
+
+SR_API int sr_renderer_present_to_device(sr_renderer *r, void *device_frame) {
+    if (!r || !device_frame) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    int rc = present(r, (uint32_t *)device_frame);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(r->pipe.stream));
+    return SR_OK;
+}
+
 SR_API int sr_renderer_read_vbuffer(sr_renderer *r, uint32_t *tri_id, float *depth) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
