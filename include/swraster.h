@@ -387,8 +387,15 @@ SR_API int sr_scene_info(const sr_scene *scene, uint64_t *triangles, uint64_t *v
 SR_API int sr_renderer_create(int32_t width, int32_t height, const sr_renderer_opts *opts,
                               sr_renderer **out);
 SR_API int sr_renderer_destroy(sr_renderer *renderer);
+/* Waits for the queued frame and checks it (see render_scene). */
+SR_API int sr_renderer_synchronize(sr_renderer *renderer);
 /* Renderer::render_scene: transform, clip, bin, raster and shade one frame.
- * Returns when the frame is complete on the device. */
+ * The frame is queued on the renderer's stream.  The first call that reads
+ * it (blit, present, an exposure step that meters it, vbuffer / colour /
+ * luminance reads, stats, sr_renderer_synchronize, the next render_scene)
+ * waits for it, checks it, reruns it once with grown buffers if a capacity
+ * overflowed, and reports any error the frame met.  Until then the scene is
+ * in use: sr_scene_destroy returns SR_ERR_BUSY. */
 SR_API int sr_renderer_render_scene(sr_renderer *renderer, const sr_scene *scene,
                                     const sr_camera *camera);
 /* Renderer::update_auto_exposure: trimmed mean of the per-64px-tile centre

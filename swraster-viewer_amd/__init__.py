@@ -200,6 +200,7 @@ _SIGS = {
     "sr_renderer_blit_to_buffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t]),
     "sr_renderer_blit_to_device": (C.c_int, [_P, _P]),
     "sr_renderer_present": (C.c_int, [_P, C.POINTER(C.c_uint8), C.c_size_t, C.c_size_t]),
+    "sr_renderer_synchronize": (C.c_int, [_P]),
     "sr_renderer_present_to_device": (C.c_int, [_P, _P]),
     "sr_renderer_exposure": (C.c_float, [_P]),
     "sr_renderer_set_exposure": (C.c_int, [_P, C.c_float]),
@@ -675,8 +676,10 @@ class Scene:
         return {"triangles": t.value, "vertices": v.value, "device_bytes": b.value}
 
     def close(self):
+        """Frees the scene; raises SwrasterError (SR_ERR_BUSY) while a
+        renderer's queued frame still uses it."""
         if self.handle:
-            lib().sr_scene_destroy(self.handle)
+            check(lib().sr_scene_destroy(self.handle))
             self.handle = C.c_void_p()
 
     def __del__(self):
@@ -699,6 +702,11 @@ class Renderer:
 
     def render_scene(self, scene, camera):
         check(lib().sr_renderer_render_scene(self.handle, scene.handle, C.byref(camera.c)))
+        self._scene = scene      # the queued frame uses it until the frame is read
+
+    def synchronize(self):
+        """Waits for the queued frame and checks it (render_scene queues it)."""
+        check(lib().sr_renderer_synchronize(self.handle))
 
     def update_auto_exposure(self, delta_time):
         check(lib().sr_renderer_update_auto_exposure(self.handle, float(delta_time)))

@@ -34,6 +34,18 @@ struct sr_renderer {
     float auto_exposure = DEFAULT_EXPOSURE, auto_exposure_target = DEFAULT_EXPOSURE, auto_exposure_ev = 1.0f;
     sr_frame_stats stats;
     bool has_frame = false;
+    // The last frame is queued and not yet checked for overflow; what a rerun
+    // needs is kept until then (the scene counts it in pending_frames).
+    bool pending = false;
+    const sr_scene *p_scene = nullptr;
+    std::vector<DevView> p_views;
+    std::vector<DevDraw> p_draws;
+    uint32_t p_xv = 0, p_in = 0, p_items = 0;
+    bool p_tangents = false, p_translucent = false;
+    // a device exposure step queued behind the pending frame: its arguments,
+    // and its state before the step saved at expo + 4 floats
+    bool p_expo = false, pe_use_host = false;
+    float pe_dt = 0.0f, pe_ev = 0.0f, pe_target = 0.0f;
 };
 
 struct sr_batch {
@@ -82,7 +94,7 @@ SR_API int sr_renderer_create(int32_t width, int32_t height, const sr_renderer_o
     if (!rc) rc = r->color.ensure(np * 12);
     if (!rc) rc = r->rgba.ensure(np * 4);
     if (!rc) rc = r->lum.ensure((size_t)r->ref_tiles * 4);
-    if (!rc) rc = r->expo.ensure(16);
+    if (!rc) rc = r->expo.ensure(32);
     if (rc) {
         r->pipe.destroy();
         delete r;
@@ -95,8 +107,70 @@ SR_API int sr_renderer_create(int32_t width, int32_t height, const sr_renderer_o
     return SR_OK;
 }
 
+// Stage times and counts of the frame just checked.
+static void fill_stats(sr_renderer *r) {
+    r->stats.input_triangles = r->p_in;
+    r->stats.setup_triangles = r->pipe.last_records;
+    r->stats.binned_pairs = r->pipe.last_pairs;
+    r->stats.tiles = r->p_items;
+    r->stats.draws = (uint32_t)r->p_draws.size();
+    r->stats.depth_rejection = r->pipe.last_hiz ? 1u : 0u;
+    r->stats.ms_setup = ev_ms(r->pipe.ev[0], r->pipe.ev[1]);
+    r->stats.ms_bin = ev_ms(r->pipe.ev[1], r->pipe.ev[2]);
+    r->stats.ms_raster_shade = ev_ms(r->pipe.ev[2], r->pipe.ev[3]);
+    r->stats.ms_raster = ev_ms(r->pipe.ev[2], r->pipe.ev[4]);
+    r->stats.ms_shade = ev_ms(r->pipe.ev[4], r->pipe.ev[3]);
+    r->stats.ms_total = ev_ms(r->pipe.ev[0], r->pipe.ev[3]);
+}
+
+static hipError_t queue_exposure(sr_renderer *r) {
+    return launch_exposure((const float *)r->lum.ptr, (uint32_t)r->ref_tiles, r->pe_dt, r->pe_use_host, r->pe_ev,
+                           r->pe_target, (float *)r->expo.ptr, r->pipe.stream);
+}
+
+// Completes the queued frame: waits for the stream, checks the frame's
+// status and, when a capacity overflowed, runs the frame again with grown
+// buffers (replaying a device exposure step queued behind it).  reran tells
+// the caller to queue again what it queued behind the frame.
+static int finish(sr_renderer *r, bool *reran = nullptr) {
+    if (reran) *reran = false;
+    if (!r->pending) return SR_OK;
+    r->pending = false;
+    hipSetDevice(r->device);
+    int rc = SR_OK;
+    bool again = false;
+    if (hipStreamSynchronize(r->pipe.stream) != hipSuccess) rc = fail(SR_ERR_HIP, "frame: %s", hipGetErrorString(hipGetLastError()));
+    if (!rc) rc = r->pipe.settle(r->p_translucent, again);
+    if (!rc && again) {
+        if (r->p_expo && hipMemcpyAsync(r->expo.ptr, (float *)r->expo.ptr + 4, 12, hipMemcpyDeviceToDevice,
+                                        r->pipe.stream) != hipSuccess)
+            rc = fail(SR_ERR_HIP, "exposure state restore");
+        if (!rc)
+            rc = r->pipe.run(r->p_views, r->p_draws, r->p_xv, r->p_in, r->p_items, r->p_tangents,
+                             (uint32_t *)r->ids.ptr, (float *)r->depth.ptr, (float *)r->color.ptr,
+                             (uint64_t)r->W * r->H, nullptr, (float *)r->lum.ptr, 0, true);
+        if (!rc && r->p_expo && queue_exposure(r) != hipSuccess) rc = fail(SR_ERR_HIP, "exposure replay");
+        if (reran) *reran = true;
+    }
+    r->p_expo = false;
+    const_cast<sr_scene *>(r->p_scene)->pending_frames--;
+    r->p_scene = nullptr;
+    if (rc) {
+        r->has_frame = false;
+        return rc;
+    }
+    fill_stats(r);
+    return SR_OK;
+}
+
+SR_API int sr_renderer_synchronize(sr_renderer *r) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    return finish(r);
+}
+
 SR_API int sr_renderer_destroy(sr_renderer *r) {
     if (!r) return SR_OK;
+    (void)finish(r);
     hipSetDevice(r->device);
     r->pipe.destroy();
     r->ids.release();
@@ -109,40 +183,47 @@ SR_API int sr_renderer_destroy(sr_renderer *r) {
     return SR_OK;
 }
 
+// The frame is queued on the renderer's stream; the first call that reads
+// it (blit, present, exposure from its luminance, vbuffer/colour reads,
+// stats, synchronize) waits for it, checks it and reruns it if a capacity
+// overflowed, so a frame costs one host round trip with its blit.
 SR_API int sr_renderer_render_scene(sr_renderer *r, const sr_scene *scene, const sr_camera *camera) {
     if (!r || !scene || !camera) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
     if (scene->device != r->device) return fail(SR_ERR_INVALID_ARGUMENT, "scene and renderer are on different devices");
-    std::vector<DevDraw> draws;
-    uint32_t in_tri = 0, xv = 0;
-    int rc = build_draws(scene, *camera, 0, in_tri, xv, draws, true);
+    int rc = finish(r);        // the previous frame's overflow check sizes this one
     if (rc) return rc;
-    std::vector<DevView> views(1);
+    r->has_frame = false;
+    r->p_draws.clear();
+    r->p_in = r->p_xv = 0;
+    rc = build_draws(scene, *camera, 0, r->p_in, r->p_xv, r->p_draws, true);
+    if (rc) return rc;
+    r->p_views.assign(1, DevView());
     int T = r->pipe.tile_size;
-    fill_view(views[0], scene, *camera, r->W, r->H, T, 0, 0, 0, 0, r->auto_exposure, 0);
-    uint32_t items = (uint32_t)(views[0].tiles_x * views[0].tiles_y);
-    rc = r->pipe.run(views, draws, xv, in_tri, items, scene->any_tangents, (uint32_t *)r->ids.ptr,
+    fill_view(r->p_views[0], scene, *camera, r->W, r->H, T, 0, 0, 0, 0, r->auto_exposure, 0);
+    r->p_items = (uint32_t)(r->p_views[0].tiles_x * r->p_views[0].tiles_y);
+    r->p_tangents = scene->any_tangents;
+    r->p_translucent = false;
+    for (const DevDraw &d : r->p_draws) r->p_translucent = r->p_translucent || !(d.flags & SR_DRAW_OPAQUE);
+    rc = r->pipe.run(r->p_views, r->p_draws, r->p_xv, r->p_in, r->p_items, r->p_tangents, (uint32_t *)r->ids.ptr,
                      (float *)r->depth.ptr, (float *)r->color.ptr, (uint64_t)r->W * r->H, nullptr,
-                     (float *)r->lum.ptr, 0, true);
-    if (rc) return rc;
+                     (float *)r->lum.ptr, 0, false);
+    if (rc) {
+        hipStreamSynchronize(r->pipe.stream);
+        return rc;
+    }
     r->has_frame = true;
-    r->stats.input_triangles = in_tri;
-    r->stats.setup_triangles = r->pipe.last_records;
-    r->stats.binned_pairs = r->pipe.last_pairs;
-    r->stats.tiles = items;
-    r->stats.draws = (uint32_t)draws.size();
-    r->stats.depth_rejection = r->pipe.last_hiz ? 1u : 0u;
-    r->stats.ms_setup = ev_ms(r->pipe.ev[0], r->pipe.ev[1]);
-    r->stats.ms_bin = ev_ms(r->pipe.ev[1], r->pipe.ev[2]);
-    r->stats.ms_raster_shade = ev_ms(r->pipe.ev[2], r->pipe.ev[3]);
-    r->stats.ms_raster = ev_ms(r->pipe.ev[2], r->pipe.ev[4]);
-    r->stats.ms_shade = ev_ms(r->pipe.ev[4], r->pipe.ev[3]);
-    r->stats.ms_total = ev_ms(r->pipe.ev[0], r->pipe.ev[3]);
+    r->pending = true;
+    r->p_expo = false;
+    r->p_scene = scene;
+    const_cast<sr_scene *>(scene)->pending_frames++;
     return SR_OK;
 }
 
 // Brings the host's exposure fields up to date with a device exposure step.
 static int sync_exposure(sr_renderer *r) {
     if (!r->expo_on_device) return SR_OK;
+    int rc = finish(r);
+    if (rc) return rc;
     hipSetDevice(r->device);
     float st[4];
     HIP_TRY(hipMemcpyAsync(st, r->expo.ptr, 12, hipMemcpyDeviceToHost, r->pipe.stream));
@@ -169,6 +250,7 @@ SR_API int sr_renderer_update_auto_exposure(sr_renderer *r, float dt) {
         r->auto_exposure = std::pow(2.0f, r->auto_exposure_ev);
         return SR_OK;
     }
+    if ((src = finish(r))) return src;
     hipSetDevice(r->device);
     std::vector<float> s(n);
     HIP_TRY(hipMemcpy(s.data(), r->lum.ptr, n * 4, hipMemcpyDeviceToHost));
@@ -202,9 +284,20 @@ SR_API int sr_renderer_update_auto_exposure(sr_renderer *r, float dt) {
 SR_API int sr_renderer_update_auto_exposure_device(sr_renderer *r, float dt) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
     if (r->ref_tiles > SR_EXPOSURE_MAX) return sr_renderer_update_auto_exposure(r, dt);
+    if (r->p_expo) {               // one step replays after a rerun; a second one waits for the frame
+        int rc = finish(r);
+        if (rc) return rc;
+    }
     hipSetDevice(r->device);
-    HIP_TRY(launch_exposure((const float *)r->lum.ptr, (uint32_t)r->ref_tiles, dt, !r->expo_on_device,
-                            r->auto_exposure_ev, r->auto_exposure_target, (float *)r->expo.ptr, r->pipe.stream));
+    r->pe_dt = dt;
+    r->pe_use_host = !r->expo_on_device;
+    r->pe_ev = r->auto_exposure_ev;
+    r->pe_target = r->auto_exposure_target;
+    if (r->pending) {
+        HIP_TRY(hipMemcpyAsync((float *)r->expo.ptr + 4, r->expo.ptr, 12, hipMemcpyDeviceToDevice, r->pipe.stream));
+        r->p_expo = true;
+    }
+    HIP_TRY(queue_exposure(r));
     r->expo_on_device = true;
     return SR_OK;
 }
@@ -223,13 +316,26 @@ SR_API int sr_renderer_set_exposure(sr_renderer *r, float exposure) {
     return SR_OK;
 }
 
+// The blit of the current frame into dst, queued behind the frame and, if
+// checking the frame reran it, queued again.
+static int blit_frame(sr_renderer *r, uint32_t *dst, bool frame_bytes) {
+    hipSetDevice(r->device);
+    for (int k = 0; k < 2; k++) {
+        HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure, dst,
+                            r->pipe.stream, r->expo_on_device ? (const float *)r->expo.ptr : nullptr, frame_bytes));
+        bool reran = false;
+        int rc = finish(r, &reran);
+        if (rc) return rc;
+        if (!reran) break;
+    }
+    return SR_OK;
+}
+
 SR_API int sr_renderer_blit_to_device(sr_renderer *r, void *device_pixels) {
     if (!r || !device_pixels) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
-    hipSetDevice(r->device);
-    HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure,
-                        (uint32_t *)device_pixels, r->pipe.stream,
-                        r->expo_on_device ? (const float *)r->expo.ptr : nullptr));
+    int rc = blit_frame(r, (uint32_t *)device_pixels, false);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(r->pipe.stream));
     return SR_OK;
 }
@@ -239,10 +345,8 @@ SR_API int sr_renderer_blit_to_buffer(sr_renderer *r, uint32_t *pixels, size_t w
     if ((int)width != r->W || (int)height != r->H)
         return fail(SR_ERR_INVALID_ARGUMENT, "buffer is %zux%zu, renderer is %dx%d", width, height, r->W, r->H);
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
-    hipSetDevice(r->device);
-    HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure,
-                        (uint32_t *)r->rgba.ptr, r->pipe.stream,
-                        r->expo_on_device ? (const float *)r->expo.ptr : nullptr));
+    int rc = blit_frame(r, (uint32_t *)r->rgba.ptr, false);
+    if (rc) return rc;
     size_t rows = (size_t)(r->H & ~1);
     HIP_TRY(hipMemcpyAsync(pixels, r->rgba.ptr, rows * (size_t)r->W * 4, hipMemcpyDeviceToHost, r->pipe.stream));
     HIP_TRY(hipStreamSynchronize(r->pipe.stream));
@@ -251,11 +355,7 @@ SR_API int sr_renderer_blit_to_buffer(sr_renderer *r, uint32_t *pixels, size_t w
 
 // The presented frame: the blit with each pixel stored as the bytes r, g, b,
 // a (App::render_frame's copy into the pixels frame, src/main.rs:574-590).
-static int present(sr_renderer *r, uint32_t *dst) {
-    HIP_TRY(launch_blit((const float *)r->color.ptr, (uint64_t)r->W * r->H, r->W, r->H, r->auto_exposure, dst,
-                        r->pipe.stream, r->expo_on_device ? (const float *)r->expo.ptr : nullptr, true));
-    return SR_OK;
-}
+static int present(sr_renderer *r, uint32_t *dst) { return blit_frame(r, dst, true); }
 
 SR_API int sr_renderer_present(sr_renderer *r, uint8_t *frame, size_t width, size_t height) {
     if (!r || !frame) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
@@ -283,6 +383,7 @@ SR_API int sr_renderer_present_to_device(sr_renderer *r, void *device_frame) {
 
 SR_API int sr_renderer_read_vbuffer(sr_renderer *r, uint32_t *tri_id, float *depth) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (int rc = finish(r)) return rc;
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
     hipSetDevice(r->device);
     size_t n = (size_t)r->W * r->H;
@@ -293,6 +394,7 @@ SR_API int sr_renderer_read_vbuffer(sr_renderer *r, uint32_t *tri_id, float *dep
 
 SR_API int sr_renderer_read_color(sr_renderer *r, float *rgb) {
     if (!r || !rgb) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = finish(r)) return rc;
     if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
     hipSetDevice(r->device);
     size_t n = (size_t)r->W * r->H;
@@ -308,6 +410,7 @@ SR_API int sr_renderer_read_color(sr_renderer *r, float *rgb) {
 
 SR_API int sr_renderer_read_tile_luminance(sr_renderer *r, float *lum, int32_t *count) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (int rc = finish(r)) return rc;
     if (count) *count = r->ref_tiles;
     if (lum) {
         hipSetDevice(r->device);
@@ -318,12 +421,14 @@ SR_API int sr_renderer_read_tile_luminance(sr_renderer *r, float *lum, int32_t *
 
 SR_API int sr_renderer_get_stats(const sr_renderer *r, sr_frame_stats *stats) {
     if (!r || !stats) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = finish(const_cast<sr_renderer *>(r))) return rc;
     *stats = r->stats;
     return SR_OK;
 }
 
 SR_API int sr_renderer_device_buffers(sr_renderer *r, void **tri_id, void **depth, void **color_rgb) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (int rc = finish(r)) return rc;
     if (tri_id) *tri_id = r->ids.ptr;
     if (depth) *depth = r->depth.ptr;
     if (color_rgb) *color_rgb = r->color.ptr;

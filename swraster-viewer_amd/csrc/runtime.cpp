@@ -376,22 +376,31 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         }
         if (!sync) return SR_OK;
         HIP_TRY(hipStreamSynchronize(stream));
-        const DevStatus &s = *(const DevStatus *)status_host.ptr;
-        last_records = s.total_records;
-        last_pairs = s.total_pairs;
-        if (s.clip_overflow)
-            return fail(SR_ERR_UNSUPPORTED, "a frustum-clipped polygon exceeded 16 vertices (degenerate rounding)");
-        bool tr_ok = !has_tr || (!s.overflow_tr && s.total_tr_pairs <= tr_cap);
-        if (!s.overflow_records && !s.overflow_pairs && !s.overflow_clip && s.total_records <= rec_cap &&
-            s.total_pairs <= pair_cap && tr_ok)
-            return SR_OK;
-        if (!tr_ok) tr_cap = s.total_tr_pairs + s.total_tr_pairs / 8 + 4096;
-        if (s.overflow_clip || s.clip_queued > clip_cap) clip_cap = s.clip_queued + s.clip_queued / 4 + 1024;
-        if (s.total_records > rec_cap || s.overflow_records) rec_cap = s.total_records + s.total_records / 8 + 1024;
-        if (s.total_pairs > pair_cap || s.overflow_pairs) pair_cap = s.total_pairs + s.total_pairs / 8 + 4096;
-        if (s.overflow_pairs && s.total_pairs <= pair_cap && !s.overflow_records) pair_cap *= 2;
+        bool again = false;
+        if ((r = settle(has_tr, again))) return r;
+        if (!again) return SR_OK;
     }
     return fail(SR_ERR_CAPACITY, "frame buffers did not converge after reallocation");
+}
+
+int Pipeline::settle(bool has_tr, bool &again) {
+    const DevStatus &s = *(const DevStatus *)status_host.ptr;
+    last_records = s.total_records;
+    last_pairs = s.total_pairs;
+    again = false;
+    if (s.clip_overflow)
+        return fail(SR_ERR_UNSUPPORTED, "a frustum-clipped polygon exceeded 16 vertices (degenerate rounding)");
+    bool tr_ok = !has_tr || (!s.overflow_tr && s.total_tr_pairs <= tr_cap);
+    if (!s.overflow_records && !s.overflow_pairs && !s.overflow_clip && s.total_records <= rec_cap &&
+        s.total_pairs <= pair_cap && tr_ok)
+        return SR_OK;
+    again = true;
+    if (!tr_ok) tr_cap = s.total_tr_pairs + s.total_tr_pairs / 8 + 4096;
+    if (s.overflow_clip || s.clip_queued > clip_cap) clip_cap = s.clip_queued + s.clip_queued / 4 + 1024;
+    if (s.total_records > rec_cap || s.overflow_records) rec_cap = s.total_records + s.total_records / 8 + 1024;
+    if (s.total_pairs > pair_cap || s.overflow_pairs) pair_cap = s.total_pairs + s.total_pairs / 8 + 4096;
+    if (s.overflow_pairs && s.total_pairs <= pair_cap && !s.overflow_records) pair_cap *= 2;
+    return SR_OK;
 }
 
 void fill_view(DevView &v, const sr_scene *scene, const sr_camera &cam, int W, int H, int T, uint32_t item_base,

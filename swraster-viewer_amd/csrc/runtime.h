@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdint>
 #include <string>
@@ -78,6 +79,9 @@ struct Pipeline {
             uint32_t total_in, uint32_t total_items, bool any_tangents, uint32_t *out_ids, float *out_depth,
             float *out_color, uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks,
             bool sync, bool use_graph = false);
+    // After the stream has drained a frame: records its counts, grows what
+    // overflowed and sets again when the frame must be run once more.
+    int settle(bool has_translucent, bool &again);
 };
 
 }  // namespace sr
@@ -91,6 +95,7 @@ struct sr_scene {
     std::vector<sr::HostPrim> prims;
     bool any_tangents = false;
     bool any_translucent = false;     // a node draws a translucent primitive
+    std::atomic<int> pending_frames{0};   // renderer frames queued and not yet checked
     uint64_t triangles = 0, vertices = 0, device_bytes = 0;
 };
 

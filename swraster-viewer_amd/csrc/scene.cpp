@@ -346,6 +346,8 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
 
 SR_API int sr_scene_destroy(sr_scene *s) {
     if (!s) return SR_OK;
+    if (s->pending_frames.load() > 0)
+        return fail(SR_ERR_BUSY, "a renderer's queued frame still uses this scene (read it or synchronize first)");
     hipSetDevice(s->device);
     hipDeviceSynchronize();
     for (auto &b : s->buffers) b.release();

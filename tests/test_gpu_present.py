@@ -69,3 +69,45 @@ def test_present_rejects_bad_frames(sr, scenes, env):
         sr.C.POINTER(sr.C.c_uint8)), W + 1, H)
     assert rc == -1                               # SR_ERR_INVALID_ARGUMENT
     r.close()
+
+
+def test_queued_frame_rerun_replays_exposure_and_blit(sr, scenes, env):
+    """render_scene queues the frame; the first read checks it.  C3's first
+    frame on a fresh renderer outgrows its bin-pair buffer, so the check
+    reruns it: a device exposure step and a blit queued behind it are
+    replayed, giving the bits and exposure of a frame checked before them
+    (synchronize) and of a warm renderer's frame."""
+    scene, spec = scenes.make_c3(width=1280, height=720, triangles=40_000)
+    scene.prepare(env, cache_key=None)
+    gscene = sr.Scene(scene)
+    cam = scenes.make_camera(spec)
+    W, H = spec["width"], spec["height"]
+
+    def frame(r, sync_first):
+        r.render_scene(gscene, cam)
+        if sync_first:
+            r.synchronize()
+        r.update_auto_exposure_device(0.5)
+        px = r.blit_to_buffer()
+        return px, r.exposure, r.stats()
+
+    r1 = sr.Renderer(W, H)
+    px1, e1, st = frame(r1, False)
+    n_in = st["input_triangles"]
+    first_pair_cap = 2 * (n_in + n_in // 8 + 1024)
+    assert st["binned_pairs"] > first_pair_cap, "the first frame must outgrow its buffers"
+    r2 = sr.Renderer(W, H)
+    px2, e2, _ = frame(r2, True)
+    r3 = sr.Renderer(W, H)
+    r3.render_scene(gscene, cam)
+    r3.synchronize()
+    px3, e3, _ = frame(r3, False)
+    assert e1 == e2 == e3
+    assert np.array_equal(px1, px2) and np.array_equal(px1, px3)
+    # the scene is in use while a frame is queued
+    r1.render_scene(gscene, cam)
+    with pytest.raises(sr.SwrasterError):
+        sr.check(sr.lib().sr_scene_destroy(gscene.handle))
+    r1.synchronize()
+    for r in (r1, r2, r3):
+        r.close()
