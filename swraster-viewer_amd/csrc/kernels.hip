@@ -294,9 +294,12 @@ __device__ __forceinline__ int classify_input_triangle(const DevDraw &D, const D
     return clip_classify(c);
 }
 
-// counts[t]: 1 inside the frustum, 0 outside, fan size | SR_COUNT_CLIPPED
-// for a triangle that crosses it (written by k_clip).
+// counts[t]: 1 inside the frustum, 0 outside; for a triangle that crosses
+// it (written by k_clip) SR_COUNT_CLIPPED | clip-queue slot << 4 | fan size.
 #define SR_COUNT_CLIPPED 0x80000000u
+
+// Post-clip triangles that count stands for.
+__device__ __forceinline__ uint32_t count_fans(uint32_t c) { return (c & SR_COUNT_CLIPPED) ? (c & 15u) : c; }
 
 // chunk[c] gets the number of triangles of chunk c (input triangles 256c ..
 // 256c + 255) that lie inside the frustum; k_clip adds the fans of the
@@ -353,6 +356,7 @@ __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const 
 // frame (status clip_overflow) rather than being cut short.
 // ---------------------------------------------------------------------------
 #define CLIP_MAXV 16
+static_assert(CLIP_MAXV - 2 < 16, "a fan size fits the count's low four bits");
 
 __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDraw *draws, int ndraws,
                                               const float4 *xp, const float4 *xa, const float4 *xt, uint32_t *counts,
@@ -420,7 +424,7 @@ __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDra
         for (int i = 0; i < len; i++) buf[((size_t)e * CLIP_MAXV + i) * 16 + c] = P[i][c];
         if (c == 0) {
             uint32_t fan = len ? (uint32_t)(len - 2) : 0u;
-            counts[t] = fan | SR_COUNT_CLIPPED;
+            counts[t] = SR_COUNT_CLIPPED | (e << 4) | fan;
             if (fan) atomicAdd(&chunk[t >> 8], fan);
         }
         wave_lds_sync();
@@ -930,7 +934,8 @@ __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view,
 }
 
 // Setup of input triangle t of draw D (index d), which k_clip_count found
-// inside the frustum (k_clip_emit sets up the fans of those that cross it).
+// inside the frustum (the fans of those that cross it come from their
+// clipped polygons, emit_clipped_fan).
 __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, const DevView *views, uint32_t t,
                                                      const float4 *xp, uint32_t id, const SetupOut &O) {
     const DevView &V = views[D.view];
@@ -946,25 +951,48 @@ __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, co
                          !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
 }
 
+// Fan triangle f, (v0, v_f+1, v_f+2), of the clipped polygon in clip-queue
+// slot e (src/renderer.rs:657-666), set up as post-clip triangle id.
+__device__ __forceinline__ Rect emit_clipped_fan(const DevView *views, const DevDraw &D, const float *buf, uint32_t e,
+                                                  uint32_t f, uint32_t id, const SetupOut &O) {
+    const DevView &V = views[D.view];
+    const DevScene &S = *V.scene;
+    CV v[3];
+    const uint32_t src[3] = {0u, f + 1u, f + 2u};
+    for (int j = 0; j < 3; j++) {
+        const float4 *q = reinterpret_cast<const float4 *>(buf + ((size_t)e * CLIP_MAXV + src[j]) * 16);
+        float4 a = q[0], b = q[1], cc = q[2], dd = q[3];
+        v[j].cx = a.x; v[j].cy = a.y; v[j].cz = a.z; v[j].cw = a.w;
+        v[j].wx = b.x; v[j].wy = b.y; v[j].wz = b.z; v[j].nx = b.w;
+        v[j].ny = cc.x; v[j].nz = cc.y; v[j].u = cc.z; v[j].v = cc.w;
+        v[j].tx = dd.x; v[j].ty = dd.y; v[j].tz = dd.z; v[j].tw = dd.w;
+    }
+    return emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, S.materials[D.material].flags,
+                         !(D.flags & SR_DRAW_OPAQUE), SRC_CLIPPED | (f << 27) | e, 0u, O);
+}
+
 // The post-clip id of input triangle t is its chunk's first id plus the
 // fan counts before it in the chunk (a workgroup scan); it is stored for
-// k_clip_emit and the per-view offsets.
+// the per-view offsets.  A triangle inside the frustum is set up from its
+// clip positions; one that crosses it has its fan triangles set up from
+// the polygon k_clip left in the clip buffer (ids tri_base[t] + f).
 __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
                                                const float4 *xp, const uint32_t *counts, const uint32_t *chunk,
-                                               uint32_t *tri_base, SetupOut O) {
+                                               const float *clip_buf, uint32_t *tri_base, SetupOut O) {
     __shared__ uint32_t sh[17];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t t = base + threadIdx.x;
         uint32_t ex, c = t < total ? counts[t] : 0u;
-        block_excl_scan(c & ~SR_COUNT_CLIPPED, ex, sh);
+        uint32_t nf = count_fans(c);
+        block_excl_scan(nf, ex, sh);
         uint32_t id = chunk[base >> 8] + ex;
         Rect r = {0, 0, 0, 0};
-        int view = 0;
+        int view = 0, d = 0;
         if (t < total) {
             tri_base[t] = id;
-            if (t == total - 1) tri_base[total] = id + (c & ~SR_COUNT_CLIPPED);
-            int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+            if (t == total - 1) tri_base[total] = id + nf;
+            d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
             if (du >= 0) {
                 view = draws[du].view;
@@ -975,49 +1003,15 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
             }
         }
         bin_rect_count(views, view, r, O.bin_count);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// k_clip_emit: fan triangles (v0, v_i, v_i+1) of the clipped polygons
-// (src/renderer.rs:657-666), one thread per (queued triangle, fan slot).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_clip_emit(const DevView *views, const DevDraw *draws, int ndraws,
-                                                   const uint32_t *queue, uint32_t qcap, const float *buf,
-                                                   const uint32_t *counts, const uint32_t *tri_base,
-                                                   const DevStatus *status, SetupOut O) {
-    uint32_t nq = min(status->clip_queued, qcap);
-    uint32_t total = nq * (uint32_t)(CLIP_MAXV - 2);
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
-        uint32_t k = base + threadIdx.x;
-        Rect r = {0, 0, 0, 0};
-        int view = 0;
-        if (k < total) {
-            uint32_t e = k / (uint32_t)(CLIP_MAXV - 2), f = k % (uint32_t)(CLIP_MAXV - 2);
-            uint32_t t = queue[e];
-            if (f < (counts[t] & ~SR_COUNT_CLIPPED)) {
-                int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
-                const DevDraw &D = draws[d];
-                view = D.view;
-                const DevView &V = views[D.view];
-                const DevScene &S = *V.scene;
-                CV v[3];
-                const uint32_t src[3] = {0u, f + 1u, f + 2u};
-                for (int j = 0; j < 3; j++) {
-                    const float4 *q = reinterpret_cast<const float4 *>(buf + ((size_t)e * CLIP_MAXV + src[j]) * 16);
-                    float4 a = q[0], b = q[1], cc = q[2], dd = q[3];
-                    v[j].cx = a.x; v[j].cy = a.y; v[j].cz = a.z; v[j].cw = a.w;
-                    v[j].wx = b.x; v[j].wy = b.y; v[j].wz = b.z; v[j].nx = b.w;
-                    v[j].ny = cc.x; v[j].nz = cc.y; v[j].u = cc.z; v[j].v = cc.w;
-                    v[j].tx = dd.x; v[j].ty = dd.y; v[j].tz = dd.z; v[j].tw = dd.w;
-                }
-                r = emit_triangle(V, D.view, &v[0], &v[1], &v[2], tri_base[t] + f, D.material,
-                                  S.materials[D.material].flags, !(D.flags & SR_DRAW_OPAQUE),
-                                  SRC_CLIPPED | (f << 27) | e, 0u, O);
-            }
+        int nclip = (c & SR_COUNT_CLIPPED) ? (int)nf : 0;
+        int maxf = wave_max(nclip);
+        for (int f = 0; f < maxf; f++) {
+            Rect rf = {0, 0, 0, 0};
+            if (f < nclip)
+                rf = emit_clipped_fan(views, draws[d], clip_buf, (c & ~SR_COUNT_CLIPPED) >> 4, (uint32_t)f,
+                                      id + (uint32_t)f, O);
+            bin_rect_count(views, view, rf, O.bin_count);
         }
-        bin_rect_count(views, view, r, O.bin_count);
     }
 }
 
@@ -2575,9 +2569,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     so.T = a.tile_size;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.tri_chunks, a.tri_base, so);
-        hipLaunchKernelGGL(k_clip_emit, dim3(256), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.clip_queue,
-                           a.clip_cap, a.clip_buf, a.tri_counts, a.tri_base, a.status, so);
+                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.tri_chunks, a.clip_buf, a.tri_base, so);
     }
     if (a.events[1]) hipEventRecord(a.events[1], st);
     e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st);

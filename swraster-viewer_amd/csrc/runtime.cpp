@@ -397,6 +397,8 @@ int Pipeline::settle(bool has_tr, bool &again) {
     again = true;
     if (!tr_ok) tr_cap = s.total_tr_pairs + s.total_tr_pairs / 8 + 4096;
     if (s.overflow_clip || s.clip_queued > clip_cap) clip_cap = s.clip_queued + s.clip_queued / 4 + 1024;
+    if (clip_cap > (1u << 27))      // clip-queue slots are 27-bit fields of the counts and record tags
+        return fail(SR_ERR_CAPACITY, "more than 2^27 triangles cross the frustum");
     if (s.total_records > rec_cap || s.overflow_records) rec_cap = s.total_records + s.total_records / 8 + 1024;
     if (s.total_pairs > pair_cap || s.overflow_pairs) pair_cap = s.total_pairs + s.total_pairs / 8 + 4096;
     if (s.overflow_pairs && s.total_pairs <= pair_cap && !s.overflow_records) pair_cap *= 2;
