@@ -670,9 +670,66 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t *cursor, uint32_t bin) {
     return (uint32_t)__shfl((int)base, leader, 64) + rank;
 }
 
+__device__ __forceinline__ int32_t top_left_bias(int32_t a, int32_t b) { return (a < 0 || (a == 0 && b > 0)) ? 0 : -1; }
+
+// Edge functions of the snapped triangle (src/tilerasterizer.rs:139-164):
+// edge 0 = v1->v2 (w0), 1 = v2->v0 (w1), 2 = v0->v1 (w2); i32 constants in
+// wrapping arithmetic, f32 coefficients and 2-px steps.
+__device__ __forceinline__ void edge_setup(int32_t p0x, int32_t p0y, int32_t p1x, int32_t p1y, int32_t p2x, int32_t p2y,
+                                           float A[3], float B[3], float C[3], float SX[3], float SY[3]) {
+    int32_t a01 = wsub(p1y, p0y), b01 = wsub(p0x, p1x);
+    int32_t c01 = wadd(wsub(wmul(p1x, p0y), wmul(p0x, p1y)), top_left_bias(a01, b01));
+    int32_t a12 = wsub(p2y, p1y), b12 = wsub(p1x, p2x);
+    int32_t c12 = wadd(wsub(wmul(p2x, p1y), wmul(p1x, p2y)), top_left_bias(a12, b12));
+    int32_t a20 = wsub(p0y, p2y), b20 = wsub(p2x, p0x);
+    int32_t c20 = wadd(wsub(wmul(p0x, p2y), wmul(p2x, p0y)), top_left_bias(a20, b20));
+    A[0] = (float)a12; B[0] = (float)b12; C[0] = (float)c12;
+    A[1] = (float)a20; B[1] = (float)b20; C[1] = (float)c20;
+    A[2] = (float)a01; B[2] = (float)b01; C[2] = (float)c01;
+    SX[0] = (float)wmul(a12, 32); SY[0] = (float)wmul(b12, 32);
+    SX[1] = (float)wmul(a20, 32); SY[1] = (float)wmul(b20, 32);
+    SX[2] = (float)wmul(a01, 32); SY[2] = (float)wmul(b01, 32);
+}
+
+// A small opaque triangle that makes no fragment: its pixel box
+// [pmnx, pmxx) x [pmny, pmxy) lies in one 64-px reference tile and spans at
+// most two quads each way, so the reference walks it with the fine raster
+// from the quad-aligned corner (src/tilerasterizer.rs:191-217, 293-383),
+// and no sample of those quads passes all three edge tests.  The values are
+// the raster's own: origin evaluation (A x + B y) + C, then +step_y for the
+// second quad row, then +step_x for the second column.
+__device__ __forceinline__ bool small_tri_uncovered(const int32_t px[3], const int32_t py[3], int32_t pmnx, int32_t pmny,
+                                                    int32_t pmxx, int32_t pmxy) {
+    int32_t ox = pmnx & ~1, oy = pmny & ~1;
+    if (pmxx - ox > 4 || pmxy - oy > 4) return false;
+    if ((pmnx >> 6) != ((pmxx - 1) >> 6) || (pmny >> 6) != ((pmxy - 1) >> 6)) return false;
+    float A[3], B[3], C[3], SX[3], SY[3];
+    edge_setup(px[0], py[0], px[1], py[1], px[2], py[2], A, B, C, SX, SY);
+    bool col2 = ox + 2 < pmxx, row2 = oy + 2 < pmxy;
+    bool covered = false;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        float X = (float)(ox * 16 + 8 + 16 * (s & 1)), Y = (float)(oy * 16 + 8 + 16 * (s >> 1));
+        bool q00 = true, q10 = true, q01 = true, q11 = true;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            float v = (A[e] * X + B[e] * Y) + C[e];
+            float vy = v + SY[e];
+            q00 = q00 && v >= 0.0f;
+            q10 = q10 && (v + SX[e]) >= 0.0f;
+            q01 = q01 && vy >= 0.0f;
+            q11 = q11 && (vy + SX[e]) >= 0.0f;
+        }
+        covered = covered || q00 || (col2 && q10) || (row2 && q01) || (col2 && row2 && q11);
+    }
+    return !covered;
+}
+
 // Snap, cull, bbox and setup of one (post-clip) triangle (bin_triangle,
 // src/renderer.rs:668-831).  Writes the geometry and raster records and
-// returns the bin rectangle (empty when culled or when the box is empty).
+// returns the bin rectangle (empty when culled, when the box is empty, or
+// for a small opaque triangle that covers no sample: small_tri_uncovered;
+// such a triangle keeps its id but gets no raster record and no bins).
 // The shading interpolators of the reference's packet (:701-754) are not
 // stored: tri_shade_attrs recomputes them, with the same operations, for the
 // few triangles that win a pixel, from the vertices that `src` locates
@@ -706,7 +763,7 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
     int32_t mxy = min(max(max(py[0], py[1]), py[2]), V.H_sub);
     int32_t pmnx = mnx >> 4, pmny = mny >> 4;
     int32_t pmxx = wadd(mxx, 16) >> 4, pmxy = wadd(mxy, 16) >> 4;
-    if (!(pmnx < pmxx && pmny < pmxy)) {
+    if (!(pmnx < pmxx && pmny < pmxy) || (!translucent && small_tri_uncovered(px, py, pmnx, pmny, pmxx, pmxy))) {
         O.geo[id] = make_uint4(0u, 0u, SR_GEO_EMPTY, (uint32_t)view);
         return none;
     }
@@ -1050,25 +1107,12 @@ struct TriSetup {
     bool coarse;
 };
 
-__device__ __forceinline__ int32_t top_left_bias(int32_t a, int32_t b) { return (a < 0 || (a == 0 && b > 0)) ? 0 : -1; }
-
 __device__ __forceinline__ void tri_setup(const uint4 g, const float4 *R, int tx0, int ty0, TriSetup &S) {
     float4 r0 = R[0], r1 = R[1], r2 = R[2], r3 = R[3];
     int32_t p0x = __float_as_int(r0.x), p0y = __float_as_int(r0.y), p1x = __float_as_int(r0.z), p1y = __float_as_int(r0.w);
     int32_t p2x = __float_as_int(r1.x), p2y = __float_as_int(r1.y);
     S.ooa = r1.z; S.oa = r1.w; S.oda = r2.x; S.odb = r2.y; S.za = r2.z; S.zda = r2.w; S.zdb = r3.x;
-    int32_t a01 = wsub(p1y, p0y), b01 = wsub(p0x, p1x);
-    int32_t c01 = wadd(wsub(wmul(p1x, p0y), wmul(p0x, p1y)), top_left_bias(a01, b01));
-    int32_t a12 = wsub(p2y, p1y), b12 = wsub(p1x, p2x);
-    int32_t c12 = wadd(wsub(wmul(p2x, p1y), wmul(p1x, p2y)), top_left_bias(a12, b12));
-    int32_t a20 = wsub(p0y, p2y), b20 = wsub(p2x, p0x);
-    int32_t c20 = wadd(wsub(wmul(p0x, p2y), wmul(p2x, p0y)), top_left_bias(a20, b20));
-    S.A[0] = (float)a12; S.B[0] = (float)b12; S.C[0] = (float)c12;
-    S.A[1] = (float)a20; S.B[1] = (float)b20; S.C[1] = (float)c20;
-    S.A[2] = (float)a01; S.B[2] = (float)b01; S.C[2] = (float)c01;
-    S.SX[0] = (float)wmul(a12, 32); S.SY[0] = (float)wmul(b12, 32);
-    S.SX[1] = (float)wmul(a20, 32); S.SY[1] = (float)wmul(b20, 32);
-    S.SX[2] = (float)wmul(a01, 32); S.SY[2] = (float)wmul(b01, 32);
+    edge_setup(p0x, p0y, p1x, p1y, p2x, p2y, S.A, S.B, S.C, S.SX, S.SY);
     int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
     int rx = tx0 & ~(SR_REF_TILE - 1), ry = ty0 & ~(SR_REF_TILE - 1);
     S.cmnx = max(mnx, rx); S.cmny = max(mny, ry);

@@ -226,6 +226,40 @@ def test_depth_rejection_tilted_overdraw(sr, orc, scenes, seed):
         assert np.array_equal(ids, o["ids"]) and np.array_equal(bits(depth), bits(o["depth"]))
 
 
+@pytest.mark.parametrize("W,H,seed", [(256, 144, 7), (130, 66, 8)])
+def test_tiny_triangles_parity(sr, orc, scenes, W, H, seed):
+    """Dense sub-pixel to 4-px triangles (most of them cover no pixel centre,
+    and setup drops those before binning), over the whole screen including
+    the partial right and bottom tiles and the 64-px tile seams: the
+    visibility buffer is the oracle's bit for bit."""
+    from util import attach_dummy, screen_triangle_scene
+    rng = np.random.default_rng(seed)
+    n = 6000
+    c = rng.uniform([-16, -16], [(W + 1) * 16, (H + 1) * 16], (n, 1, 2))
+    ang = rng.uniform(0, 2 * np.pi, (n, 3))
+    rad = rng.uniform(2, 40, (n, 3))           # subpixels: 1/8 px to 2.5 px from the centre
+    tris = np.round(c + np.stack([np.cos(ang), np.sin(ang)], -1) * rad[..., None])
+    a = (tris[:, 1, 0] - tris[:, 0, 0]) * (tris[:, 2, 1] - tris[:, 0, 1]) - \
+        (tris[:, 2, 0] - tris[:, 0, 0]) * (tris[:, 1, 1] - tris[:, 0, 1])
+    tris[a > 0] = tris[a > 0][:, ::-1]
+    depths = rng.uniform(1.0, 100.0, (n, 3))
+    scene, cam = screen_triangle_scene(sr, tris, depths, W, H)
+    attach_dummy(sr, scene)
+    gscene = sr.Scene(scene)
+    camera = scenes.make_camera(cam)
+    r = sr.Renderer(W, H)
+    r.render_scene(gscene, camera)
+    ids, depth = r.read_vbuffer()
+    st = r.stats()
+    r.close()
+    o = render_oracle(orc, scene, camera, W, H)
+    assert np.array_equal(ids, o["ids"]) and np.array_equal(bits(depth), bits(o["depth"]))
+    covered = int((ids != 0xFFFFFFFF).sum())
+    print(f"[tiny] {W}x{H}: {st['setup_triangles']} set up, {st['binned_pairs']} bin entries, "
+          f"{covered} covered pixels", flush=True)
+    assert st["binned_pairs"] < st["setup_triangles"]     # most of them were dropped before binning
+
+
 def test_determinism(sr, scenes, c1):
     scene, spec, gscene = c1
     cam = scenes.make_camera(spec)
