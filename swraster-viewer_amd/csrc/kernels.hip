@@ -1,10 +1,12 @@
 // kernels.hip -- the per-frame rasterization pipeline for gfx950 (MI355X).
 //
 // Stage map (reference paths are relative to the reference tree):
-//   k_transform     SoA vertex fetch, MVP / model / normal transform, once per
-//                   (draw, vertex)          src/renderer.rs:499-521
-//   k_clip_count    frustum clipping of clip-space positions to count fan
-//                   triangles               src/renderer.rs:579-665
+//   k_front         one launch of two parts: SoA vertex fetch and MVP / model
+//                   / normal transform once per (draw, vertex)
+//                                           src/renderer.rs:499-521
+//                   and the frustum classification of every input triangle
+//                   (fan counts, clip queue) src/renderer.rs:579-665
+//   k_clip          clipping of the queued triangles with all attributes
 //   k_scan_*        device-wide exclusive scan (post-clip ids, bin offsets)
 //   k_setup         clipping with attributes, snap, cull, bbox, setup records,
 //                   per-tile bin counts     src/renderer.rs:668-846
@@ -171,20 +173,16 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
     return -1;
 }
 
-// The frame's counters and status cleared (by k_transform's threads, or by
-// k_frame_begin when there are no vertices).
+// Clears the frame's bin counts over the whole grid, in k_front (or in
+// k_frame_begin when there are no vertices); the frame's status is cleared
+// by the upload that precedes the first kernel.
 struct FrameBegin {
-    DevStatus *status;
     uint32_t *bin_count, *tr_count;
     uint32_t items, tr_slots;
 };
 
 __device__ __forceinline__ void frame_begin(const FrameBegin &F) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        uint32_t *w = reinterpret_cast<uint32_t *>(F.status);
-        for (uint32_t k = 0; k < (uint32_t)(sizeof(DevStatus) / 4); k++) w[k] = 0u;
-    }
     for (uint32_t k = i; k < F.items; k += gridDim.x * blockDim.x) F.bin_count[k] = 0u;
     if (F.tr_count)
         for (uint32_t k = i; k < F.tr_slots; k += gridDim.x * blockDim.x) F.tr_count[k] = 0u;
@@ -193,8 +191,22 @@ __device__ __forceinline__ void frame_begin(const FrameBegin &F) {
 __global__ void __launch_bounds__(256) k_frame_begin(FrameBegin F) { frame_begin(F); }
 
 // ---------------------------------------------------------------------------
-// k_transform: one thread per (draw, vertex)
+// Vertex transform (k_front): one thread per (draw, vertex)
 // ---------------------------------------------------------------------------
+// Clip position MVP p of source vertex s (render_primitive_inner,
+// src/renderer.rs:496-521); the transform and the frustum classification
+// both take it from here, so they agree bit for bit.
+__device__ __forceinline__ void clip_position(const DevDraw &D, const DevScene &S, uint32_t s, float c[4]) {
+    float x = S.px[s], y = S.py[s], z = S.pz[s];
+    for (int r = 0; r < 4; r++) {
+        float acc = D.mvp[r] * x;
+        acc = acc + D.mvp[4 + r] * y;
+        acc = acc + D.mvp[8 + r] * z;
+        acc = acc + D.mvp[12 + r] * 1.0f;
+        c[r] = acc;
+    }
+}
+
 // Transform of vertex i of draw D (render_primitive_inner, src/renderer.rs:
 // 496-521): clip = MVP p, world = M p, normal = M3 n, tangent = M3 t.
 __device__ __forceinline__ void transform_vertex(const DevDraw &D, const DevScene &S, uint32_t i, float4 *xp,
@@ -203,13 +215,7 @@ __device__ __forceinline__ void transform_vertex(const DevDraw &D, const DevScen
     float x = S.px[s], y = S.py[s], z = S.pz[s];
     float nx = S.nx[s], ny = S.ny[s], nz = S.nz[s];
     float c[4], w[3], n[3];
-    for (int r = 0; r < 4; r++) {
-        float acc = D.mvp[r] * x;
-        acc = acc + D.mvp[4 + r] * y;
-        acc = acc + D.mvp[8 + r] * z;
-        acc = acc + D.mvp[12 + r] * 1.0f;
-        c[r] = acc;
-    }
+    clip_position(D, S, s, c);
     for (int r = 0; r < 3; r++) {
         float acc = D.model[r] * x;
         acc = acc + D.model[4 + r] * y;
@@ -248,12 +254,11 @@ __device__ __forceinline__ int wave_uniform(int d) {
     return __ballot(d != d0) == 0ull ? d0 : -1;
 }
 
-__global__ void __launch_bounds__(256) k_transform(const DevView *__restrict__ views,
-                                                   const DevDraw *__restrict__ draws, int ndraws, uint32_t total,
-                                                   float4 *__restrict__ xp, float4 *__restrict__ xa,
-                                                   float4 *__restrict__ xt, FrameBegin F) {
-    frame_begin(F);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+// Vertices b*256+lane, stepping by nb*256, of the transform's nb workgroups.
+__device__ __forceinline__ void transform_vertices(const DevView *views, const DevDraw *draws, int ndraws,
+                                                   uint32_t total, float4 *xp, float4 *xa, float4 *xt, uint32_t b,
+                                                   uint32_t nb) {
+    for (uint32_t i = b * blockDim.x + threadIdx.x; i < total; i += nb * blockDim.x) {
         int d = find_last_le(draws, ndraws, i, &DevDraw::xv_base);
         int du = wave_uniform(d);
         if (du >= 0) {
@@ -274,23 +279,19 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ---------------------------------------------------------------------------
-// k_clip_count: fan triangles per input triangle.  Triangles inside the
-// frustum make one, rejected ones none; those that cross a plane are queued
-// for k_clip, which writes their count.
+// Triangle classification (k_front): fan triangles per input triangle.
+// Triangles inside the frustum make one, rejected ones none; those that
+// cross a plane are queued for k_clip, which writes their count.
 // ---------------------------------------------------------------------------
 // Frustum classification of input triangle t of draw D (-1 = inside, or no
-// clipping needed).
-__device__ __forceinline__ int classify_input_triangle(const DevDraw &D, const DevView *views, uint32_t t,
-                                                       const float4 *xp) {
+// clipping needed), from clip positions computed here as the transform
+// computes them.
+__device__ __forceinline__ int classify_input_triangle(const DevDraw &D, const DevView *views, uint32_t t) {
     if (!(D.flags & SR_DRAW_CLIP)) return -1;
     const DevScene &S = *views[D.view].scene;
     uint32_t lt = t - D.in_tri_base;
     float c[3][4];
-    for (int k = 0; k < 3; k++) {
-        uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-        float4 p = xp[D.xv_base + vi];
-        c[k][0] = p.x; c[k][1] = p.y; c[k][2] = p.z; c[k][3] = p.w;
-    }
+    for (int k = 0; k < 3; k++) clip_position(D, S, D.vtx_base + S.indices[D.idx_base + 3 * lt + k], c[k]);
     return clip_classify(c);
 }
 
@@ -304,21 +305,22 @@ __device__ __forceinline__ uint32_t count_fans(uint32_t c) { return (c & SR_COUN
 // chunk[c] gets the number of triangles of chunk c (input triangles 256c ..
 // 256c + 255) that lie inside the frustum; k_clip adds the fans of the
 // clipped ones, and k_scan_top turns the sums into each chunk's first id.
-__global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const DevDraw *draws, int ndraws,
-                                                    uint32_t total, const float4 *xp, uint32_t *counts,
-                                                    uint32_t *chunk, uint32_t *queue, uint32_t qcap,
-                                                    DevStatus *status) {
+// Workgroup b of nb (256 threads) takes chunks b, b + nb, ...
+__device__ __forceinline__ void classify_triangles(const DevView *views, const DevDraw *draws, int ndraws,
+                                                   uint32_t total, uint32_t *counts, uint32_t *chunk,
+                                                   uint32_t *queue, uint32_t qcap, DevStatus *status, uint32_t b,
+                                                   uint32_t nb) {
     __shared__ uint32_t wsum[4];
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = nb * blockDim.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+    for (uint32_t base = b * blockDim.x; base < total; base += stride) {
         uint32_t t = base + threadIdx.x;
         bool q = false, in = false;
         if (t < total) {
             int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
-            int cls = du >= 0 ? classify_input_triangle(draws[du], views, t, xp)
-                              : classify_input_triangle(draws[d], views, t, xp);
+            int cls = du >= 0 ? classify_input_triangle(draws[du], views, t)
+                              : classify_input_triangle(draws[d], views, t);
             in = cls == -1;
             counts[t] = in ? 1u : 0u;
             q = cls >= 0;
@@ -341,6 +343,24 @@ __global__ void __launch_bounds__(256) k_clip_count(const DevView *views, const 
             }
         }
     }
+}
+
+// The frame's first kernel: workgroups [0, tri_blocks) classify the input
+// triangles, the others transform the vertices (no data passes between the
+// two: the classification computes its clip positions itself), and every
+// workgroup helps clear the bin counts.
+__global__ void __launch_bounds__(256) k_front(const DevView *__restrict__ views, const DevDraw *__restrict__ draws,
+                                               int ndraws, uint32_t total_in, uint32_t total_xv,
+                                               float4 *__restrict__ xp, float4 *__restrict__ xa,
+                                               float4 *__restrict__ xt, uint32_t *counts, uint32_t *chunk,
+                                               uint32_t *queue, uint32_t qcap, DevStatus *status, FrameBegin F,
+                                               uint32_t tri_blocks) {
+    frame_begin(F);
+    if (blockIdx.x < tri_blocks)
+        classify_triangles(views, draws, ndraws, total_in, counts, chunk, queue, qcap, status, blockIdx.x, tri_blocks);
+    else
+        transform_vertices(views, draws, ndraws, total_xv, xp, xa, xt, blockIdx.x - tri_blocks,
+                           gridDim.x - tri_blocks);
 }
 
 // ---------------------------------------------------------------------------
@@ -990,7 +1010,7 @@ __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view,
     }
 }
 
-// Setup of input triangle t of draw D (index d), which k_clip_count found
+// Setup of input triangle t of draw D (index d), which the classification (k_front) found
 // inside the frustum (the fans of those that cross it come from their
 // clipped polygons, emit_clipped_fan).
 __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, const DevView *views, uint32_t t,
@@ -2575,20 +2595,19 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const uint32_t GRID_CAP = 256 * 16;
     if (a.events[0]) hipEventRecord(a.events[0], st);
     FrameBegin fb;
-    fb.status = a.status;
     fb.bin_count = a.bin_count;
     fb.items = a.total_items;
     fb.tr_count = a.has_translucent ? a.tr_count : nullptr;
     fb.tr_slots = a.tr_slots;
-    if (a.total_xv > 0)
-        hipLaunchKernelGGL(k_transform, dim3(grid_for(a.total_xv, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr, fb);
+    uint32_t tri_blocks = a.total_in > 0 ? grid_for(a.total_in, 256, GRID_CAP) : 0u;
+    uint32_t vtx_blocks = a.total_xv > 0 ? grid_for(a.total_xv, 256, GRID_CAP) : 0u;
+    if (tri_blocks + vtx_blocks > 0)
+        hipLaunchKernelGGL(k_front, dim3(tri_blocks + vtx_blocks), dim3(256), 0, st, a.views, a.draws, a.num_draws,
+                           a.total_in, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr,
+                           a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap, a.status, fb, tri_blocks);
     else
         hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, fb);
     if (a.total_in > 0) {
-        hipLaunchKernelGGL(k_clip_count, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
-                           a.status);
         hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv, a.xv + a.total_xv,
                            a.any_tangents ? a.xt : nullptr, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
                            a.clip_buf, a.status);

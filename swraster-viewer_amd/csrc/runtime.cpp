@@ -196,9 +196,7 @@ int Pipeline::init(int dev, int tile) {
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-    int r = status_host.ensure(sizeof(DevStatus));
-    if (r) return r;
-    return status.ensure(sizeof(DevStatus));
+    return status_host.ensure(sizeof(DevStatus));
 }
 
 void Pipeline::destroy() {
@@ -206,7 +204,7 @@ void Pipeline::destroy() {
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
     DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_chunks, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
-                      &list, &rec_geo, &rec_raster, &status, &center, &clip_queue, &clip_buf, &tr_count,
+                      &list, &rec_geo, &rec_raster, &center, &clip_queue, &clip_buf, &tr_count,
                       &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color};
     for (DevBuf *b : bufs) b->release();
     stage.release();
@@ -223,17 +221,18 @@ void Pipeline::drop_graph() {
     memset(&graph_key, 0, sizeof(graph_key));
 }
 
-// The frame's whole launch sequence (uploads of the view and draw tables from
-// pinned staging, every kernel, the status read-back) as one executable
-// graph, re-captured only when the launch shape or a buffer changes.  The
-// sequence clears memory with fill kernels only (no memset nodes).
-int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
+// The frame's whole launch sequence (the upload of the view and draw tables
+// and the zeroed status from pinned staging, every kernel, the status
+// read-back) as one executable graph, re-captured only when the launch shape
+// or a buffer changes.  The sequence clears memory with the upload and fill
+// kernels only (no memset nodes).
+int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t upload) {
     GraphKey k;
     memset(&k, 0, sizeof(k));
     k.n[0] = (uint64_t)a.num_views; k.n[1] = (uint64_t)a.num_draws; k.n[2] = a.total_xv; k.n[3] = a.total_in;
     k.n[4] = a.total_items; k.n[5] = (uint64_t)a.tile_size; k.n[6] = (uint64_t)a.any_tangents; k.n[7] = a.rec_cap;
     k.n[8] = a.pair_cap; k.n[9] = a.clip_cap; k.n[10] = (uint64_t)a.persistent_blocks; k.n[11] = vbytes;
-    k.n[12] = dbytes; k.n[13] = a.color_plane; k.n[14] = (uint64_t)a.has_translucent;
+    k.n[12] = upload; k.n[13] = a.color_plane; k.n[14] = (uint64_t)a.has_translucent;
     k.n[15] = ((uint64_t)a.tr_cap << 32) | a.tr_slots | ((uint64_t)a.hiz << 31);
     const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_chunks, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
@@ -245,7 +244,7 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes) {
     if (!graph_exec || memcmp(&k, &graph_key, sizeof(k)) != 0) {
         drop_graph();
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = hipMemcpyAsync((void *)a.views, stage.ptr, vbytes + dbytes, hipMemcpyHostToDevice, stream);
+        hipError_t e = hipMemcpyAsync((void *)a.views, stage.ptr, upload, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = launch_frame(a, stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(status_host.ptr, a.status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream);
@@ -271,15 +270,18 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
                   uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks, bool sync,
                   bool use_graph) {
     HIP_TRY(hipSetDevice(device));
-    // the view and draw tables share one device buffer (draws at a 256-byte
-    // aligned offset) so one copy uploads both
+    // the view table, the draw table and the frame's status (zeroed) share
+    // one device buffer at 256-byte aligned offsets, so one copy uploads the
+    // tables and clears the status before the first kernel
     size_t vbytes = (hv.size() * sizeof(DevView) + 255) & ~(size_t)255;
-    size_t dbytes = std::max<size_t>(hd.size(), 1) * sizeof(DevDraw);
+    size_t dbytes = (std::max<size_t>(hd.size(), 1) * sizeof(DevDraw) + 255) & ~(size_t)255;
+    size_t upload = vbytes + dbytes + sizeof(DevStatus);
     int r;
-    if ((r = stage.ensure(vbytes + dbytes))) return r;
+    if ((r = stage.ensure(upload))) return r;
     memcpy(stage.ptr, hv.data(), hv.size() * sizeof(DevView));
     if (!hd.empty()) memcpy((char *)stage.ptr + vbytes, hd.data(), hd.size() * sizeof(DevDraw));
-    if ((r = views.ensure(vbytes + dbytes))) return r;
+    memset((char *)stage.ptr + vbytes + dbytes, 0, sizeof(DevStatus));
+    if ((r = views.ensure(upload))) return r;
     if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 48))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
     if ((r = tri_counts.ensure(((size_t)total_in + 1) * 4))) return r;
@@ -342,7 +344,7 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.clip_queue = (uint32_t *)clip_queue.ptr;
         a.clip_buf = (float *)clip_buf.ptr;
         a.clip_cap = clip_cap;
-        a.status = (DevStatus *)status.ptr;
+        a.status = (DevStatus *)((char *)views.ptr + vbytes + dbytes);
         a.out_ids = out_ids;
         a.out_depth = out_depth;
         a.out_color = out_color;
@@ -367,12 +369,12 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.hiz = tile_size == 16 && last_pairs > 32ull * total_items && last_pairs > 4ull * last_records;
         last_hiz = a.hiz != 0;
         if (use_graph) {
-            if ((r = launch_graph(a, vbytes, dbytes))) return r;
+            if ((r = launch_graph(a, vbytes, upload))) return r;
         } else {
-            HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, vbytes + dbytes, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, upload, hipMemcpyHostToDevice, stream));
             for (int i = 0; i < 5; i++) a.events[i] = ev[i];
             HIP_TRY(launch_frame(a, stream));
-            HIP_TRY(hipMemcpyAsync(status_host.ptr, status.ptr, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipMemcpyAsync(status_host.ptr, a.status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
         }
         if (!sync) return SR_OK;
         HIP_TRY(hipStreamSynchronize(stream));

@@ -60,7 +60,7 @@ struct Pipeline {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     DevBuf views, xv, xt, tri_counts, tri_chunks, tri_base, bin_count, bin_start, bin_end, partials, list;
-    DevBuf rec_geo, rec_raster, status, center, clip_queue, clip_buf;
+    DevBuf rec_geo, rec_raster, center, clip_queue, clip_buf;
     DevBuf tr_count, tr_start, tr_end, tr_list, tr_scratch, center_color;
     PinBuf stage, status_host;
     uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0, tr_cap = 0;
@@ -74,7 +74,7 @@ struct Pipeline {
     int init(int device, int tile);
     void destroy();
     void drop_graph();
-    int launch_graph(const FrameArgs &a, size_t vbytes, size_t dbytes);
+    int launch_graph(const FrameArgs &a, size_t vbytes, size_t upload);
     int run(const std::vector<DevView> &views, const std::vector<DevDraw> &draws, uint32_t total_xv,
             uint32_t total_in, uint32_t total_items, bool any_tangents, uint32_t *out_ids, float *out_depth,
             float *out_color, uint64_t color_plane, uint32_t *out_rgba, float *out_lum, int persistent_blocks,
