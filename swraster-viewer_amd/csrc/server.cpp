@@ -385,8 +385,11 @@ void sr_server::run() {
             cv.wait(lk, [&] { return stop.load() || gen != seen; });
             seen = gen;
         } else {
+            // batches in flight: poll their events, yielding the core between
+            // polls (a timed sleep overshoots by tens of microseconds, a
+            // sizeable part of a frame)
             lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            std::this_thread::yield();
             lk.lock();
         }
     }
