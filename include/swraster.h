@@ -441,9 +441,10 @@ SR_API int sr_renderer_device_buffers(sr_renderer *renderer, void **tri_id, void
 /* ---- batched views (one launch sequence for many views) ------------------ */
 SR_API int sr_batch_create(const sr_batch_opts *opts, sr_batch **out);
 SR_API int sr_batch_destroy(sr_batch *batch);
-/* Changes the persistent raster workgroup count (> 0; 0 keeps it) and hipGraph
- * use of an existing batch; the tile size is fixed at creation.  Used by the
- * autotuner (no reference counterpart). */
+/* Changes the persistent raster workgroup count (> 0; 0 keeps it; -1 launches
+ * one raster workgroup per (view, tile) instead) and hipGraph use of an
+ * existing batch; the tile size is fixed at creation.  Used by the autotuner
+ * (no reference counterpart). */
 SR_API int sr_batch_configure(sr_batch *batch, int32_t persistent_blocks, int32_t use_graph);
 /* Renders every view; per-view outputs are written to device memory owned
  * by the batch and read back with sr_batch_read_*. */

@@ -1,7 +1,8 @@
 """Autotuner for the batched path.
 
 Picks the screen-tile size (8, 16 or 32 px) and the number of persistent
-raster workgroups for a set of views by timing the real batch, and keeps the
+raster workgroups (or -1: one raster workgroup per (view, tile), which wins
+for few large views) for a set of views by timing the real batch, and keeps the
 choice in a JSON cache keyed by device and a coarse workload signature (view
 count, log2 of total pixels and of total triangles).  Numerics do not depend
 on either knob; only time does.  The bin lists hold one 4-byte entry per
@@ -15,7 +16,7 @@ import time
 
 CACHE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "assets", ".cache", "autotune.json")
 TILES = (8, 16, 32)
-WORKGROUPS = (1024, 2048, 4096)     # 4, 8 and 16 per CU on a 256-CU MI355X
+WORKGROUPS = (-1, 1024, 2048, 4096)     # grid launch; 4, 8 and 16 per CU on a 256-CU MI355X
 
 
 def signature(views):

