@@ -144,4 +144,5 @@ struct DevStatus {
     uint32_t clip_overflow;       // a clipped polygon outgrew its room (frame fails)
     uint32_t total_tr_pairs;      // (translucent triangle, 64-px tile) pairs
     uint32_t overflow_tr;
+    uint32_t seq;                 // the frame's number, set by the host in the upload
 };

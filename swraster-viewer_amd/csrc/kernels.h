@@ -36,6 +36,7 @@ struct FrameArgs {
     float *clip_buf;              // clip_cap * 16 vertices * 16 channels
     uint32_t clip_cap;
     DevStatus *status;
+    DevStatus *status_out;        // pinned host copy of the status, stored by k_shade
     // outputs (per view at DevView.fb_offset)
     uint32_t *out_ids;
     float *out_depth;

@@ -1553,6 +1553,7 @@ struct RasterArgs {
     SR_GLOBAL uint32_t *center_ids;         // 4 winners per 64px tile: its centre quad, on screen or not
     SR_GLOBAL float4 *center_color;         // 4 (colour, depth) per 64px tile for the translucent pass (or null)
     SR_GLOBAL DevStatus *status;
+    DevStatus *status_out;                  // pinned host memory for the frame's status
     int32_t persistent;
 };
 
@@ -2093,7 +2094,17 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
 // looping over four pixels each).
 template <int T, int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T * T > NT ? 4 : 5))) k_shade(RasterArgs P) {
+    // the frame's status is final once shading starts (no later kernel
+    // writes it): the first workgroup stores it in the host's pinned copy
+    // with system-scope stores (after its pixels, where no register is
+    // live), so no read-back copy follows the frame
     shade_item<T, NT>(P, blockIdx.x);
+    constexpr uint32_t STATUS_WORDS = sizeof(DevStatus) / 4;
+    static_assert(STATUS_WORDS <= NT, "one thread per status word");
+    if (blockIdx.x == 0 && threadIdx.x < STATUS_WORDS && P.status_out)
+        __hip_atomic_store(reinterpret_cast<uint32_t *>(P.status_out) + threadIdx.x,
+                           reinterpret_cast<const SR_GLOBAL uint32_t *>(P.status)[threadIdx.x], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---------------------------------------------------------------------------
@@ -2668,6 +2679,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     P.out_depth = (SR_GLOBAL float *)a.out_depth; P.out_color = (SR_GLOBAL float *)a.out_color;
     P.color_plane = a.color_plane; P.out_rgba = (SR_GLOBAL uint32_t *)a.out_rgba;
     P.out_lum = (SR_GLOBAL float *)a.out_lum; P.status = (SR_GLOBAL DevStatus *)a.status;
+    P.status_out = a.status_out;
     P.center_ids = (SR_GLOBAL uint32_t *)a.center_ids;
     P.center_color = (SR_GLOBAL float4 *)(a.has_translucent ? a.center_color : nullptr);
     P.persistent = a.persistent_blocks > 0;

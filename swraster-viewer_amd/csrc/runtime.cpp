@@ -64,13 +64,13 @@ void DevBuf::release() {
     ptr = nullptr;
     bytes = 0;
 }
-int PinBuf::ensure(size_t need) {
+int PinBuf::ensure(size_t need, unsigned flags) {
     if (need <= bytes && ptr) return SR_OK;
     if (ptr) hipHostFree(ptr);
     ptr = nullptr;
     bytes = 0;
     size_t alloc = std::max<size_t>(need, 256);
-    hipError_t e = hipHostMalloc(&ptr, alloc, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&ptr, alloc, flags);
     if (e != hipSuccess) {
         ptr = nullptr;
         return fail(SR_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", alloc, hipGetErrorString(e));
@@ -196,7 +196,8 @@ int Pipeline::init(int dev, int tile) {
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
-    return status_host.ensure(sizeof(DevStatus));
+    // coherent: the shading kernel stores the frame's status here
+    return status_host.ensure(sizeof(DevStatus), hipHostMallocCoherent);
 }
 
 void Pipeline::destroy() {
@@ -246,8 +247,6 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t upload) {
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipMemcpyAsync((void *)a.views, stage.ptr, upload, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = launch_frame(a, stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(status_host.ptr, a.status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(stream, &g);
         if (e == hipSuccess) e = ce;
@@ -280,7 +279,10 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     if ((r = stage.ensure(upload))) return r;
     memcpy(stage.ptr, hv.data(), hv.size() * sizeof(DevView));
     if (!hd.empty()) memcpy((char *)stage.ptr + vbytes, hd.data(), hd.size() * sizeof(DevDraw));
-    memset((char *)stage.ptr + vbytes + dbytes, 0, sizeof(DevStatus));
+    DevStatus init;
+    memset(&init, 0, sizeof(init));
+    init.seq = ++seq;
+    memcpy((char *)stage.ptr + vbytes + dbytes, &init, sizeof(init));
     if ((r = views.ensure(upload))) return r;
     if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 48))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
@@ -345,6 +347,8 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.clip_buf = (float *)clip_buf.ptr;
         a.clip_cap = clip_cap;
         a.status = (DevStatus *)((char *)views.ptr + vbytes + dbytes);
+        a.status_out = (DevStatus *)status_host.ptr;
+        status_dev = a.status;
         a.out_ids = out_ids;
         a.out_depth = out_depth;
         a.out_color = out_color;
@@ -374,7 +378,6 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
             HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, upload, hipMemcpyHostToDevice, stream));
             for (int i = 0; i < 5; i++) a.events[i] = ev[i];
             HIP_TRY(launch_frame(a, stream));
-            HIP_TRY(hipMemcpyAsync(status_host.ptr, a.status, sizeof(DevStatus), hipMemcpyDeviceToHost, stream));
         }
         if (!sync) return SR_OK;
         HIP_TRY(hipStreamSynchronize(stream));
@@ -385,8 +388,16 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     return fail(SR_ERR_CAPACITY, "frame buffers did not converge after reallocation");
 }
 
+const DevStatus &Pipeline::frame_status() {
+    DevStatus *h = (DevStatus *)status_host.ptr;
+    if (h->seq != seq && status_dev) {
+        if (hipMemcpy(h, status_dev, sizeof(DevStatus), hipMemcpyDeviceToHost) != hipSuccess) (void)hipGetLastError();
+    }
+    return *h;
+}
+
 int Pipeline::settle(bool has_tr, bool &again) {
-    const DevStatus &s = *(const DevStatus *)status_host.ptr;
+    const DevStatus &s = frame_status();
     last_records = s.total_records;
     last_pairs = s.total_pairs;
     again = false;

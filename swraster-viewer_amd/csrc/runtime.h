@@ -27,7 +27,7 @@ struct DevBuf {
 struct PinBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
-    int ensure(size_t need);
+    int ensure(size_t need, unsigned flags = hipHostMallocDefault);
     void release();
 };
 
@@ -82,6 +82,12 @@ struct Pipeline {
     // After the stream has drained a frame: records its counts, grows what
     // overflowed and sets again when the frame must be run once more.
     int settle(bool has_translucent, bool &again);
+    // The last frame's status after its stream has drained: the copy the
+    // shading kernel stored in status_host, or, if that copy's frame number
+    // does not match, a copy read from the device.
+    const DevStatus &frame_status();
+    uint32_t seq = 0;                 // number of the last frame launched
+    DevStatus *status_dev = nullptr;  // its status on the device
 };
 
 }  // namespace sr

@@ -345,7 +345,7 @@ bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
 void sr_server::retire(Slot &s, std::unique_lock<std::mutex> &lk) {
     // a capacity overflow observed after the fact: re-run this batch
     // synchronously (the pipeline grows its buffers and repeats the frame)
-    const DevStatus &st = *(const DevStatus *)s.pipe.status_host.ptr;
+    const DevStatus &st = s.pipe.frame_status();
     int rc = SR_OK;
     bool tr_over = s.color && (st.overflow_tr || st.total_tr_pairs > s.pipe.tr_cap);
     if (st.overflow_records || st.overflow_pairs || st.overflow_clip || st.total_records > s.pipe.rec_cap ||
