@@ -269,7 +269,11 @@ def test_determinism(sr, scenes, c1):
     assert np.array_equal(a["px"], b["px"])
 
 
-def test_batch_matches_plain(sr, scenes, c1):
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_batch_matches_plain(sr, scenes, c1, tile):
+    """Views of several sizes in one batch, at each screen-tile size, with
+    the persistent raster and with one raster workgroup per (view, tile):
+    every view is the plain renderer's frame bit for bit."""
     scene, spec, gscene = c1
     views, plain = [], []
     for i, (w, h) in enumerate([(320, 240), (640, 480), (1280, 720), (200, 150)]):
@@ -278,7 +282,7 @@ def test_batch_matches_plain(sr, scenes, c1):
         cam = scenes.make_camera(s, w, h)
         views.append((gscene, cam, w, h, 2.0))
         plain.append(render_gpu(sr, gscene, cam, w, h))
-    b = sr.Batch()
+    b = sr.Batch(tile_size=tile)
     for persistent in (0, -1):          # the default persistent raster, then one workgroup per item
         b.configure(persistent_blocks=persistent)
         b.render(views)
@@ -286,6 +290,7 @@ def test_batch_matches_plain(sr, scenes, c1):
             ids, depth = b.read_vbuffer(i)
             assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"]))
             assert np.array_equal(b.read_pixels(i), p["px"])
+    b.close()
 
 
 def test_batch_with_translucent_views_matches_plain(sr, scenes, env, c1):
