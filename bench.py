@@ -24,6 +24,11 @@ Prints one JSON line:
                  traffic: HBM bytes per launch measured with rocprofv3
                  FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a
                  device copy of known size (profiles/traffic_<config>.json)
+                 valu: the kernel's VALU issue time (SQ_INSTS_VALU per launch,
+                 profiles/valu_<config>.json, x 4 cycles / 1024 SIMDs /
+                 2400 MHz) over its live launch time: the bound that holds
+                 for these kernels, whose IEEE divisions and square roots
+                 keep them on the VALU rather than on HBM
   frame_roofline the whole frame with its algorithmic bytes (vertices,
                  indices, pixels, texels)
   cpu_baseline   the CPU oracle in reference-like threaded mode (128-triangle
@@ -422,6 +427,19 @@ def main():
             tr = json.load(fh).get("bytes_per_launch", {})
         if dom in tr:
             out["roofline"]["traffic"] = tr[dom]
+    # the kernel's VALU issue bound: its VALU instructions per launch
+    # (SQ_INSTS_VALU, committed profile) x 4 cycles per wave64 instruction
+    # over 1024 SIMDs at 2400 MHz, against its live launch time
+    valu_path = os.path.join(REPO, "profiles", f"valu_{args.config}.json")
+    if os.path.exists(valu_path):
+        with open(valu_path) as fh:
+            vj = json.load(fh)
+        e = vj.get("per_launch", {}).get(dom)
+        if e:
+            issue_us = e["valu_insts"] * 4.0 / vj["simds"] / vj["clock_mhz"]
+            out["roofline"]["valu"] = {"insts_per_launch": e["valu_insts"], "issue_us": round(issue_us, 2),
+                                       "frac": round(issue_us / (k_ms * 1e3), 3),
+                                       "peak": "614.4 G wave64 VALU instructions/s", "source": vj["source"]}
 
     if not args.no_cpu_baseline:
         orc = entry.load_oracle()
