@@ -1994,6 +1994,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32
 // skybox, colour planes or the fused tonemapped RGBA8, and the centre-quad
 // luminance of each 64px tile (src/tilerasterizer.rs:384-476).
 // ---------------------------------------------------------------------------
+// Value of lane K of this lane's quad (lanes 4q..4q+3 of the wave), by a DPP
+// quad permutation (one VALU move, no LDS round trip).  All four lanes of
+// the quad must be active.
+template <int K> __device__ __forceinline__ float quad_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+
 template <int T, int NT>
 __device__ void shade_item(const RasterArgs &P, uint32_t item) {
     constexpr int QW = T / 2;                 // quads per row
@@ -2034,12 +2041,9 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
             float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
             z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
         }
-        float qb1[4], qb2[4];
-        int qbase = lane & ~3;
-        for (int k = 0; k < 4; k++) {
-            qb1[k] = __shfl(b1, qbase + k, 64);
-            qb2[k] = __shfl(b2, qbase + k, 64);
-        }
+        // every lane of the wave is active here (one thread per pixel)
+        float qb1[4] = {quad_lane<0>(b1), quad_lane<1>(b1), quad_lane<2>(b1), quad_lane<3>(b1)};
+        float qb2[4] = {quad_lane<0>(b2), quad_lane<1>(b2), quad_lane<2>(b2), quad_lane<3>(b2)};
         f3 color = mk3(0.0f, 0.0f, 0.0f);
         if (on_screen || center) {
             bool covered = has && z != INFINITY;
@@ -2077,8 +2081,7 @@ __device__ void shade_item(const RasterArgs &P, uint32_t item) {
         }
         // centre-quad luminance of the 64px tile (src/tilerasterizer.rs:103-106)
         float lum = (color.x * 0.2126f + color.y * 0.7152f) + color.z * 0.0722f;
-        float l0 = __shfl(lum, qbase, 64), l1 = __shfl(lum, qbase + 1, 64);
-        float l2 = __shfl(lum, qbase + 2, 64), l3 = __shfl(lum, qbase + 3, 64);
+        float l0 = quad_lane<0>(lum), l1 = quad_lane<1>(lum), l2 = quad_lane<2>(lum), l3 = quad_lane<3>(lum);
         if (center && s == 0) {
             int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
             P.out_lum[V.lum_offset + (uint32_t)ri] = (((l0 + l1) + l2) + l3) * 0.25f;
