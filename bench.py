@@ -117,7 +117,7 @@ def run_batched(args):
     bviews = [(scenes[k], cam, spec["width"], spec["height"], 0.0) for (k, spec), cam in zip(views, cams)]
     import swraster_viewer_amd.autotune as autotune
     if args.tile:
-        tuned = {"tile_size": args.tile, "persistent_blocks": 2048, "cached": None}
+        tuned = {"tile_size": args.tile, "persistent_blocks": -1, "cached": None}   # one raster workgroup per item
     else:
         tuned = autotune.tune(sr, bviews, reps=1)
     batch = sr.Batch(tile_size=tuned["tile_size"])
@@ -313,7 +313,8 @@ def main():
     ap.add_argument("--requests", type=int, default=240, help="c5: requests per rate")
     ap.add_argument("--views", type=int, default=256, help="c4: views per batch")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch without a hipGraph")
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="tile size (c4: skips the autotuner, one raster workgroup per (view, tile))")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")

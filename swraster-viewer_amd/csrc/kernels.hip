@@ -124,6 +124,21 @@ __device__ __forceinline__ int find_last_le(const T *arr, int n, uint32_t key, u
     return lo;
 }
 
+// find_last_le where the active lanes' keys increase with the lane index
+// (consecutive triangles or vertices): one wave-uniform search for the first
+// active lane's key answers for every lane when the last active lane's key
+// still lies before the next entry, which is the usual case; otherwise each
+// lane searches.
+template <class T>
+__device__ __forceinline__ int find_last_le_consecutive(const T *arr, int n, uint32_t key, uint32_t T::*field) {
+    uint64_t act = __ballot(1);
+    uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
+    uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)key, 63 - __clzll((long long)act));
+    int d0 = find_last_le(arr, n, k0, field);
+    if (d0 + 1 >= n || arr[d0 + 1].*field > k1) return d0;
+    return find_last_le(arr, n, key, field);
+}
+
 // ---------------------------------------------------------------------------
 // Clip-space vertex with attributes (Vertex, src/renderer.rs:30-37)
 // ---------------------------------------------------------------------------
@@ -259,7 +274,7 @@ __device__ __forceinline__ void transform_vertices(const DevView *views, const D
                                                    uint32_t total, float4 *xp, float4 *xa, float4 *xt, uint32_t b,
                                                    uint32_t nb) {
     for (uint32_t i = b * blockDim.x + threadIdx.x; i < total; i += nb * blockDim.x) {
-        int d = find_last_le(draws, ndraws, i, &DevDraw::xv_base);
+        int d = find_last_le_consecutive(draws, ndraws, i, &DevDraw::xv_base);
         int du = wave_uniform(d);
         if (du >= 0) {
             const DevDraw &D = draws[du];
@@ -317,7 +332,7 @@ __device__ __forceinline__ void classify_triangles(const DevView *views, const D
         uint32_t t = base + threadIdx.x;
         bool q = false, in = false;
         if (t < total) {
-            int d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+            int d = find_last_le_consecutive(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
             int cls = du >= 0 ? classify_input_triangle(draws[du], views, t)
                               : classify_input_triangle(draws[d], views, t);
@@ -1069,7 +1084,7 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
         if (t < total) {
             tri_base[t] = id;
             if (t == total - 1) tri_base[total] = id + nf;
-            d = find_last_le(draws, ndraws, t, &DevDraw::in_tri_base);
+            d = find_last_le_consecutive(draws, ndraws, t, &DevDraw::in_tri_base);
             int du = wave_uniform(d);
             if (du >= 0) {
                 view = draws[du].view;
