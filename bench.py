@@ -18,8 +18,10 @@ Prints one JSON line:
                      records, read once) + 4 B per bin entry + 4 B per pixel
                      (triangle id written)
                    k_shade: 20 B per pixel (id read, depth and three colour
-                     planes written) + 192 B per visible triangle (raster +
-                     shading records) + 4 B per covered pixel of a textured
+                     planes written) + 224 B per visible triangle (16 B geometry
+                     and 64 B raster records, three vertices' transformed
+                     position 16 B and attributes 32 B) + 4 B per covered
+                     pixel of a textured
                      material (one base-colour texel)
                  traffic: HBM bytes per launch measured with rocprofv3
                  FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a
@@ -58,7 +60,7 @@ def algorithmic_bytes(scene, w, h, n_cov, setup_tris, pairs, visible_tris):
     texels = 4 * n_cov if textured else 0
     frame = per_vertex * nv + 12 * nt + 12 * w * h + texels
     kernels = {"k_raster": 80 * setup_tris + 4 * pairs + 4 * w * h,
-               "k_shade": 20 * w * h + 192 * visible_tris + texels}
+               "k_shade": 20 * w * h + 224 * visible_tris + texels}
     return frame, kernels
 
 
