@@ -220,7 +220,7 @@ def run_served(args):
         renderers[(w, h)] = sr.Renderer(w, h)
         targets[(w, h)] = C.c_void_p()
         assert hip.hipMalloc(C.byref(targets[(w, h)]), w * h * 4) == 0
-    server = sr.Server(admission_bytes=64 << 30, max_batch_views=args.max_batch, streams=2)
+    server = sr.Server(admission_bytes=64 << 30, max_batch_views=args.max_batch, streams=2, tile_size=args.tile)
     idx = [server.add_scene(s) for s in scenes]
 
     def plain(reqs):
@@ -316,7 +316,7 @@ def main():
     ap.add_argument("--views", type=int, default=256, help="c4: views per batch")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch without a hipGraph")
     ap.add_argument("--tile", type=int, default=0,
-                    help="tile size (c4: skips the autotuner, one raster workgroup per (view, tile))")
+                    help="tile size (c4: skips the autotuner, one raster workgroup per (view, tile); c5: the server's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
