@@ -945,6 +945,40 @@ struct Tile {
     float center_luminance = 1.0f;
 };
 
+// Renderer::update_auto_exposure (src/renderer.rs:258-290) on the tiles'
+// centre luminances: trimmed mean of their log2 (10% cut at each end of the
+// f32::total_cmp order), the target exposure that maps it to the metered
+// grey, and the exponential step of ev towards it (tau = 1 s).  Updates ev
+// and target; returns the exposure 2^ev.
+static float exposure_step(const float *lum, size_t n, float dt, float &ev, float &target_out) {
+    std::vector<float> s(n);
+    for (size_t i = 0; i < n; i++) s[i] = std::log2(std::fmax(lum[i], 1e-4f));
+    // f32::total_cmp order
+    auto key = [](float f) {
+        int32_t b; memcpy(&b, &f, 4);
+        return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
+    };
+    std::sort(s.begin(), s.end(), [&](float a, float b) { return key(a) < key(b); });
+    size_t trim = (size_t)std::floor((float)n * 0.10f);
+    trim = std::min(trim, (n - 1) / 2);
+    float sum = 0.0f;
+    for (size_t i = trim; i < n - trim; i++) sum += s[i];
+    float mean = sum / (float)(n - 2 * trim);
+    // tonemap_inverse_scalar(0.45) (src/util.rs:43-47)
+    float y = std::fmin(std::fmax(0.45f, 0.0f), 1.0f);
+    float k = 0.2f;
+    float denom = std::fmax(1.0f + k - y, 1e-6f);
+    float meter = std::fmax((y * k) / denom, 1e-4f);
+    float target_ev = std::log2(meter) - mean;
+    float target = std::fmin(std::fmax(std::pow(2.0f, target_ev), 0.05f), 32.0f);
+    target_out = target;
+    float tev = std::log2(target);
+    float tau = std::fmax(1.0f, 1e-4f);
+    float alpha = 1.0f - std::exp(-(std::fmax(dt, 0.0f) / tau));
+    ev += (tev - ev) * alpha;
+    return std::pow(2.0f, ev);
+}
+
 struct Renderer;
 static void render_tile(Renderer &r, Tile &t, const Scene &s, const sr_camera &c, bool fp64);
 
@@ -1265,32 +1299,9 @@ struct Renderer {
     void update_auto_exposure(float dt) {
         size_t n = tiles.size();
         if (n == 0) return;
-        std::vector<float> s(n);
-        for (size_t i = 0; i < n; i++) s[i] = std::log2(std::fmax(tiles[i]->center_luminance, 1e-4f));
-        // f32::total_cmp order
-        auto key = [](float f) {
-            int32_t b; memcpy(&b, &f, 4);
-            return b ^ (int32_t)(((uint32_t)(b >> 31)) >> 1);
-        };
-        std::sort(s.begin(), s.end(), [&](float a, float b) { return key(a) < key(b); });
-        size_t trim = (size_t)std::floor((float)n * 0.10f);
-        trim = std::min(trim, (n - 1) / 2);
-        float sum = 0.0f;
-        for (size_t i = trim; i < n - trim; i++) sum += s[i];
-        float mean = sum / (float)(n - 2 * trim);
-        // tonemap_inverse_scalar(0.45) (src/util.rs:43-47)
-        float y = std::fmin(std::fmax(0.45f, 0.0f), 1.0f);
-        float k = 0.2f;
-        float denom = std::fmax(1.0f + k - y, 1e-6f);
-        float meter = std::fmax((y * k) / denom, 1e-4f);
-        float target_ev = std::log2(meter) - mean;
-        float target = std::fmin(std::fmax(std::pow(2.0f, target_ev), 0.05f), 32.0f);
-        auto_exposure_target = target;
-        float tev = std::log2(auto_exposure_target);
-        float tau = std::fmax(1.0f, 1e-4f);
-        float alpha = 1.0f - std::exp(-(std::fmax(dt, 0.0f) / tau));
-        auto_exposure_ev += (tev - auto_exposure_ev) * alpha;
-        auto_exposure = std::pow(2.0f, auto_exposure_ev);
+        std::vector<float> lum(n);
+        for (size_t i = 0; i < n; i++) lum[i] = tiles[i]->center_luminance;
+        auto_exposure = exposure_step(lum.data(), n, dt, auto_exposure_ev, auto_exposure_target);
     }
 
     void blit(uint32_t *pixels, size_t bw, size_t bh) const {
@@ -1552,8 +1563,10 @@ static V3x4<T> pbr_shader(const Scene &s, const sr_camera &cam, const RasterPack
     return color;
 }
 
+// The four normalized view directions of the quad at pix (compute_skybox,
+// src/tilerasterizer.rs:478-505, before the cubemap lookup).
 template <class T>
-static V3x4<T> compute_skybox(const Scene &s, const sr_camera &cam, IVec2 pix) {
+static V3x4<T> skybox_direction(const sr_camera &cam, IVec2 pix) {
     V4<T> px = {{(T)((float)pix.x + 0.5f), (T)((float)pix.x + 1.5f), (T)((float)pix.x + 0.5f), (T)((float)pix.x + 1.5f)}};
     V4<T> py = {{(T)((float)pix.y + 0.5f), (T)((float)pix.y + 0.5f), (T)((float)pix.y + 1.5f), (T)((float)pix.y + 1.5f)}};
     V4<T> one = V4<T>::splat(1);
@@ -1566,7 +1579,12 @@ static V3x4<T> compute_skybox(const Scene &s, const sr_camera &cam, IVec2 pix) {
         const float *row = m + 4 * r;  // column r of the transposed matrix
         *out[r] = ((ndc_x * (T)row[0] + ndc_y * (T)row[1]) + one * (T)row[2]) + (T)row[3];
     }
-    V3x4<T> n = normalize(d);
+    return normalize(d);
+}
+
+template <class T>
+static V3x4<T> compute_skybox(const Scene &s, const sr_camera &cam, IVec2 pix) {
+    V3x4<T> n = skybox_direction<T>(cam, pix);
     return srgb_to_linear_fast(sample_cubemap_rgb(s.cubemap, n, usplat(0)));
 }
 
@@ -1898,6 +1916,73 @@ __attribute__((visibility("default"))) int orc_cubemap_uv(const float n[12], flo
     U4 s;
     cubemap_uv_from_normal(v, u, vv, s);
     for (int i = 0; i < 4; i++) { uv[2 * i] = u[i]; uv[2 * i + 1] = vv[i]; face[i] = s[i]; }
+    return 0;
+}
+
+// pbr_shader on one packet with constant interpolators: every vertex has
+// normal n, world position p (1/w = 1) and texture coordinate uv; du_dv is
+// the packet's UV derivative term.  All four lanes shade the same point;
+// lane 0's colour in f32 and in the f64 instantiation.
+__attribute__((visibility("default"))) int orc_shade_probe(void *sp, const sr_camera *cam, int material,
+                                                           const float n[3], const float p[3], const float uv[2],
+                                                           const float du_dv[4], float out32[3], double out64[3]) {
+    const Scene &s = *(const Scene *)sp;
+    if (material < 0 || material >= (int)s.materials.size()) return -1;
+    RasterPacket pk;
+    memset(&pk, 0, sizeof(pk));
+    Vec3 nn = v3(n[0], n[1], n[2]), pp = v3(p[0], p[1], p[2]);
+    pk.normals = vinterp(nn, nn, nn);
+    pk.tangents = vinterp(v3(1, 0, 0), v3(1, 0, 0), v3(1, 0, 0));
+    pk.tangent_sign = sinterp(1.0f, 1.0f, 1.0f);
+    pk.one_over_w = sinterp(1.0f, 1.0f, 1.0f);
+    pk.z_over_w = sinterp(0.5f, 0.5f, 0.5f);
+    pk.pos_world_over_w = vinterp(pp, pp, pp);
+    pk.u_over_w = sinterp(uv[0], uv[0], uv[0]);
+    pk.v_over_w = sinterp(uv[1], uv[1], uv[1]);
+    for (int i = 0; i < 4; i++) pk.du_dv[i] = du_dv[i];
+    const Material &m = s.materials[material];
+    V4<float> z = V4<float>::splat(0.0f);
+    V3x4<float> c = pbr_shader<float>(s, *cam, pk, m, z, z, false, V3x4<float>::splat(0));
+    V4<double> zd = V4<double>::splat(0.0);
+    V3x4<double> cd = pbr_shader<double>(s, *cam, pk, m, zd, zd, false, V3x4<double>::splat(0));
+    out32[0] = c.x[0]; out32[1] = c.y[0]; out32[2] = c.z[0];
+    out64[0] = cd.x[0]; out64[1] = cd.y[0]; out64[2] = cd.z[0];
+    return 0;
+}
+
+// compute_skybox's four view directions for the quad at (x, y) (x, y even):
+// dir[lane * 3 + axis].
+__attribute__((visibility("default"))) int orc_skybox_dir(const sr_camera *cam, int x, int y, float dir[12]) {
+    V3x4<float> d = skybox_direction<float>(*cam, IVec2{x, y});
+    for (int l = 0; l < 4; l++) { dir[3 * l] = d.x[l]; dir[3 * l + 1] = d.y[l]; dir[3 * l + 2] = d.z[l]; }
+    return 0;
+}
+
+// The scene's environment samplers on one lane: which = 0 the specular
+// cubemap trilinearly at direction a (mip level `level`), 1 the BRDF LUT
+// bilinearly at (u, v) = (a[0], a[1]) with clamp-to-edge (as pbr_shader
+// reads it), 2 the skybox cubemap (nearest, mip 0) at direction a.
+__attribute__((visibility("default"))) int orc_env_sample(void *sp, int which, const float a[3], float level,
+                                                          float out[3]) {
+    const Scene &s = *(const Scene *)sp;
+    V3x4<float> d = V3x4<float>::from3(a[0], a[1], a[2]), c;
+    if (which == 0) c = sample_cubemap_trilinear_rgb(s.specular, d, V4<float>::splat(level));
+    else if (which == 1)
+        c = sample_bilinear_rgb(s.lut, SR_WRAP_CLAMP_TO_EDGE, SR_WRAP_CLAMP_TO_EDGE, V4<float>::splat(a[0]),
+                                V4<float>::splat(a[1]), 0, usplat(0));
+    else if (which == 2) c = sample_cubemap_rgb(s.cubemap, d, usplat(0));
+    else return -1;
+    out[0] = c.x[0]; out[1] = c.y[0]; out[2] = c.z[0];
+    return 0;
+}
+
+// The auto-exposure step on given tile luminances, from (ev, target);
+// state = {ev, target, exposure} after the step.
+__attribute__((visibility("default"))) int orc_exposure_step(const float *lum, size_t n, float dt, float ev,
+                                                             float target, float state[3]) {
+    if (n == 0) return -1;
+    float e = exposure_step(lum, n, dt, ev, target);
+    state[0] = ev; state[1] = target; state[2] = e;
     return 0;
 }
 

@@ -44,6 +44,10 @@ def lib():
         L.orc_quantize.argtypes = [C.c_float, C.c_float]
         L.orc_mip_chain.argtypes = [P, P, P]
         L.orc_sort_translucent.argtypes = [P, P, C.c_size_t]
+        L.orc_shade_probe.argtypes = [P, P, C.c_int, P, P, P, P, P, P]
+        L.orc_skybox_dir.argtypes = [P, C.c_int, C.c_int, P]
+        L.orc_env_sample.argtypes = [P, C.c_int, P, C.c_float, P]
+        L.orc_exposure_step.argtypes = [P, C.c_size_t, C.c_float, C.c_float, C.c_float, P]
         _lib = L
     return _lib
 
@@ -152,3 +156,47 @@ def sort_translucent(avg_z):
     order = np.empty(len(z), np.uint32)
     lib().orc_sort_translucent(_p(z), _p(order), len(z))
     return order
+
+
+def _f32(a, n):
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    assert a.size == n
+    return a
+
+
+def shade_probe(scene, camera, material, normal, pos, uv=(0.0, 0.0), du_dv=(0.0, 0.0, 0.0, 0.0)):
+    """pbr_shader at one point of a packet with constant interpolators
+    (normal, world position with 1/w = 1, uv): (f32 rgb, f64 rgb)."""
+    n, p, t, d = _f32(normal, 3), _f32(pos, 3), _f32(uv, 2), _f32(du_dv, 4)
+    o32 = np.empty(3, np.float32)
+    o64 = np.empty(3, np.float64)
+    rc = lib().orc_shade_probe(scene.handle, C.addressof(camera.c), int(material), _p(n), _p(p), _p(t), _p(d),
+                               _p(o32), _p(o64))
+    assert rc == 0
+    return o32, o64
+
+
+def skybox_dir(camera, x, y):
+    """compute_skybox's normalized view directions of the quad at (x, y): (4, 3)."""
+    out = np.empty(12, np.float32)
+    lib().orc_skybox_dir(C.addressof(camera.c), int(x), int(y), _p(out))
+    return out.reshape(4, 3)
+
+
+def env_sample(scene, which, a, level=0.0):
+    """which: 'specular' (trilinear at direction a, mip level), 'lut'
+    (bilinear, clamp-to-edge, at (a[0], a[1])), 'sky' (nearest, mip 0)."""
+    k = {"specular": 0, "lut": 1, "sky": 2}[which]
+    v = np.zeros(3, np.float32)
+    v[: len(a)] = a
+    out = np.empty(3, np.float32)
+    assert lib().orc_env_sample(scene.handle, k, _p(v), float(level), _p(out)) == 0
+    return out
+
+
+def exposure_step(lum, dt, ev, target):
+    """The auto-exposure step on tile luminances: (ev, target, exposure)."""
+    l = np.ascontiguousarray(lum, np.float32)
+    st = np.empty(3, np.float32)
+    assert lib().orc_exposure_step(_p(l), l.size, float(dt), float(ev), float(target), _p(st)) == 0
+    return float(st[0]), float(st[1]), float(st[2])
