@@ -2019,6 +2019,9 @@ template <int K> __device__ __forceinline__ float quad_lane(float v) {
 template <int T, int NT>
 __device__ void shade_item(const RasterArgs &P, uint32_t item) {
     constexpr int QW = T / 2;                 // quads per row
+    // the DPP quad exchanges below read all four lanes of each quad: every
+    // lane must be active, so the pixel loop has no partial wave
+    static_assert((T * T) % NT == 0 && NT % 64 == 0, "one thread per pixel in whole waves");
     const int tid = threadIdx.x, lane = tid & 63;
     const DevView &V = P.views[find_view(P, item)];
     const DevScene &S = *V.scene;

@@ -343,13 +343,13 @@ bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
 }
 
 void sr_server::retire(Slot &s, std::unique_lock<std::mutex> &lk) {
-    // a capacity overflow observed after the fact: re-run this batch
-    // synchronously (the pipeline grows its buffers and repeats the frame)
-    const DevStatus &st = s.pipe.frame_status();
-    int rc = SR_OK;
-    bool tr_over = s.color && (st.overflow_tr || st.total_tr_pairs > s.pipe.tr_cap);
-    if (st.overflow_records || st.overflow_pairs || st.overflow_clip || st.total_records > s.pipe.rec_cap ||
-        st.total_pairs > s.pipe.pair_cap || st.clip_overflow || tr_over) {
+    // settle reads the batch's status, records its counts (which choose the
+    // depth-rejecting raster for the slot's next batch) and, on a capacity
+    // overflow, grows the capacities first; the batch then re-runs
+    // synchronously with the grown buffers
+    bool again = false;
+    int rc = s.pipe.settle(s.color != nullptr, again);
+    if (rc == SR_OK && again) {
         lk.unlock();
         rc = s.pipe.run(s.hv, s.draws, s.xv, s.in_tri, s.items, s.tangents, s.ids, s.depth, s.color,
                         s.color ? s.plane : 0, s.rgba, s.lum, persistent, true);
