@@ -12,17 +12,21 @@ Prints one JSON line:
   mtri_per_s     submitted triangles / frame time
   p50/p99 ms     per-frame latency over the timed steps
   roofline       the dominant kernel of the frame (k_raster or k_shade, the
-                 longer by HIP events on the renderer's stream): algorithmic
-                 bytes per launch over its average launch time, against 8 TB/s;
-                   k_raster: 80 B per set-up triangle (geometry + raster
-                     records, read once) + 4 B per bin entry + 4 B per pixel
-                     (triangle id written)
-                   k_shade: 20 B per pixel (id read, depth and three colour
-                     planes written) + 224 B per visible triangle (16 B geometry
-                     and 64 B raster records, three vertices' transformed
-                     position 16 B and attributes 32 B) + 4 B per covered
-                     pixel of a textured
+                 longer by HIP events on the renderer's stream): compulsory
+                 bytes per launch over its average launch time, against 8 TB/s.
+                 Compulsory bytes count only the frame's own data, never the
+                 pipeline's intermediate records, bins or cache-resident
+                 lookup tables:
+                   k_raster: 8 B per pixel (visibility buffer written:
+                     triangle id and depth)
+                   k_shade: 20 B per pixel (visibility buffer written and read
+                     back, RGBA8 written) + 4 B per covered pixel of a textured
                      material (one base-colour texel)
+                 implementation_bytes_per_launch: what the kernel reads and
+                 writes of the pipeline's own records as well (k_shade: the
+                 winner's 16 B geometry and 64 B raster records and its three
+                 transformed vertices, 224 B per visible triangle), reported
+                 beside, never used for frac
                  traffic: HBM bytes per launch measured with rocprofv3
                  FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a
                  device copy of known size (profiles/traffic_<config>.json)
@@ -31,11 +35,16 @@ Prints one JSON line:
                  2400 MHz) over its live launch time: the bound that holds
                  for these kernels, whose IEEE divisions and square roots
                  keep them on the VALU rather than on HBM
-  frame_roofline the whole frame with its algorithmic bytes (vertices,
-                 indices, pixels, texels)
+  frame_roofline the whole frame with its compulsory bytes
+                 (24 + 8 [textured]) N_vert + 12 N_tri + 20 W H + 4 N_cov
+                 (vertices, indices, visibility buffer write and read, RGBA8,
+                 base-colour texels)
   cpu_baseline   the CPU oracle in reference-like threaded mode (128-triangle
-                 jobs into shared 64px-tile queues, one job per tile) on this
-                 host's cores, on a bounded sample of the same frames
+                 jobs into shared 64px-tile queues, one job per tile) on all of
+                 this host's cores, on a bounded sample of the same frames
+  shading_fp64   FP32 shading on the GPU against the FP64 instantiation of the
+                 oracle's shader on the same visibility buffer: max, p99.9 and
+                 mean absolute error of the linear colour over covered pixels
 """
 import argparse
 import json
@@ -52,16 +61,30 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mtri/s & Gpix/s @1080p; batched frames/s vs plain; p50/p99 frame latency"
 
 
-def algorithmic_bytes(scene, w, h, n_cov, setup_tris, pairs, visible_tris):
+def algorithmic_bytes(scene, w, h, n_cov, visible_tris):
+    """Compulsory bytes of the frame and of its two per-tile kernels, and the
+    shading kernel's implementation bytes (its record reads) beside them."""
     textured = any(m.get("base_color_tex") is not None for m in scene.materials)
     nv = sum(len(p["positions"]) for p in scene.primitives)
     nt = scene.triangles
     per_vertex = 24 + (8 if textured else 0)
     texels = 4 * n_cov if textured else 0
-    frame = per_vertex * nv + 12 * nt + 12 * w * h + texels
-    kernels = {"k_raster": 80 * setup_tris + 4 * pairs + 4 * w * h,
-               "k_shade": 20 * w * h + 224 * visible_tris + texels}
-    return frame, kernels
+    frame = per_vertex * nv + 12 * nt + 20 * w * h + texels
+    kernels = {"k_raster": 8 * w * h, "k_shade": 20 * w * h + texels}
+    implementation = {"k_shade": 20 * w * h + texels + 224 * visible_tris}
+    return frame, kernels, implementation
+
+
+def cpu_share():
+    """Threads for the CPU baseline: every core this process may run on,
+    within the CPU share the host grants one GPU's jobs (OMP_NUM_THREADS
+    where the host sets it)."""
+    visible = len(os.sched_getaffinity(0))
+    cores = visible
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return cores, f"{cores} threads: the CPU share of one GPU, {visible} cores visible"
 
 
 def copy_calibration(mib):
@@ -93,10 +116,8 @@ def run_batched(args):
     pool = sc.make_c4_pool()
     views = sc.make_c4_views(pool, views=args.views)
     env = sr.Environment.load()
-    scenes = []
-    for p in pool:
-        p.prepare(env, cache_key="c2_s2_g724")      # same terrain as C2: its voxel grid
-        scenes.append(sr.Scene(p))
+    sc.prepare_c4_pool(pool, env, device=0)      # one voxel grid for the shared terrain
+    scenes = [sr.Scene(p) for p in pool]
     cams = [sc.make_camera(spec) for _, spec in views]
     L = sr.lib()
     hip = C.CDLL("libamdhip64.so")
@@ -165,39 +186,15 @@ def run_batched(args):
 
 
 def c5_scenes(sr, sc):
-    """Eight resident scenes mixing the configurations: C1, C2, C3 and the
-    five largest heightfields of the C4 pool."""
+    """The served configuration's eight resident scenes (swraster-viewer_amd/
+    scenes.py make_c5_scenes: 12.3M triangles, 240 distinct 2048^2 textures),
+    their voxel grids baked on the GPU."""
     env = sr.Environment.load()
     out = []
-    for name in ("c1", "c2", "c3"):
-        data, _ = sc.CONFIGS[name]()
-        data.prepare(env, cache_key=data.name)
-        out.append(data)
-    for data in sc.make_c4_pool()[-5:]:
-        data.prepare(env, cache_key="c2_s2_g724")
+    for data in sc.make_c5_scenes():
+        data.prepare(env, device=0)
         out.append(data)
     return out
-
-
-def c5_requests(sc, datas, n, rate, seed):
-    """n requests with Poisson arrivals at `rate`/s: random scene, orbit
-    camera around its bounds, resolution from {720p, 1080p, 1440p, 2160p}."""
-    rng = np.random.Generator(np.random.PCG64(seed))
-    res = ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    reqs = []
-    for i in range(n):
-        k = int(rng.integers(len(datas)))
-        w, h = res[int(rng.integers(len(res)))]
-        _, _, centre, diag = datas[k].bounds()
-        az, el = rng.uniform(0, 2 * np.pi), rng.uniform(0.15, 0.6)
-        dist = diag * rng.uniform(0.45, 0.8)
-        pos = (centre[0] + dist * np.cos(el) * np.sin(az), centre[1] + dist * np.sin(el),
-               centre[2] + dist * np.cos(el) * np.cos(az))
-        yaw, pitch = sc.look_pose(pos, centre)
-        spec = sc.camera_spec(pos, yaw, pitch, np.pi / 3, w, h, 2.0 * diag)
-        reqs.append((float(arrivals[i]), k, sc.make_camera(spec), w, h))
-    return reqs
 
 
 def run_served(args):
@@ -217,7 +214,7 @@ def run_served(args):
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     renderers, targets = {}, {}
     for (w, h) in ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)):
-        renderers[(w, h)] = sr.Renderer(w, h)
+        renderers[(w, h)] = sr.Renderer(w, h, tile_size=args.tile)     # the server's tile size
         targets[(w, h)] = C.c_void_p()
         assert hip.hipMalloc(C.byref(targets[(w, h)]), w * h * 4) == 0
     server = sr.Server(admission_bytes=64 << 30, max_batch_views=args.max_batch, streams=2, tile_size=args.tile)
@@ -234,6 +231,8 @@ def run_served(args):
             sr.check(L.sr_renderer_render_scene(r.handle, scenes[k].handle, C.byref(cam.c)))
             sr.check(L.sr_renderer_update_auto_exposure(r.handle, 0.0))
             sr.check(L.sr_renderer_blit_to_device(r.handle, targets[(w, h)]))
+            # blit_to_device returns once the frame and its blit have completed
+            # (it synchronizes the renderer's stream): latency at completion
             lat.append(time.perf_counter() - t0 - a)
         span = time.perf_counter() - t0 - reqs[0][0]
         return np.array(lat), span
@@ -264,13 +263,14 @@ def run_served(args):
         return np.array(lat), t_last - reqs[0][0]
 
     # warm-up: code, buffers and pipeline capacities for every resolution
-    warm = c5_requests(sc, datas, 48, 2000.0, 99)
+    warm = sc.make_c5_requests(datas, 48, 2000.0, 99)
     plain(warm)
     served(warm)
     rows = []
     for rate in [float(r) for r in args.rates.split(",")]:
-        reqs = c5_requests(sc, datas, args.requests, rate, int(rate))
-        row = {"offered_per_s": rate}
+        # at least two seconds of arrivals (or `requests`, whichever is fewer)
+        reqs = sc.make_c5_requests(datas, min(args.requests, max(60, int(2 * rate))), rate, int(rate))
+        row = {"offered_per_s": rate, "requests": len(reqs)}
         for name, fn in (("plain", plain), ("served", served)):
             lat, span = fn(reqs)
             row[name] = {"frames_per_s": round(len(reqs) / span, 1), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
@@ -294,11 +294,13 @@ def run_served(args):
         "metric": "served frames/s and p50/p99 latency vs the plain per-frame path (Poisson requests)",
         "value": top["served"]["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": args.requests,
         "warmup": 48, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (C1, C2, C3 and five C4 heightfields resident; seeded requests)",
+        "dtype": "f32", "data": "synthetic (scenes.py make_c5_scenes: icosphere clusters, patched heightfields "
+                                "and overdraw triangles resident; seeded requests)",
         "config": {"workload": "c5_served_mixed", "scenes": len(scenes),
                    "triangles_resident": int(sum(d.triangles for d in datas)),
+                   "texture_bytes_resident": int(sum(4 * w * h for d in datas for w, h, _, _ in d.textures)),
                    "resolutions": "720p/1080p/1440p/2160p", "admission_bytes": 64 << 30, "streams": 2,
-                   "max_batch_views": args.max_batch, "requests_per_rate": args.requests},
+                   "max_batch_views": args.max_batch, "requests_per_rate": f"min({args.requests}, max(60, 2 x rate))"},
         "rates": rows, "equal_p99": cmp, "plain_p99_at_saturation_ms": bound,
     }
 
@@ -309,7 +311,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
-    ap.add_argument("--rates", default="250,500,1000,1500,2000,2500,3000,4000,6000",
+    ap.add_argument("--rates", default="50,250,500,1000,1500,2000,2500,3000,4000,6000",
                     help="c5: offered request rates (req/s)")
     ap.add_argument("--max-batch", type=int, default=8, help="c5: most requests per server batch")
     ap.add_argument("--requests", type=int, default=240, help="c5: requests per rate")
@@ -381,7 +383,7 @@ def main():
     n_cov = int((ids != sr.NO_ID).sum())
     visible = int(np.unique(ids[ids != sr.NO_ID]).size)
     st = r.stats()
-    frame_b, kernel_bytes = algorithmic_bytes(scene, w, h, n_cov, st["setup_triangles"], st["binned_pairs"], visible)
+    frame_b, kernel_bytes, impl_bytes = algorithmic_bytes(scene, w, h, n_cov, visible)
     kernel_ms = {"k_raster": stage["ms_raster"] / args.steps, "k_shade": stage["ms_shade"] / args.steps}
     dom = max(kernel_ms, key=kernel_ms.get)
     k_ms = kernel_ms[dom]
@@ -418,6 +420,7 @@ def main():
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": None, "algorithmic_bytes_per_launch": kernel_b,
+                     "implementation_bytes_per_launch": impl_bytes.get(dom),
                      "avg_launch_ms": round(k_ms, 5),
                      "kernels_ms": {k: round(v, 5) for k, v in kernel_ms.items()},
                      "kernels_algorithmic_bytes": kernel_bytes},
@@ -446,7 +449,7 @@ def main():
 
     if not args.no_cpu_baseline:
         orc = entry.load_oracle()
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores, share = cpu_share()
         o = orc.OracleRenderer(w, h)
         osc = orc.OracleScene(scene)
         o.render_scene(osc, cam, threads=cores, canonical=False)
@@ -463,8 +466,21 @@ def main():
                                "kind": "port",
                                "sample": f"{len(times)} frames of the same workload after 1 warm-up "
                                          f"(median {med * 1e3:.1f} ms/frame, "
-                                         f"{scene.triangles / med / 1e6:.2f} Mtri/s)"}
+                                         f"{scene.triangles / med / 1e6:.2f} Mtri/s; {share})"}
         out["gpu_vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+        # the checker's FP64 shader on the GPU's visibility buffer
+        o.render_scene(osc, cam, threads=cores, fp64=True)
+        f = o.read_frame(fp64=True)
+        rgb = r.read_color()
+        cov = ids != sr.NO_ID
+        if np.array_equal(ids, f["ids"]) and cov.any():
+            err = np.abs(rgb.astype(np.float64) - f["rgb64"])[cov]
+            out["shading_fp64"] = {"max_abs": float(f"{err.max():.4g}"),
+                                   "p99_9_abs": float(f"{np.percentile(err, 99.9):.4g}"),
+                                   "mean_abs": float(f"{err.mean():.4g}"), "pixels": int(cov.sum()),
+                                   "quantity": "linear RGB before exposure and tonemap"}
+        else:
+            out["shading_fp64"] = {"error": "visibility buffers differ; no error reported"}
     if args.config == "c2" and not args.no_extras:
         # the metric's other two parts, on their own workloads (C4, C5)
         del r, gscene

@@ -75,25 +75,6 @@ static inline void mat_point(const float *m, const float p[3], float out[4]) {
     }
 }
 
-template <class F> static void parallel_for(size_t n, int threads, F &&fn) {
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
-    if (threads == 1 || n < 2) {
-        for (size_t i = 0; i < n; i++) fn(i);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::vector<std::thread> pool;
-    for (int t = 0; t < threads; t++)
-        pool.emplace_back([&]() {
-            for (;;) {
-                size_t i = next.fetch_add(1);
-                if (i >= n) break;
-                fn(i);
-            }
-        });
-    for (auto &th : pool) th.join();
-}
-
 // ---------------------------------------------------------------------------
 // Mip chains (src/texture.rs:45-128)
 // ---------------------------------------------------------------------------

@@ -1,12 +1,36 @@
 // prep.h -- host-side load-time input producers and camera math.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "swraster.h"
 
 namespace sr {
+
+// fn(i) for i in [0, n) over `threads` host threads (all cores when <= 0),
+// each taking the next index from a shared counter.
+template <class F> void parallel_for(size_t n, int threads, F &&fn) {
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads == 1 || n < 2) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads && (size_t)t < n; t++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                size_t i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i);
+            }
+        });
+    for (auto &th : pool) th.join();
+}
 
 void build_mip_chain(uint32_t width, uint32_t height, int type, const uint32_t *base, std::vector<uint32_t> &data,
                      std::vector<uint32_t> &offsets, std::vector<uint32_t> &widths, std::vector<uint32_t> &heights);

@@ -217,14 +217,28 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
     s->vertices = nv;
     s->any_tangents = any_tan;
 
-    // textures: mip chains into one pool
+    // textures: mip chains (built on the host's cores, one texture per
+    // task) into one pool
     std::vector<DevTexture> texs(d->num_textures);
     std::vector<uint32_t> pool;
     for (int i = 0; i < d->num_textures; i++) {
         const sr_texture_desc &t = d->textures[i];
         if (!t.texels || t.width == 0 || t.height == 0) return bail(fail(SR_ERR_INVALID_ARGUMENT, "texture %d empty", i));
-        std::vector<uint32_t> data, off, w, h;
-        build_mip_chain(t.width, t.height, t.type, t.texels, data, off, w, h);
+    }
+    struct Chain { std::vector<uint32_t> data, off, w, h; };
+    std::vector<Chain> chains(d->num_textures);
+    parallel_for((size_t)d->num_textures, std::min(16, (int)std::max(1u, std::thread::hardware_concurrency())),
+                 [&](size_t i) {
+                     const sr_texture_desc &t = d->textures[i];
+                     Chain &c = chains[i];
+                     build_mip_chain(t.width, t.height, t.type, t.texels, c.data, c.off, c.w, c.h);
+                 });
+    size_t pool_size = 0;
+    for (const Chain &c : chains) pool_size += c.data.size();
+    pool.reserve(pool_size);
+    for (int i = 0; i < d->num_textures; i++) {
+        const sr_texture_desc &t = d->textures[i];
+        std::vector<uint32_t> &data = chains[i].data, &off = chains[i].off, &w = chains[i].w, &h = chains[i].h;
         if (off.size() > SR_MAX_MIPS) return bail(fail(SR_ERR_UNSUPPORTED, "texture %d has too many mips", i));
         DevTexture &o = texs[i];
         memset(&o, 0, sizeof(o));
@@ -241,6 +255,7 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
             o.mip_h[k] = h[k];
         }
         pool.insert(pool.end(), data.begin(), data.end());
+        std::vector<uint32_t>().swap(data);
     }
 
     DevScene ds;

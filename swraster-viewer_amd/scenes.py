@@ -28,6 +28,14 @@ is roll * pitch * yaw, so a positive pitch field looks *down*).
       the terrain centre (azimuth U[0, 2pi), distance U[45, 75], height
       U[12, 30]).  The scenes share the terrain function, so the voxel grid
       built for C2 lights all of them.
+  C5  eight resident scenes mixing the three configurations, 12.3M
+      triangles and 240 distinct 2048^2 RGBA8 uniform-noise textures
+      (4.03 GB of base levels): icosphere clusters of 64 and 256 spheres at
+      subdivision 4 (one textured material per sphere, 48 textures each),
+      heightfields of 724^2, 1448^2 and 1535^2 vertices cut into 8 x 6
+      patches (one textured material per patch), a 512^2 untextured
+      heightfield and two 100,000-triangle overdraw scenes (C3's generator,
+      seeds 3 and 31).
 """
 import json
 import os
@@ -246,6 +254,124 @@ CONFIGS = {"c1": make_c1, "c2": make_c2, "c3": make_c3}
 
 
 # ---------------------------------------------------------------------------
+# C5: resident scenes of the served configuration
+# ---------------------------------------------------------------------------
+def noise_texture(rng, size=2048):
+    """size^2 RGBA8 uniform noise (opaque), packed, sRGB base colour."""
+    t = rng.integers(0, 1 << 32, size * size, dtype=np.uint64).astype(np.uint32) | np.uint32(0xFF)
+    return (size, size, SR_TEX_SRGB, t)
+
+
+def textured_material(rng, tex):
+    col = rng.uniform(0.2, 1.0, 3)
+    return {"base_color": (col[0], col[1], col[2], 1.0), "metallic": rng.uniform(0, 1),
+            "roughness": rng.uniform(0.05, 1), "base_color_tex": (tex, SR_WRAP_REPEAT, SR_WRAP_REPEAT),
+            "alpha_mode": SR_ALPHA_OPAQUE}
+
+
+def make_icosphere_cluster(seed, spheres, subdiv, textures, tex_size=2048, name=None):
+    """C1's construction with more spheres: centres U[-4,4]^3, radii
+    U[0.5,1], spherical UVs, one material per sphere; materials take the
+    `textures` distinct noise textures in turn."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v, f = icosphere(subdiv)
+    uv = np.stack([0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * np.pi), np.arccos(np.clip(v[:, 1], -1, 1)) / np.pi],
+                  1).astype(np.float32)
+    texs = [noise_texture(rng, tex_size) for _ in range(textures)]
+    prims, meshes, nodes, mats = [], [], [], []
+    for i in range(spheres):
+        c = rng.uniform(-4, 4, 3).astype(np.float32)
+        r = np.float32(rng.uniform(0.5, 1.0))
+        prims.append({"positions": v, "normals": v, "texcoords": uv, "tangents": None,
+                      "indices": f.ravel(), "material": i})
+        meshes.append((i, 1))
+        nodes.append((_translate_scale(c, r), i))
+        mats.append(textured_material(rng, i % textures))
+    return SceneData(prims, meshes, nodes, mats, texs, name=name or f"c5_ico_s{seed}_n{spheres}")
+
+
+def make_patched_heightfield(seed, grid, patches=(8, 6), textures=True, tex_size=2048, name=None):
+    """C2's terrain (seed 2 heights) at `grid`^2 vertices cut into patches
+    (px x pz grid cells each, neighbours sharing their border vertices), one
+    material per patch with its own noise texture spanning the patch."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pos, nrm, _, _ = heightfield(grid, 50.0, 5.0, np.random.Generator(np.random.PCG64(2)))
+    P = pos.reshape(grid, grid, 3)
+    N = nrm.reshape(grid, grid, 3)
+    cuts_i = np.linspace(0, grid - 1, patches[0] + 1).round().astype(int)
+    cuts_j = np.linspace(0, grid - 1, patches[1] + 1).round().astype(int)
+    prims, mats, texs = [], [], []
+    for a in range(patches[0]):
+        for b in range(patches[1]):
+            i0, i1, j0, j1 = cuts_i[a], cuts_i[a + 1], cuts_j[b], cuts_j[b + 1]
+            ni, nj = i1 - i0 + 1, j1 - j0 + 1
+            p = np.ascontiguousarray(P[i0:i1 + 1, j0:j1 + 1].reshape(-1, 3))
+            n = np.ascontiguousarray(N[i0:i1 + 1, j0:j1 + 1].reshape(-1, 3))
+            ii, jj = np.meshgrid(np.arange(ni), np.arange(nj), indexing="ij")
+            uv = np.stack([ii / (ni - 1), jj / (nj - 1)], -1).reshape(-1, 2).astype(np.float32)
+            qi, qj = np.meshgrid(np.arange(ni - 1), np.arange(nj - 1), indexing="ij")
+            v00 = (qi * nj + qj).ravel()
+            v01 = (qi * nj + qj + 1).ravel()
+            v10 = ((qi + 1) * nj + qj).ravel()
+            v11 = ((qi + 1) * nj + qj + 1).ravel()
+            idx = np.stack([np.stack([v00, v01, v10], 1), np.stack([v10, v01, v11], 1)], 1).reshape(-1)
+            m = len(mats)
+            prims.append({"positions": p, "normals": n, "texcoords": uv, "tangents": None,
+                          "indices": idx.astype(np.uint32), "material": m})
+            if textures:
+                texs.append(noise_texture(rng, tex_size))
+                mats.append(textured_material(rng, len(texs) - 1))
+            else:
+                col = rng.uniform(0.2, 1.0, 3)
+                mats.append({"base_color": (col[0], col[1], col[2], 1.0), "metallic": 0.0, "roughness": 0.5})
+    return SceneData(prims, [(0, len(prims))], [(np.eye(4, dtype=np.float32), 0)], mats, texs,
+                     name=name or f"c5_hf_s{seed}_g{grid}")
+
+
+C5_SCENES = (
+    ("ico64", lambda: make_icosphere_cluster(51, 64, 4, 48)),
+    ("ico256", lambda: make_icosphere_cluster(52, 256, 4, 48)),
+    ("hf724", lambda: make_patched_heightfield(53, 724)),
+    ("hf1448", lambda: make_patched_heightfield(54, 1448)),
+    ("hf1535", lambda: make_patched_heightfield(55, 1535)),
+    ("hf512", lambda: make_patched_heightfield(56, 512, textures=False)),
+    ("c3a", lambda: make_c3(seed=3)[0]),
+    ("c3b", lambda: make_c3(seed=31)[0]),
+)
+
+
+def make_c5_scenes(names=None):
+    """The served configuration's resident scenes (all, or those named)."""
+    return [fn() for name, fn in C5_SCENES if names is None or name in names]
+
+
+C5_RESOLUTIONS = ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
+
+
+def make_c5_requests(datas, n, rate, seed):
+    """n requests [(arrival s, scene index, camera, width, height)] with
+    Poisson arrivals at `rate`/s: a random scene, an orbit camera around its
+    bounds (azimuth U[0, 2pi), elevation U[0.15, 0.6] rad, distance
+    U[0.45, 0.8] x the scene diagonal, aimed at the centre, 60 deg, far =
+    2 x diagonal) and a resolution from {720p, 1080p, 1440p, 2160p}."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    reqs = []
+    for i in range(n):
+        k = int(rng.integers(len(datas)))
+        w, h = C5_RESOLUTIONS[int(rng.integers(len(C5_RESOLUTIONS)))]
+        _, _, centre, diag = datas[k].bounds()
+        az, el = rng.uniform(0, 2 * np.pi), rng.uniform(0.15, 0.6)
+        dist = diag * rng.uniform(0.45, 0.8)
+        pos = (centre[0] + dist * np.cos(el) * np.sin(az), centre[1] + dist * np.sin(el),
+               centre[2] + dist * np.cos(el) * np.cos(az))
+        yaw, pitch = look_pose(pos, centre)
+        spec = camera_spec(pos, yaw, pitch, np.pi / 3, w, h, 2.0 * diag)
+        reqs.append((float(arrivals[i]), k, make_camera(spec), w, h))
+    return reqs
+
+
+# ---------------------------------------------------------------------------
 # C4: batched views over a pool of heightfields
 # ---------------------------------------------------------------------------
 C4_RESOLUTIONS = ((320, 240), (640, 480), (1280, 720), (1920, 1080))
@@ -280,6 +406,16 @@ def make_c4_pool(seed=4, scenes=16, tris_min=10_000, tris_max=2_000_000, tex_siz
                  "base_color_tex": (0, SR_WRAP_REPEAT, SR_WRAP_REPEAT)}]
         pool.append(SceneData(prims, [(0, 1)], [(np.eye(4, dtype=np.float32), 0)], mats, [tex],
                               name=f"c4_g{n}"))
+    return pool
+
+
+def prepare_c4_pool(pool, environment, device=None):
+    """Environment and voxel grid for every pool scene: the grid is baked
+    once, from the largest scene, and shared (the scenes sample one terrain)."""
+    src = pool[-1]
+    src.prepare(environment, device=device)
+    for p in pool[:-1]:
+        p.attach(src.environment, src.voxels)
     return pool
 
 
