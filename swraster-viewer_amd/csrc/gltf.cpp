@@ -628,10 +628,11 @@ struct Loader {
         uint32_t w = 0, h = 0;
         std::vector<uint32_t> px;
         std::string err;
-        if (bytes.size() >= 3 && bytes[0] == 0xFF && bytes[1] == 0xD8 && bytes[2] == 0xFF)
-            return fail(SR_ERR_UNSUPPORTED, "texture '%s' is JPEG; only PNG textures are decoded", uri.c_str());
-        if (!png_decode_rgba8(bytes.data(), bytes.size(), w, h, px, err))
-            return fail(SR_ERR_INVALID_ARGUMENT, "texture '%s': %s", uri.c_str(), err.c_str());
+        bool jpeg = bytes.size() >= 3 && bytes[0] == 0xFF && bytes[1] == 0xD8 && bytes[2] == 0xFF;
+        bool ok = jpeg ? jpeg_decode_rgba8(bytes.data(), bytes.size(), w, h, px, err)
+                       : png_decode_rgba8(bytes.data(), bytes.size(), w, h, px, err);
+        if (!ok)
+            return fail(SR_ERR_INVALID_ARGUMENT, "texture '%s': %s: %s", uri.c_str(), jpeg ? "JPEG" : "PNG", err.c_str());
         sr_texture_desc td;
         td.width = w;
         td.height = h;
@@ -794,6 +795,22 @@ SR_API int sr_png_decode(const uint8_t *data, size_t size, uint32_t *width, uint
     std::vector<uint32_t> px;
     std::string err;
     if (!png_decode_rgba8(data, size, w, h, px, err)) return fail(SR_ERR_INVALID_ARGUMENT, "PNG: %s", err.c_str());
+    *width = w;
+    *height = h;
+    if (texels) {
+        if (capacity < px.size()) return fail(SR_ERR_INVALID_ARGUMENT, "texel buffer holds %zu, image has %zu", capacity, px.size());
+        memcpy(texels, px.data(), px.size() * 4);
+    }
+    return SR_OK;
+}
+
+SR_API int sr_jpeg_decode(const uint8_t *data, size_t size, uint32_t *width, uint32_t *height, uint32_t *texels,
+                          size_t capacity) {
+    if (!data || !width || !height) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
+    uint32_t w = 0, h = 0;
+    std::vector<uint32_t> px;
+    std::string err;
+    if (!jpeg_decode_rgba8(data, size, w, h, px, err)) return fail(SR_ERR_INVALID_ARGUMENT, "JPEG: %s", err.c_str());
     *width = w;
     *height = h;
     if (texels) {

@@ -16,4 +16,9 @@ bool zlib_inflate(const uint8_t *data, size_t size, std::vector<uint8_t> &out, s
 bool png_decode_rgba8(const uint8_t *data, size_t size, uint32_t &width, uint32_t &height, std::vector<uint32_t> &out,
                       std::string &err);
 
+// Decode a JPEG (baseline, extended sequential or progressive Huffman,
+// 8-bit, one or three components) into packed r<<24 | g<<16 | b<<8 | 0xFF.
+bool jpeg_decode_rgba8(const uint8_t *data, size_t size, uint32_t &width, uint32_t &height, std::vector<uint32_t> &out,
+                       std::string &err);
+
 }  // namespace sr
