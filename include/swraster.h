@@ -364,6 +364,13 @@ SR_API int sr_gltf_destroy(sr_gltf *gltf);
  * only the size is returned. */
 SR_API int sr_png_decode(const uint8_t *data, size_t size, uint32_t *width, uint32_t *height,
                          uint32_t *texels, size_t capacity);
+/* JPEG decoding as used for glTF textures (the reference's image crate,
+ * src/texture.rs:897-1015): baseline, extended sequential and progressive
+ * Huffman, 8-bit, grey or YCbCr (RGB with an Adobe marker), any sampling
+ * factors, restart intervals; packed r<<24|g<<16|b<<8|0xFF texels.  With
+ * texels NULL only the size is returned. */
+SR_API int sr_jpeg_decode(const uint8_t *data, size_t size, uint32_t *width, uint32_t *height,
+                          uint32_t *texels, size_t capacity);
 
 /* ---- camera ------------------------------------------------------------- */
 /* RenderCamera::new (src/rendercamera.rs:28-63): yaw/pitch from the look-at
@@ -433,6 +440,16 @@ SR_API int sr_renderer_read_color(sr_renderer *renderer, float *rgb);
 /* Centre-quad luminance of every 64px tile (src/tilerasterizer.rs:103-106). */
 SR_API int sr_renderer_read_tile_luminance(sr_renderer *renderer, float *lum, int32_t *count);
 SR_API int sr_renderer_get_stats(const sr_renderer *renderer, sr_frame_stats *stats);
+
+/* Parity hook: the last frame's per-tile lists (bins) of set-up triangle
+ * ids, the form of the reference's per-tile packet queues
+ * (src/bumpqueue.rs:96-112).  *items receives the number of screen tiles;
+ * start and end (items entries each, either may be NULL) bound tile i's ids
+ * in list, which takes up to list_len of them (list may be NULL).  Each
+ * tile's ids ascend: the queue order of the reference's single-threaded
+ * front end. */
+SR_API int sr_renderer_read_bins(sr_renderer *renderer, uint32_t *start, uint32_t *end, uint32_t *list,
+                                 uint64_t list_len, uint32_t *items);
 /* Device pointers of the last frame's visibility buffer and colour (for
  * in-HBM consumers); valid until the next render call. */
 SR_API int sr_renderer_device_buffers(sr_renderer *renderer, void **tri_id, void **depth,

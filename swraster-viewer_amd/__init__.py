@@ -184,6 +184,8 @@ _SIGS = {
     "sr_gltf_destroy": (C.c_int, [_P]),
     "sr_png_decode": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint32), C.c_size_t]),
+    "sr_jpeg_decode": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                C.POINTER(C.c_uint32), C.c_size_t]),
     "sr_camera_new": (C.c_int, [C.c_float * 3, C.c_float * 3, C.c_float, C.c_float, C.c_float, C.c_float,
                                 C.POINTER(sr_camera)]),
     "sr_camera_from_pose": (C.c_int, [C.c_float * 3, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -224,6 +226,8 @@ _SIGS = {
     "sr_server_read_pixels": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_uint32)]),
     "sr_server_read_vbuffer": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
     "sr_server_release": (C.c_int, [_P, C.c_uint64]),
+    "sr_renderer_read_bins": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.c_uint64, C.POINTER(C.c_uint32)]),
     "sr_server_drain": (C.c_int, [_P]),
     "sr_server_clock": (C.c_double, [_P]),
 }
@@ -250,6 +254,18 @@ def check(code):
     if code != SR_OK:
         raise SwrasterError(code, lib().sr_last_error().decode(errors="replace"))
     return code
+
+
+def jpeg_decode(data):
+    """Decode JPEG bytes with the library's decoder (the one the glTF loader
+    uses): (height, width) array of packed r<<24|g<<16|b<<8|0xFF texels."""
+    buf = bytes(data)
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().sr_jpeg_decode(buf, len(buf), C.byref(w), C.byref(h), None, 0))
+    out = np.empty(w.value * h.value, np.uint32)
+    check(lib().sr_jpeg_decode(buf, len(buf), C.byref(w), C.byref(h), out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               out.size))
+    return out.reshape(h.value, w.value)
 
 
 def png_decode(data):
@@ -762,6 +778,18 @@ class Renderer:
         out = np.empty(n.value, np.float32)
         check(lib().sr_renderer_read_tile_luminance(self.handle, _fptr(out), C.byref(n)))
         return out
+
+    def read_bins(self):
+        """The last frame's per-tile lists: (start, end, list) arrays."""
+        n = C.c_uint32()
+        check(lib().sr_renderer_read_bins(self.handle, None, None, None, 0, C.byref(n)))
+        start = np.empty(n.value, np.uint32)
+        end = np.empty(n.value, np.uint32)
+        check(lib().sr_renderer_read_bins(self.handle, _uptr(start), _uptr(end), None, 0, C.byref(n)))
+        total = int(self.stats()["binned_pairs"])
+        lst = np.empty(max(total, 1), np.uint32)
+        check(lib().sr_renderer_read_bins(self.handle, None, None, _uptr(lst), total, C.byref(n)))
+        return start, end, lst[:total]
 
     def stats(self):
         s = sr_frame_stats()

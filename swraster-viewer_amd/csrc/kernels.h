@@ -19,7 +19,8 @@ struct FrameArgs {
     int32_t tile_size;            // 8, 16 or 32
     int32_t any_tangents;
     // transient buffers
-    float4 *xv;                   // total_xv clip positions, then 2 * total_xv attribute float4
+    float4 *xv;                   // total_xv clip positions, 2 * total_xv attribute float4, then
+                                  // total_xv snaps (px, py, 1/w, z/w)
     float4 *xt;                   // tangent float4 per transformed vertex (or null)
     uint32_t *tri_counts;         // total_in
     uint32_t *tri_chunks;         // (total_in + 255) / 256 fan-count sums, then first ids

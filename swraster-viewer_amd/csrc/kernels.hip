@@ -148,6 +148,7 @@ struct CV {
     float nx, ny, nz;             // normal
     float u, v;                   // uv
     float tx, ty, tz, tw;         // tangent
+    float oow;                    // 1 / cw
 };
 
 // Transformed vertices are two arrays: clip-space positions (one float4 per
@@ -222,10 +223,33 @@ __device__ __forceinline__ void clip_position(const DevDraw &D, const DevScene &
     }
 }
 
+// Per-vertex part of bin_triangle (src/renderer.rs:699-709, 834-846): the
+// snapped sub-pixel position round(((ndc + 1) W / 2) 16) (y flipped), 1/w and
+// z/w as z (1/w).  Every operation depends on the vertex and the view only,
+// so the transform computes it once per vertex for the triangles that need
+// no clipping, and clipped fans compute it from their polygon's vertices.
+struct SnapV {
+    int32_t px, py;
+    float oow, zow;
+};
+__device__ __forceinline__ SnapV snap_vertex(float Wf, float Hf, float cx, float cy, float cz, float cw) {
+    SnapV v;
+    float nxv = cx / cw, nyv = cy / cw;
+    float sx = ((nxv + 1.0f) * Wf) / 2.0f;
+    float sy = ((1.0f - nyv) * Hf) / 2.0f;
+    v.px = sat_i32(roundf(sx * 16.0f));
+    v.py = sat_i32(roundf(sy * 16.0f));
+    v.oow = 1.0f / cw;
+    v.zow = cz * v.oow;
+    return v;
+}
+
 // Transform of vertex i of draw D (render_primitive_inner, src/renderer.rs:
-// 496-521): clip = MVP p, world = M p, normal = M3 n, tangent = M3 t.
-__device__ __forceinline__ void transform_vertex(const DevDraw &D, const DevScene &S, uint32_t i, float4 *xp,
-                                                 float4 *xa, float4 *xt) {
+// 496-521): clip = MVP p, world = M p, normal = M3 n, tangent = M3 t; and the
+// vertex's snap for the view (xs: px, py as integer bits, 1/w, z/w).
+__device__ __forceinline__ void transform_vertex(const DevDraw &D, const DevView &V, uint32_t i, float4 *xp,
+                                                 float4 *xa, float4 *xt, float4 *xs) {
+    const DevScene &S = *V.scene;
     uint32_t s = D.vtx_base + (i - D.xv_base);
     float x = S.px[s], y = S.py[s], z = S.pz[s];
     float nx = S.nx[s], ny = S.ny[s], nz = S.nz[s];
@@ -243,6 +267,8 @@ __device__ __forceinline__ void transform_vertex(const DevDraw &D, const DevScen
         n[r] = an;
     }
     xp[i] = make_float4(c[0], c[1], c[2], c[3]);
+    SnapV sv = snap_vertex(V.Wf, V.Hf, c[0], c[1], c[2], c[3]);
+    xs[i] = make_float4(__int_as_float(sv.px), __int_as_float(sv.py), sv.oow, sv.zow);
     xa[2 * (size_t)i] = make_float4(w[0], w[1], w[2], n[0]);
     xa[2 * (size_t)i + 1] = make_float4(n[1], n[2], S.tu[s], S.tv[s]);
     if (xt) {
@@ -271,17 +297,17 @@ __device__ __forceinline__ int wave_uniform(int d) {
 
 // Vertices b*256+lane, stepping by nb*256, of the transform's nb workgroups.
 __device__ __forceinline__ void transform_vertices(const DevView *views, const DevDraw *draws, int ndraws,
-                                                   uint32_t total, float4 *xp, float4 *xa, float4 *xt, uint32_t b,
-                                                   uint32_t nb) {
+                                                   uint32_t total, float4 *xp, float4 *xa, float4 *xt, float4 *xs,
+                                                   uint32_t b, uint32_t nb) {
     for (uint32_t i = b * blockDim.x + threadIdx.x; i < total; i += nb * blockDim.x) {
         int d = find_last_le_consecutive(draws, ndraws, i, &DevDraw::xv_base);
         int du = wave_uniform(d);
         if (du >= 0) {
             const DevDraw &D = draws[du];
-            transform_vertex(D, *views[D.view].scene, i, xp, xa, xt);
+            transform_vertex(D, views[D.view], i, xp, xa, xt, xs);
         } else {
             const DevDraw &D = draws[d];
-            transform_vertex(D, *views[D.view].scene, i, xp, xa, xt);
+            transform_vertex(D, views[D.view], i, xp, xa, xt, xs);
         }
     }
 }
@@ -367,14 +393,14 @@ __device__ __forceinline__ void classify_triangles(const DevView *views, const D
 __global__ void __launch_bounds__(256) k_front(const DevView *__restrict__ views, const DevDraw *__restrict__ draws,
                                                int ndraws, uint32_t total_in, uint32_t total_xv,
                                                float4 *__restrict__ xp, float4 *__restrict__ xa,
-                                               float4 *__restrict__ xt, uint32_t *counts, uint32_t *chunk,
-                                               uint32_t *queue, uint32_t qcap, DevStatus *status, FrameBegin F,
-                                               uint32_t tri_blocks) {
+                                               float4 *__restrict__ xt, float4 *__restrict__ xs, uint32_t *counts,
+                                               uint32_t *chunk, uint32_t *queue, uint32_t qcap, DevStatus *status,
+                                               FrameBegin F, uint32_t tri_blocks) {
     frame_begin(F);
     if (blockIdx.x < tri_blocks)
         classify_triangles(views, draws, ndraws, total_in, counts, chunk, queue, qcap, status, blockIdx.x, tri_blocks);
     else
-        transform_vertices(views, draws, ndraws, total_xv, xp, xa, xt, blockIdx.x - tri_blocks,
+        transform_vertices(views, draws, ndraws, total_xv, xp, xa, xt, xs, blockIdx.x - tri_blocks,
                            gridDim.x - tri_blocks);
 }
 
@@ -770,23 +796,14 @@ __device__ __forceinline__ bool small_tri_uncovered(const int32_t px[3], const i
 // few triangles that win a pixel, from the vertices that `src` locates
 // (SRC_* encoding below).
 #define SRC_CLIPPED 0x80000000u       // src0 = SRC_CLIPPED | fan << 27 | clip slot
-__device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV *v1, const CV *v2, uint32_t id,
-                              int material, uint32_t matflags, bool translucent, uint32_t src0, uint32_t src1,
-                              const SetupOut &O) {
+__device__ Rect emit_triangle(const DevView &V, int view, const SnapV sv[3], uint32_t id, int material,
+                              uint32_t matflags, bool translucent, uint32_t src0, uint32_t src1, const SetupOut &O) {
     Rect none = {0, 0, 0, 0};
     if (id >= O.cap) {
         atomicOr(&O.status->overflow_records, 1u);
         return none;
     }
-    const CV *tv[3] = {v0, v1, v2};
-    int32_t px[3], py[3];
-    for (int k = 0; k < 3; k++) {
-        float nxv = tv[k]->cx / tv[k]->cw, nyv = tv[k]->cy / tv[k]->cw;
-        float sx = ((nxv + 1.0f) * V.Wf) / 2.0f;
-        float sy = ((1.0f - nyv) * V.Hf) / 2.0f;
-        px[k] = sat_i32(roundf(sx * 16.0f));
-        py[k] = sat_i32(roundf(sy * 16.0f));
-    }
+    int32_t px[3] = {sv[0].px, sv[1].px, sv[2].px}, py[3] = {sv[0].py, sv[1].py, sv[2].py};
     int32_t area = wsub(wmul(wsub(px[1], px[0]), wsub(py[2], py[0])), wmul(wsub(px[2], px[0]), wsub(py[1], py[0])));
     if (area > 0) {
         O.geo[id] = make_uint4(0u, 0u, SR_GEO_EMPTY, (uint32_t)view);
@@ -803,9 +820,7 @@ __device__ Rect emit_triangle(const DevView &V, int view, const CV *v0, const CV
         return none;
     }
     float ooa = 1.0f / (float)wabs(area);
-    float oow[3], zow[3];
-    for (int k = 0; k < 3; k++) oow[k] = 1.0f / tv[k]->cw;
-    for (int k = 0; k < 3; k++) zow[k] = tv[k]->cz * oow[k];
+    float oow[3] = {sv[0].oow, sv[1].oow, sv[2].oow}, zow[3] = {sv[0].zow, sv[1].zow, sv[2].zow};
     uint32_t flags = 0;
     if (matflags & SR_MAT_ALPHA_TESTED) flags |= SR_GEO_ALPHA;
     if (matflags & SR_MAT_NORMAL_MAP) flags |= SR_GEO_NORMAL_MAP;
@@ -828,10 +843,12 @@ struct TriSource {
     const SR_GLOBAL float4 *xp;         // transformed clip-space positions
     const SR_GLOBAL float4 *xa;         // transformed attributes (2 per vertex)
     const SR_GLOBAL float4 *xt;         // transformed tangents (or null)
+    const SR_GLOBAL float4 *xs;         // per-vertex snap (px, py, 1/w, z/w)
     const SR_GLOBAL float *clip_buf;    // clipped polygons
 };
 
-// Transformed vertices of input triangle src0 of draw D.
+// Transformed vertices of input triangle src0 of draw D; the first float4 is
+// the vertex's snap, whose z is 1/w (shading reads no clip position).
 __device__ __forceinline__ void load_input_tri(const TriSource &T, const DevDraw &D, uint32_t src0, float4 q[3][4]) {
     const DevScene &S = *T.views[D.view].scene;
     uint32_t lt = src0 - D.in_tri_base;
@@ -839,15 +856,17 @@ __device__ __forceinline__ void load_input_tri(const TriSource &T, const DevDraw
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         size_t i = (size_t)D.xv_base + S.indices[D.idx_base + 3 * lt + k];
-        q[k][0] = T.xp[i]; q[k][1] = T.xa[2 * i]; q[k][2] = T.xa[2 * i + 1];
+        q[k][0] = T.xs[i]; q[k][1] = T.xa[2 * i]; q[k][2] = T.xa[2 * i + 1];
         q[k][3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (tan) q[k][3] = T.xt[i];
     }
 }
 
-// The three vertices of the triangle whose raster record is R: the indexed
-// transformed vertices of an unclipped input triangle (src0 = input triangle,
-// src1 = draw), or vertices 0, f+1, f+2 of a clipped polygon.
+// The three vertices of the triangle whose raster record is R, with their
+// 1/w: the indexed transformed vertices of an unclipped input triangle
+// (src0 = input triangle, src1 = draw; 1/w from the transform's snap), or
+// vertices 0, f+1, f+2 of a clipped polygon (1/w computed here).  The clip
+// position is not needed for shading and is left unset.
 __device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R, CV v[3]) {
     uint32_t src0 = __float_as_uint(R[3].z), src1 = __float_as_uint(R[3].w);
     float4 q[3][4];
@@ -858,6 +877,7 @@ __device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R,
             uint32_t vi = j == 0 ? 0u : f + (uint32_t)j;
             const float4 *p = reinterpret_cast<const float4 *>(T.clip_buf + ((size_t)e * 16u + vi) * 16u);
             q[j][0] = p[0]; q[j][1] = p[1]; q[j][2] = p[2]; q[j][3] = p[3];
+            q[j][0].z = 1.0f / q[j][0].w;
         }
     } else {
         int du = wave_uniform((int)src1);
@@ -866,7 +886,7 @@ __device__ __forceinline__ void load_tri_cv(const TriSource &T, const float4 *R,
     }
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        v[j].cx = q[j][0].x; v[j].cy = q[j][0].y; v[j].cz = q[j][0].z; v[j].cw = q[j][0].w;
+        v[j].oow = q[j][0].z;
         v[j].wx = q[j][1].x; v[j].wy = q[j][1].y; v[j].wz = q[j][1].z; v[j].nx = q[j][1].w;
         v[j].ny = q[j][2].x; v[j].nz = q[j][2].y; v[j].u = q[j][2].z; v[j].v = q[j][2].w;
         v[j].tx = q[j][3].x; v[j].ty = q[j][3].y; v[j].tz = q[j][3].z; v[j].tw = q[j][3].w;
@@ -886,7 +906,7 @@ __device__ __forceinline__ void tri_shade_attrs(const CV v[3], const float4 *R, 
     float oow[3], uw[3], vw[3];
     f3 pw[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) oow[k] = 1.0f / v[k].cw;
+    for (int k = 0; k < 3; k++) oow[k] = v[k].oow;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         uw[k] = v[k].u * oow[k];
@@ -1029,18 +1049,18 @@ __device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view,
 // inside the frustum (the fans of those that cross it come from their
 // clipped polygons, emit_clipped_fan).
 __device__ __forceinline__ Rect setup_input_triangle(const DevDraw &D, int d, const DevView *views, uint32_t t,
-                                                     const float4 *xp, uint32_t id, const SetupOut &O) {
+                                                     const float4 *xs, uint32_t id, const SetupOut &O) {
     const DevView &V = views[D.view];
     const DevScene &S = *V.scene;
     uint32_t lt = t - D.in_tri_base;
-    CV v[3];
+    SnapV v[3];
     for (int k = 0; k < 3; k++) {
         uint32_t vi = S.indices[D.idx_base + 3 * lt + k];
-        float4 q = xp[D.xv_base + vi];    // setup reads clip positions only
-        v[k].cx = q.x; v[k].cy = q.y; v[k].cz = q.z; v[k].cw = q.w;
+        float4 q = xs[D.xv_base + vi];    // the transform's per-vertex snap
+        v[k].px = __float_as_int(q.x); v[k].py = __float_as_int(q.y); v[k].oow = q.z; v[k].zow = q.w;
     }
-    return emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, S.materials[D.material].flags,
-                         !(D.flags & SR_DRAW_OPAQUE), t, (uint32_t)d, O);
+    return emit_triangle(V, D.view, v, id, D.material, S.materials[D.material].flags, !(D.flags & SR_DRAW_OPAQUE),
+                         t, (uint32_t)d, O);
 }
 
 // Fan triangle f, (v0, v_f+1, v_f+2), of the clipped polygon in clip-queue
@@ -1049,18 +1069,14 @@ __device__ __forceinline__ Rect emit_clipped_fan(const DevView *views, const Dev
                                                   uint32_t f, uint32_t id, const SetupOut &O) {
     const DevView &V = views[D.view];
     const DevScene &S = *V.scene;
-    CV v[3];
+    SnapV v[3];
     const uint32_t src[3] = {0u, f + 1u, f + 2u};
     for (int j = 0; j < 3; j++) {
-        const float4 *q = reinterpret_cast<const float4 *>(buf + ((size_t)e * CLIP_MAXV + src[j]) * 16);
-        float4 a = q[0], b = q[1], cc = q[2], dd = q[3];
-        v[j].cx = a.x; v[j].cy = a.y; v[j].cz = a.z; v[j].cw = a.w;
-        v[j].wx = b.x; v[j].wy = b.y; v[j].wz = b.z; v[j].nx = b.w;
-        v[j].ny = cc.x; v[j].nz = cc.y; v[j].u = cc.z; v[j].v = cc.w;
-        v[j].tx = dd.x; v[j].ty = dd.y; v[j].tz = dd.z; v[j].tw = dd.w;
+        float4 a = *reinterpret_cast<const float4 *>(buf + ((size_t)e * CLIP_MAXV + src[j]) * 16);
+        v[j] = snap_vertex(V.Wf, V.Hf, a.x, a.y, a.z, a.w);
     }
-    return emit_triangle(V, D.view, &v[0], &v[1], &v[2], id, D.material, S.materials[D.material].flags,
-                         !(D.flags & SR_DRAW_OPAQUE), SRC_CLIPPED | (f << 27) | e, 0u, O);
+    return emit_triangle(V, D.view, v, id, D.material, S.materials[D.material].flags, !(D.flags & SR_DRAW_OPAQUE),
+                         SRC_CLIPPED | (f << 27) | e, 0u, O);
 }
 
 // The post-clip id of input triangle t is its chunk's first id plus the
@@ -1069,7 +1085,7 @@ __device__ __forceinline__ Rect emit_clipped_fan(const DevView *views, const Dev
 // clip positions; one that crosses it has its fan triangles set up from
 // the polygon k_clip left in the clip buffer (ids tri_base[t] + f).
 __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDraw *draws, int ndraws, uint32_t total,
-                                               const float4 *xp, const uint32_t *counts, const uint32_t *chunk,
+                                               const float4 *xs, const uint32_t *counts, const uint32_t *chunk,
                                                const float *clip_buf, uint32_t *tri_base, SetupOut O) {
     __shared__ uint32_t sh[17];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1088,10 +1104,10 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
             int du = wave_uniform(d);
             if (du >= 0) {
                 view = draws[du].view;
-                if (c == 1u) r = setup_input_triangle(draws[du], du, views, t, xp, id, O);
+                if (c == 1u) r = setup_input_triangle(draws[du], du, views, t, xs, id, O);
             } else {
                 view = draws[d].view;
-                if (c == 1u) r = setup_input_triangle(draws[d], d, views, t, xp, id, O);
+                if (c == 1u) r = setup_input_triangle(draws[d], d, views, t, xs, id, O);
             }
         }
         bin_rect_count(views, view, r, O.bin_count);
@@ -1129,6 +1145,150 @@ __global__ void __launch_bounds__(256) k_scatter(const DevView *views, const uin
             }
         }
         bin_rect_scatter(views, (int)view, r, id, cursor, list, pair_cap, status_w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_bin_sort: every bin's list in submission order
+// ---------------------------------------------------------------------------
+// The scatter claims list slots with atomics, so a bin's entries arrive in
+// any order.  Record ids are the canonical submission order (node distance,
+// opaque then translucent primitives, triangle, clip-fan index) and a record
+// appears at most once in a bin, so sorting each bin's segment ascending
+// restores exactly the order in which the reference's single-threaded front
+// end pushes packets into a tile queue (src/bumpqueue.rs:96-112).
+//
+// All sorts are the bitonic network in its all-ascending form: stage k
+// first compares i with i ^ (k - 1), then with i ^ j for j = k/4 .. 1, the
+// smaller id going to the lower index.  Positions past the list hold +inf
+// (no record has id 0xFFFFFFFF), so a power-of-two network sorts any length.
+#define BIN_SORT_LDS 4096
+
+// One wave sorts n <= 64 R ids held as v[t] = element t * 64 + lane.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint32_t v[R]) {
+    const int lane = threadIdx.x & 63;
+    constexpr int P = 64 * R;
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1) {
+        // mirror step: partner i ^ (k - 1)
+        {
+            uint32_t nv[R];
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                int m = k - 1;
+                int tp = t ^ (m >> 6), lp = lane ^ (m & 63);
+                uint32_t p = (m & 63) ? (uint32_t)__shfl((int)v[tp], lp, 64) : v[tp];
+                bool lower = (t * 64 + lane) < (tp * 64 + lp);
+                nv[t] = lower ? min(v[t], p) : max(v[t], p);
+            }
+#pragma unroll
+            for (int t = 0; t < R; t++) v[t] = nv[t];
+        }
+#pragma unroll
+        for (int j = k >> 2; j > 0; j >>= 1) {
+            uint32_t nv[R];
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                int tp = t ^ (j >> 6), lp = lane ^ (j & 63);
+                uint32_t p = (j & 63) ? (uint32_t)__shfl_xor((int)v[tp], j & 63, 64) : v[tp];
+                bool lower = (t * 64 + lane) < (tp * 64 + lp);
+                nv[t] = lower ? min(v[t], p) : max(v[t], p);
+            }
+#pragma unroll
+            for (int t = 0; t < R; t++) v[t] = nv[t];
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_sort_list(uint32_t *list, uint32_t n) {
+    const int lane = threadIdx.x & 63;
+    uint32_t v[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        uint32_t i = (uint32_t)(t * 64 + lane);
+        v[t] = i < n ? list[i] : 0xFFFFFFFFu;
+    }
+    wave_bitonic<R>(v);
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        uint32_t i = (uint32_t)(t * 64 + lane);
+        if (i < n) list[i] = v[t];
+    }
+}
+
+// The whole workgroup sorts one list of n > 256 ids: in LDS up to
+// BIN_SORT_LDS, else in place in global memory (every access of a step is
+// ordered by the barrier between steps; the workgroup's waves share one CU).
+__device__ void group_sort_list(uint32_t *list, uint32_t n, uint32_t *sk) {
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    const uint32_t nt = blockDim.x;
+    if (P <= BIN_SORT_LDS) {
+        for (uint32_t i = threadIdx.x; i < P; i += nt) sk[i] = i < n ? list[i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= P; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                uint32_t m = j == (k >> 1) ? k - 1 : j;
+                for (uint32_t i = threadIdx.x; i < P; i += nt) {
+                    uint32_t l = i ^ m;
+                    if (l > i) {
+                        uint32_t a = sk[i], b = sk[l];
+                        if (a > b) {
+                            sk[i] = b;
+                            sk[l] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t i = threadIdx.x; i < n; i += nt) list[i] = sk[i];
+        __syncthreads();
+        return;
+    }
+    for (uint32_t k = 2; k <= P; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            uint32_t m = j == (k >> 1) ? k - 1 : j;
+            for (uint32_t i = threadIdx.x; i < P; i += nt) {
+                uint32_t l = i ^ m;
+                if (l > i && l < n) {       // l >= n is +inf: no exchange
+                    uint32_t a = list[i], b = list[l];
+                    if (a > b) {
+                        list[i] = b;
+                        list[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Workgroups of 256 take four consecutive bins at a time, one per wave;
+// lists longer than 256 are then sorted by the whole workgroup in turn.
+__global__ void __launch_bounds__(256) k_bin_sort(const uint32_t *bin_start, const uint32_t *bin_end, uint32_t *list,
+                                                  uint32_t items, uint32_t pair_cap) {
+    __shared__ uint32_t sk[BIN_SORT_LDS];
+    __shared__ uint32_t big_s[4], big_n[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t g = blockIdx.x; g * 4u < items; g += gridDim.x) {
+        uint32_t b = g * 4u + (uint32_t)wave;
+        uint32_t s = 0, n = 0;
+        if (b < items) {
+            s = min(bin_start[b], pair_cap);
+            n = min(bin_end[b], pair_cap) - s;
+        }
+        if (n > 1 && n <= 64) wave_sort_list<1>(list + s, n);
+        else if (n > 64 && n <= 128) wave_sort_list<2>(list + s, n);
+        else if (n > 128 && n <= 256) wave_sort_list<4>(list + s, n);
+        if (lane == 0) {
+            big_s[wave] = s;
+            big_n[wave] = n > 256 ? n : 0u;
+        }
+        __syncthreads();
+        for (int w = 0; w < 4; w++)
+            if (big_n[w]) group_sort_list(list + big_s[w], big_n[w], sk);
+        __syncthreads();
     }
 }
 
@@ -1221,13 +1381,18 @@ __device__ __forceinline__ float chain_fast(float v, float sy, float sx, float f
     return r2;
 }
 
-__device__ __forceinline__ float edge_chain(const TriSetup &S, int e, int bx, int by, int sx, int sy, int i, int j) {
+__device__ __forceinline__ float edge_chain_v(float A, float B, float C, float SX, float SY, int bx, int by, int sx,
+                                             int sy, int i, int j) {
     float X = (float)(bx * 16 + 8 + 16 * sx);
     float Y = (float)(by * 16 + 8 + 16 * sy);
-    float v = (S.A[e] * X + S.B[e] * Y) + S.C[e];
+    float v = (A * X + B * Y) + C;
     bool ok = true;
-    float r = chain_fast(v, S.SY[e], S.SX[e], (float)j, (float)i, ok);
-    return ok ? r : chain_serial(v, S.SY[e], S.SX[e], j, i);
+    float r = chain_fast(v, SY, SX, (float)j, (float)i, ok);
+    return ok ? r : chain_serial(v, SY, SX, j, i);
+}
+
+__device__ __forceinline__ float edge_chain(const TriSetup &S, int e, int bx, int by, int sx, int sy, int i, int j) {
+    return edge_chain_v(S.A[e], S.B[e], S.C[e], S.SX[e], S.SY[e], bx, by, sx, sy, i, j);
 }
 
 // ---------------------------------------------------------------------------
@@ -1381,17 +1546,22 @@ __device__ __forceinline__ void voxel_lookup(const DevScene &S, f3 p, f3 rgb[4],
 // b1[4], b2[4]: the quad's four barycentric pairs; lane: this pixel's lane.
 // TR: the translucent form, which blends over cur (src/shader.rs:66-76).
 // ---------------------------------------------------------------------------
+// The quad's four w = 1 / interpolated 1/w of this pixel's triangle at each
+// lane's barycentrics (src/shader.rs:123-130: du_dv takes all four lanes'
+// w), from the quad's barycentric pairs b1[4], b2[4].
+__device__ __forceinline__ void quad_w(float oa, float oda, float odb, const float b1[4], const float b2[4],
+                                       float wv[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) wv[j] = 1.0f / ((oa + b1[j] * oda) + b2[j] * odb);
+}
+
+// B1, B2: this lane's barycentrics; wv: the quad's four w (quad_w).
 template <bool TR>
-__device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, const float4 *R, const float4 *Sh, const float4 *Tn, bool nmap,
-                          const float b1[4], const float b2[4], int lane, f3 cur) {
-    float4 r1 = R[1], r2 = R[2], r3 = R[3];
-    float oa = r1.w, oda = r2.x, odb = r2.y;
-    int material = __float_as_int(r3.y);
+__device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, int material, const float4 *Sh,
+                                          const float4 *Tn, bool nmap, float B1, float B2, const float wv[4], int lane,
+                                          f3 cur) {
     const DevMaterial &M = S.materials[material];
     float4 s0 = Sh[0], s1 = Sh[1], s2 = Sh[2], s3 = Sh[3], s4 = Sh[4], s5 = Sh[5], s6 = Sh[6];
-    float wv[4];
-    for (int j = 0; j < 4; j++) wv[j] = 1.0f / ((oa + b1[j] * oda) + b2[j] * odb);
-    float B1 = b1[lane], B2 = b2[lane];
     float w = wv[lane];
     f3 na = mk3(s0.x, s0.y, s0.z), nda = mk3(s0.w, s1.x, s1.y), ndb = mk3(s1.z, s1.w, s2.x);
     f3 pa = mk3(s2.y, s2.z, s2.w), pda = mk3(s3.x, s3.y, s3.z), pdb = mk3(s3.w, s4.x, s4.y);
@@ -2016,94 +2186,237 @@ template <int K> __device__ __forceinline__ float quad_lane(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
 }
 
+// The tile's triangle table: when a tile shows at most CAP distinct
+// triangles, each is set up once, by one thread, into an LDS row (TT_WORDS
+// words: the two edges and the depth interpolators the barycentric chains
+// need, then the shading interpolators and tangent frame of
+// tri_shade_attrs), and every pixel of the triangle reads its row.  Ids are
+// deduplicated through an open-addressing hash of 2 x NT slots and ranked
+// densely.  Denser tiles (a triangle or two per pixel, where a row would
+// serve almost no other pixel) set up each pixel's triangle in the pixel's
+// thread.
+#define TT_WORDS 60
+template <int NT> struct ShadeLds {
+    static constexpr int HS = 2 * NT;                         // hash slots
+    // table rows: 256-thread workgroups stay within 32 KB of LDS, so five
+    // fit a CU's 160 KB (five waves per SIMD, as the VGPR budget allows)
+    static constexpr int CAP = NT == 64 ? 32 : (NT == 256 ? 112 : 256);
+    uint32_t key[HS];
+    uint16_t rank[HS];
+    uint32_t tri[CAP];
+    float4 tab[CAP][TT_WORDS / 4];
+    uint32_t wsum[NT / 64];
+};
+
+__device__ __forceinline__ uint32_t tt_hash(uint32_t id, int hs) { return (id * 2654435761u) & (uint32_t)(hs - 1); }
+
+// Row layout: [0] A1 B1 C1 SX1, [1] SY1 A2 B2 C2, [2] SX2 SY2 ooa oa,
+// [3] oda odb za zda, [4] zdb origin(ox0 | oy0 << 16) flags(coarse 1,
+// normal map 2) material, [5..11] shading interpolators Sh[0..6],
+// [12..14] tangent frame Tn[0..2] (normal-mapped triangles only).
+__device__ __forceinline__ void tt_fill(const RasterArgs &P, uint32_t id, int tx0, int ty0, float4 *row) {
+    const float4 *R = P.raster + 4 * (size_t)id;
+    uint4 g = P.geo[id];
+    bool nmap = (g.z & SR_GEO_NORMAL_MAP) != 0;
+    {
+        TriSetup ts;
+        tri_setup(g, R, tx0, ty0, ts);
+        row[0] = make_float4(ts.A[1], ts.B[1], ts.C[1], ts.SX[1]);
+        row[1] = make_float4(ts.SY[1], ts.A[2], ts.B[2], ts.C[2]);
+        row[2] = make_float4(ts.SX[2], ts.SY[2], ts.ooa, ts.oa);
+        row[3] = make_float4(ts.oda, ts.odb, ts.za, ts.zda);
+        row[4] = make_float4(ts.zdb, __uint_as_float((uint32_t)ts.ox0 | ((uint32_t)ts.oy0 << 16)),
+                             __uint_as_float((ts.coarse ? 1u : 0u) | (nmap ? 2u : 0u)), R[3].y);
+    }
+    CV tv[3];
+    float4 Sh[7], Tn[3];
+    load_tri_cv(P.src, R, tv);
+    tri_shade_attrs(tv, R, nmap, Sh, Tn);
+#pragma unroll
+    for (int k = 0; k < 7; k++) row[5 + k] = Sh[k];
+    if (nmap) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) row[12 + k] = Tn[k];
+    }
+}
+
 template <int T, int NT>
 __device__ void shade_item(const RasterArgs &P, uint32_t item) {
     constexpr int QW = T / 2;                 // quads per row
-    // the DPP quad exchanges below read all four lanes of each quad: every
-    // lane must be active, so the pixel loop has no partial wave
-    static_assert((T * T) % NT == 0 && NT % 64 == 0, "one thread per pixel in whole waves");
-    const int tid = threadIdx.x, lane = tid & 63;
+    // one thread per pixel, in whole waves: the DPP quad exchanges below
+    // read all four lanes of each quad, so every lane must be active
+    static_assert(T * T == NT && NT % 64 == 0, "one thread per pixel in whole waves");
+    using L = ShadeLds<NT>;
+    __shared__ L sm;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const DevView &V = P.views[find_view(P, item)];
     const DevScene &S = *V.scene;
     uint32_t tile = item - V.item_base;
     int tx0 = (int)(tile % (uint32_t)V.tiles_x) * T, ty0 = (int)(tile / (uint32_t)V.tiles_x) * T;
     uint32_t rec0 = P.tri_base[V.in_tri_base];
     bool fused = (V.flags & 1u) != 0;
-    for (int pb = 0; pb < T * T; pb += NT) {
-        int p = pb + tid;
-        int q = p >> 2, s = p & 3;
-        int qx = q % QW, qy = q / QW;
-        int sx = s & 1, sy = s >> 1;
-        int px = tx0 + 2 * qx + sx, py = ty0 + 2 * qy + sy;
-        bool on_screen = px < V.W && py < V.H;
-        bool center = centre_tile(tx0, ty0) && q == 0;
-        size_t o = V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px;
-        uint32_t vid = 0xFFFFFFFFu;
-        if (center) vid = P.center_ids[4 * (V.lum_offset + (uint32_t)((ty0 / 64) * V.ref_tiles_x + (tx0 / 64))) + s];
-        else if (on_screen) vid = P.out_ids[o];
-        bool has = vid != 0xFFFFFFFFu;
-        uint32_t id = has ? vid + rec0 : 0u;
-        float b1 = 0.0f, b2 = 0.0f, z = INFINITY;
-        if (has) {
-            uint4 g = P.geo[id];
-            TriSetup ts;
-            tri_setup(g, P.raster + 4 * (size_t)id, tx0, ty0, ts);
-            int qpx = tx0 + 2 * qx, qpy = ty0 + 2 * qy;
-            int bx = ts.ox0, by = ts.oy0;
-            quad_origin(ts, qpx, qpy, bx, by);
-            int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
-            float w1 = edge_chain(ts, 1, bx, by, sx, sy, i, j);
-            float w2 = edge_chain(ts, 2, bx, by, sx, sy, i, j);
-            b1 = w1 * ts.ooa;
-            b2 = w2 * ts.ooa;
-            float w = 1.0f / ((ts.oa + b1 * ts.oda) + b2 * ts.odb);
-            z = ((ts.za + b1 * ts.zda) + b2 * ts.zdb) * w;
+    const int p = tid;
+    int q = p >> 2, s = p & 3;
+    int qx = q % QW, qy = q / QW;
+    int sx = s & 1, sy = s >> 1;
+    int px = tx0 + 2 * qx + sx, py = ty0 + 2 * qy + sy;
+    int qpx = tx0 + 2 * qx, qpy = ty0 + 2 * qy;
+    bool on_screen = px < V.W && py < V.H;
+    bool center = centre_tile(tx0, ty0) && q == 0;
+    size_t o = V.fb_offset + (size_t)py * (size_t)V.W + (size_t)px;
+    uint32_t vid = 0xFFFFFFFFu;
+    if (center) vid = P.center_ids[4 * (V.lum_offset + (uint32_t)((ty0 / 64) * V.ref_tiles_x + (tx0 / 64))) + s];
+    else if (on_screen) vid = P.out_ids[o];
+    bool has = vid != 0xFFFFFFFFu;
+    uint32_t id = has ? vid + rec0 : 0u;
+
+    // ---- the tile's distinct triangles: hash, dense ranks, rows ----
+    for (int k = tid; k < L::HS; k += NT) sm.key[k] = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t h = 0;
+    bool owner = false;
+    if (has) {
+        h = tt_hash(id, L::HS);
+        for (;;) {      // at most NT ids in 2 NT slots: a free slot is always found
+            uint32_t old = atomicCAS(&sm.key[h], 0xFFFFFFFFu, id);
+            if (old == 0xFFFFFFFFu || old == id) {
+                owner = old == 0xFFFFFFFFu;
+                break;
+            }
+            h = (h + 1u) & (uint32_t)(L::HS - 1);
         }
-        // every lane of the wave is active here (one thread per pixel)
+    }
+    uint64_t om = __ballot(owner);
+    if (lane == 0) sm.wsum[wave] = (uint32_t)__popcll(om);
+    __syncthreads();
+    uint32_t before = 0, ntri = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        uint32_t c = sm.wsum[w];
+        if (w < wave) before += c;
+        ntri += c;
+    }
+    const bool table = ntri <= (uint32_t)L::CAP;     // the same in every thread
+    if (table && owner) {
+        uint32_t r = before + (uint32_t)__popcll(om & ((1ull << lane) - 1ull));
+        sm.rank[h] = (uint16_t)r;
+        sm.tri[r] = id;
+    }
+    __syncthreads();
+    if (table && (uint32_t)tid < ntri) tt_fill(P, sm.tri[tid], tx0, ty0, sm.tab[tid]);
+    __syncthreads();
+    const float4 *rw = sm.tab[(table && has) ? sm.rank[h] : 0];
+
+    // ---- barycentrics and depth of the visible triangle ----
+    float b1 = 0.0f, b2 = 0.0f, z = INFINITY, w = 0.0f;
+    float oa = 0.0f, oda = 0.0f, odb = 0.0f;
+    if (has) {
+        float A1, B1, C1, SX1, SY1, A2, B2, C2, SX2, SY2, ooa, za, zda, zdb;
+        int ox0, oy0;
+        bool coarse;
+        if (table) {
+            float4 r0 = rw[0], r1 = rw[1], r2 = rw[2], r3 = rw[3], r4 = rw[4];
+            A1 = r0.x; B1 = r0.y; C1 = r0.z; SX1 = r0.w;
+            SY1 = r1.x; A2 = r1.y; B2 = r1.z; C2 = r1.w;
+            SX2 = r2.x; SY2 = r2.y; ooa = r2.z; oa = r2.w;
+            oda = r3.x; odb = r3.y; za = r3.z; zda = r3.w;
+            zdb = r4.x;
+            uint32_t org = __float_as_uint(r4.y);
+            ox0 = (int)(org & 0xFFFFu);
+            oy0 = (int)(org >> 16);
+            coarse = (__float_as_uint(r4.z) & 1u) != 0u;
+        } else {
+            TriSetup ts;
+            tri_setup(P.geo[id], P.raster + 4 * (size_t)id, tx0, ty0, ts);
+            A1 = ts.A[1]; B1 = ts.B[1]; C1 = ts.C[1]; SX1 = ts.SX[1]; SY1 = ts.SY[1];
+            A2 = ts.A[2]; B2 = ts.B[2]; C2 = ts.C[2]; SX2 = ts.SX[2]; SY2 = ts.SY[2];
+            ooa = ts.ooa; oa = ts.oa; oda = ts.oda; odb = ts.odb; za = ts.za; zda = ts.zda; zdb = ts.zdb;
+            ox0 = ts.ox0; oy0 = ts.oy0; coarse = ts.coarse;
+        }
+        // the walk origin of the pixel's quad (quad_origin; the quad is walked)
+        int bx = ox0, by = oy0;
+        if (coarse) {
+            bx = ox0 + ((qpx - ox0) & ~15);
+            by = oy0 + ((qpy - oy0) & ~15);
+        }
+        int i = (qpx - bx) >> 1, j = (qpy - by) >> 1;
+        float w1 = edge_chain_v(A1, B1, C1, SX1, SY1, bx, by, sx, sy, i, j);
+        float w2 = edge_chain_v(A2, B2, C2, SX2, SY2, bx, by, sx, sy, i, j);
+        b1 = w1 * ooa;
+        b2 = w2 * ooa;
+        w = 1.0f / ((oa + b1 * oda) + b2 * odb);
+        z = ((za + b1 * zda) + b2 * zdb) * w;
+    }
+    // every lane of the wave is active here (one thread per pixel).  When
+    // the quad shows one triangle, its lanes' own w are the shading's four
+    // w bit for bit (the same expression on the same barycentrics);
+    // otherwise each lane evaluates them for its triangle.
+    float wv[4] = {quad_lane<0>(w), quad_lane<1>(w), quad_lane<2>(w), quad_lane<3>(w)};
+    {
         float qb1[4] = {quad_lane<0>(b1), quad_lane<1>(b1), quad_lane<2>(b1), quad_lane<3>(b1)};
         float qb2[4] = {quad_lane<0>(b2), quad_lane<1>(b2), quad_lane<2>(b2), quad_lane<3>(b2)};
-        f3 color = mk3(0.0f, 0.0f, 0.0f);
-        if (on_screen || center) {
-            bool covered = has && z != INFINITY;
-            if (covered) {
+        uint32_t q0 = __float_as_uint(quad_lane<0>(__uint_as_float(vid)));
+        bool one_tri = __float_as_uint(quad_lane<1>(__uint_as_float(vid))) == q0 &&
+                       __float_as_uint(quad_lane<2>(__uint_as_float(vid))) == q0 &&
+                       __float_as_uint(quad_lane<3>(__uint_as_float(vid))) == q0;
+        if (has && !one_tri) quad_w(oa, oda, odb, qb1, qb2, wv);
+    }
+    f3 color = mk3(0.0f, 0.0f, 0.0f);
+    if (on_screen || center) {
+        bool covered = has && z != INFINITY;
+        if (covered) {
+            float4 Sh[7], Tn[3];
+            bool nmap;
+            int material;
+            if (table) {
+#pragma unroll
+                for (int k = 0; k < 7; k++) Sh[k] = rw[5 + k];
+                float4 r4 = rw[4];
+                nmap = (__float_as_uint(r4.z) & 2u) != 0;
+                material = __float_as_int(r4.w);
+                if (nmap) {
+#pragma unroll
+                    for (int k = 0; k < 3; k++) Tn[k] = rw[12 + k];
+                }
+            } else {
                 const float4 *R = P.raster + 4 * (size_t)id;
                 CV tv[3];
-                float4 Sh[7], Tn[3];
+                nmap = (P.geo[id].z & SR_GEO_NORMAL_MAP) != 0;
+                material = __float_as_int(R[3].y);
                 load_tri_cv(P.src, R, tv);
-                bool nmap = (P.geo[id].z & SR_GEO_NORMAL_MAP) != 0;
                 tri_shade_attrs(tv, R, nmap, Sh, Tn);
-                color = shade_pixel<false>(S, V, R, Sh, Tn, nmap, qb1, qb2, s, color);
-            } else {
-                color = skybox_pixel(S, V, tx0 + 2 * qx, ty0 + 2 * qy, sx, sy);
             }
+            color = shade_pixel<false>(S, V, material, Sh, Tn, nmap, b1, b2, wv, s, color);
+        } else {
+            color = skybox_pixel(S, V, qpx, qpy, sx, sy);
         }
-        if (on_screen) {
-            P.out_depth[o] = z;
-            if (fused) {
-                float k = 0.2f;
-                float cr = color.x * V.exposure, cg = color.y * V.exposure, cb = color.z * V.exposure;
-                cr = (cr / (cr + k)) * (1.0f + k);
-                cg = (cg / (cg + k)) * (1.0f + k);
-                cb = (cb / (cb + k)) * (1.0f + k);
-                P.out_rgba[o] = (sat_u8(cr * 255.0f) << 24) | (sat_u8(cg * 255.0f) << 16) |
-                                (sat_u8(cb * 255.0f) << 8) | 0xFFu;
-            } else {
-                P.out_color[o] = color.x;
-                P.out_color[P.color_plane + o] = color.y;
-                P.out_color[2 * P.color_plane + o] = color.z;
-            }
+    }
+    if (on_screen) {
+        P.out_depth[o] = z;
+        if (fused) {
+            float k = 0.2f;
+            float cr = color.x * V.exposure, cg = color.y * V.exposure, cb = color.z * V.exposure;
+            cr = (cr / (cr + k)) * (1.0f + k);
+            cg = (cg / (cg + k)) * (1.0f + k);
+            cb = (cb / (cb + k)) * (1.0f + k);
+            P.out_rgba[o] = (sat_u8(cr * 255.0f) << 24) | (sat_u8(cg * 255.0f) << 16) | (sat_u8(cb * 255.0f) << 8) |
+                            0xFFu;
+        } else {
+            P.out_color[o] = color.x;
+            P.out_color[P.color_plane + o] = color.y;
+            P.out_color[2 * P.color_plane + o] = color.z;
         }
-        if (center && P.center_color) {
-            int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
-            P.center_color[4 * (V.lum_offset + (uint32_t)ri) + s] = make_float4(color.x, color.y, color.z, z);
-        }
-        // centre-quad luminance of the 64px tile (src/tilerasterizer.rs:103-106)
-        float lum = (color.x * 0.2126f + color.y * 0.7152f) + color.z * 0.0722f;
-        float l0 = quad_lane<0>(lum), l1 = quad_lane<1>(lum), l2 = quad_lane<2>(lum), l3 = quad_lane<3>(lum);
-        if (center && s == 0) {
-            int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
-            P.out_lum[V.lum_offset + (uint32_t)ri] = (((l0 + l1) + l2) + l3) * 0.25f;
-        }
+    }
+    if (center && P.center_color) {
+        int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
+        P.center_color[4 * (V.lum_offset + (uint32_t)ri) + s] = make_float4(color.x, color.y, color.z, z);
+    }
+    // centre-quad luminance of the 64px tile (src/tilerasterizer.rs:103-106)
+    float lum = (color.x * 0.2126f + color.y * 0.7152f) + color.z * 0.0722f;
+    float l0 = quad_lane<0>(lum), l1 = quad_lane<1>(lum), l2 = quad_lane<2>(lum), l3 = quad_lane<3>(lum);
+    if (center && s == 0) {
+        int ri = (ty0 / 64) * V.ref_tiles_x + (tx0 / 64);
+        P.out_lum[V.lum_offset + (uint32_t)ri] = (((l0 + l1) + l2) + l3) * 0.25f;
     }
 }
 
@@ -2501,7 +2814,11 @@ __global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
                     load_tri_cv(P.src, R, tv);
                     bool nmap = (P.geo[id].z & SR_GEO_NORMAL_MAP) != 0;
                     tri_shade_attrs(tv, R, nmap, Sh, Tn);
-                    cur = shade_pixel<true>(S, V, R, Sh, Tn, nmap, qb1, qb2, sl, cur);
+                    float wv[4];
+                    quad_w(__uint_as_float(st[G_OA * 256 + k]), __uint_as_float(st[G_ODA * 256 + k]),
+                           __uint_as_float(st[G_ODB * 256 + k]), qb1, qb2, wv);
+                    cur = shade_pixel<true>(S, V, __float_as_int(R[3].y), Sh, Tn, nmap, qb1[sl], qb2[sl], wv, sl,
+                                            cur);
                 }
             }
         }
@@ -2636,7 +2953,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (tri_blocks + vtx_blocks > 0)
         hipLaunchKernelGGL(k_front, dim3(tri_blocks + vtx_blocks), dim3(256), 0, st, a.views, a.draws, a.num_draws,
                            a.total_in, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr,
-                           a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap, a.status, fb, tri_blocks);
+                           a.xv + 3 * (size_t)a.total_xv, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
+                           a.status, fb, tri_blocks);
     else
         hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, fb);
     if (a.total_in > 0) {
@@ -2664,19 +2982,23 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     so.T = a.tile_size;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv, a.tri_counts, a.tri_chunks, a.clip_buf, a.tri_base, so);
+                           a.num_draws, a.total_in, a.xv + 3 * (size_t)a.total_xv, a.tri_counts, a.tri_chunks,
+                           a.clip_buf, a.tri_base, so);
     }
     if (a.events[1]) hipEventRecord(a.events[1], st);
     e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(a.rec_cap, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
                        a.bin_end, a.list, a.pair_cap, a.status, a.tile_size);
+    hipLaunchKernelGGL(k_bin_sort, dim3(grid_for((a.total_items + 3) / 4, 1, GRID_CAP)), dim3(256), 0, st, a.bin_start,
+                       a.bin_end, a.list, a.total_items, a.pair_cap);
     TriSource src;
     src.views = (const SR_GLOBAL DevView *)a.views;
     src.draws = (const SR_GLOBAL DevDraw *)a.draws;
     src.xp = (const SR_GLOBAL float4 *)a.xv;
     src.xa = (const SR_GLOBAL float4 *)(a.xv + a.total_xv);
     src.xt = (const SR_GLOBAL float4 *)(a.any_tangents ? a.xt : nullptr);
+    src.xs = (const SR_GLOBAL float4 *)(a.xv + 3 * (size_t)a.total_xv);
     src.clip_buf = (const SR_GLOBAL float *)a.clip_buf;
     if (a.has_translucent) {
         uint32_t g = grid_for(a.rec_cap, 256, GRID_CAP);

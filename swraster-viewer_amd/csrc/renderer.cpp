@@ -426,6 +426,23 @@ SR_API int sr_renderer_get_stats(const sr_renderer *r, sr_frame_stats *stats) {
     return SR_OK;
 }
 
+SR_API int sr_renderer_read_bins(sr_renderer *r, uint32_t *start, uint32_t *end, uint32_t *list, uint64_t list_len,
+                                 uint32_t *items) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (int rc = finish(r)) return rc;
+    if (!r->has_frame) return fail(SR_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    hipSetDevice(r->device);
+    size_t n = r->p_items;
+    if (items) *items = (uint32_t)n;
+    if (start) HIP_TRY(hipMemcpy(start, r->pipe.bin_start.ptr, n * 4, hipMemcpyDeviceToHost));
+    if (end) HIP_TRY(hipMemcpy(end, r->pipe.bin_end.ptr, n * 4, hipMemcpyDeviceToHost));
+    if (list && list_len) {
+        uint64_t total = std::min<uint64_t>(r->pipe.last_pairs, r->pipe.pair_cap);
+        HIP_TRY(hipMemcpy(list, r->pipe.list.ptr, std::min<uint64_t>(total, list_len) * 4, hipMemcpyDeviceToHost));
+    }
+    return SR_OK;
+}
+
 SR_API int sr_renderer_device_buffers(sr_renderer *r, void **tri_id, void **depth, void **color_rgb) {
     if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
     if (int rc = finish(r)) return rc;

@@ -284,7 +284,7 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     init.seq = ++seq;
     memcpy((char *)stage.ptr + vbytes + dbytes, &init, sizeof(init));
     if ((r = views.ensure(upload))) return r;
-    if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 48))) return r;
+    if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 64))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
     if ((r = tri_counts.ensure(((size_t)total_in + 1) * 4))) return r;
     if ((r = tri_chunks.ensure(((size_t)total_in / 256 + 2) * 4))) return r;

@@ -204,7 +204,8 @@ static int blit_views(Slot &s) {
 static uint64_t bin_estimate(const sr_scene *s, const sr_request &r) {
     // setup records (80 B) and clip/scan state per input triangle, one 4-byte
     // bin entry per triangle and about one per pixel, transformed vertices
-    return s->triangles * 96 + (uint64_t)r.width * r.height * 4 + s->vertices * 48;
+    // (clip position, attributes and snap: 64 B)
+    return s->triangles * 96 + (uint64_t)r.width * r.height * 4 + s->vertices * 64;
 }
 
 bool sr_server::launch_batch(Slot &s, std::unique_lock<std::mutex> &lk) {
