@@ -441,6 +441,24 @@ SR_API int sr_renderer_read_color(sr_renderer *renderer, float *rgb);
 SR_API int sr_renderer_read_tile_luminance(sr_renderer *renderer, float *lum, int32_t *count);
 SR_API int sr_renderer_get_stats(const sr_renderer *renderer, sr_frame_stats *stats);
 
+/* One frame without allocating device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Renders
+ * `scene` from `camera` at width x height into the caller's device buffers
+ * (width*height each): tri_id and depth as sr_renderer_read_vbuffer gives
+ * them, rgba as sr_renderer_blit_to_buffer packs the frame with the given
+ * exposure (<= 0: the reference's initial 2.0; every row, also an odd last
+ * one), on `stream` (a hipStream_t; NULL: the null stream), and waits for
+ * it.  The frame's transient buffers come from one device array of
+ * SR_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its first
+ * call on that GPU until the process ends; calls that overlap on one GPU
+ * race on it.  SR_ERR_CAPACITY: the frame's buffers do not fit (checked
+ * before launching for the sizes known up front, after it for the records,
+ * list entries and clipped triangles the frame produced).
+ * SR_ERR_UNSUPPORTED: translucent primitives or normal-mapped materials. */
+#define SR_NOALLOC_SCRATCH_BYTES (160ull << 20)
+SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camera, int32_t width, int32_t height,
+                                   float exposure, uint32_t *tri_id, float *depth, uint32_t *rgba, void *stream);
 /* Parity hook: the last frame's per-tile lists (bins) of set-up triangle
  * ids, the form of the reference's per-tile packet queues
  * (src/bumpqueue.rs:96-112).  *items receives the number of screen tiles;

@@ -197,6 +197,7 @@ _SIGS = {
     "sr_renderer_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(sr_renderer_opts), C.POINTER(_P)]),
     "sr_renderer_destroy": (C.c_int, [_P]),
     "sr_renderer_render_scene": (C.c_int, [_P, _P, C.POINTER(sr_camera)]),
+    "sr_render_frame_noalloc": (C.c_int, [_P, C.POINTER(sr_camera), C.c_int32, C.c_int32, C.c_float, _P, _P, _P, _P]),
     "sr_renderer_update_auto_exposure": (C.c_int, [_P, C.c_float]),
 "sr_renderer_update_auto_exposure_device": (C.c_int, [_P, C.c_float]),
     "sr_renderer_blit_to_buffer": (C.c_int, [_P, C.POINTER(C.c_uint32), C.c_size_t, C.c_size_t]),
