@@ -146,3 +146,12 @@ struct DevStatus {
     uint32_t overflow_tr;
     uint32_t seq;                 // the frame's number, set by the host in the upload
 };
+
+// Working state of the stable binning (radix passes over the bin index),
+// uploaded zeroed right after the status: the digit histograms of the
+// passes before the last and the expansion's total.
+struct BinSortState {
+    uint32_t digit_hist[2][256];
+    uint32_t chunk_total;         // pairs of all records (the expansion's size)
+    uint32_t pad[3];
+};

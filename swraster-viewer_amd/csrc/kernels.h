@@ -29,7 +29,12 @@ struct FrameArgs {
     uint32_t *bin_start;          // total_items
     uint32_t *bin_end;            // total_items (cursor, then end)
     uint32_t *scan_partials;      // scratch for scans
-    uint32_t *list;               // pair_cap
+    uint32_t *list;               // pair_cap record ids, each bin's in submission order
+    uint32_t *chunk_pairs;              // (total_in + 255) / 256 + 1: pairs of each setup chunk, then first pair
+    uint32_t *pair_keys, *pair_vals;    // pair_cap each: (bin, record) pairs in record order, then every
+    uint32_t *pair_keys2, *pair_vals2;  // pair_cap each: other radix pass's output (two passes or more)
+    uint32_t *sort_hist;                // 256 * bin_sort_tiles(pair_cap): per-tile digit counts, then offsets
+    BinSortState *sort;                 // zeroed by the upload, right after the status
     uint4 *rec_geo;               // rec_cap
     float4 *rec_raster;           // rec_cap * 4
     uint32_t rec_cap, pair_cap;
@@ -61,14 +66,26 @@ struct FrameArgs {
     hipEvent_t events[5];         // optional stage markers: start, setup done, bins done, shade done, raster done
 };
 
+// Stable binning sizes: bits of the bin index, radix passes (8 bits each;
+// 0 if more than 3 would be needed, i.e. 2^24 bins or more), and the
+// 4096-pair tiles of a pass.
+int bin_sort_bits(uint32_t bins);
+int bin_sort_passes(uint32_t bins);
+uint32_t bin_sort_tiles(uint32_t pair_cap);
+
 // Launch the whole per-frame sequence on a stream.  Returns hipSuccess or the
 // first launch error.
 hipError_t launch_frame(const FrameArgs &a, hipStream_t stream);
 
-// Exclusive scan of n u32 (out, and out2 if not null); the total goes to
-// *total (device).  partials holds (n + 4095) / 4096 + 1 words.
+// Exclusive scan of n u32 (out, and out2 if not null: the same, or the
+// inclusive scan when flags has bit 0); the total goes to *total (device).
+// partials holds (n + 4095) / 4096 + 1 words.  With digit_passes > 0, the
+// input is a bin-count array and digit_hist (256 * digit_passes words,
+// zeroed beforehand) receives its radix digit histograms (bits 8p..8p+7 of
+// the bin index for pass p).
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
-                       uint32_t *total, hipStream_t stream);
+                       uint32_t *total, hipStream_t stream, uint32_t flags = 0, uint32_t *digit_hist = nullptr,
+                       int digit_passes = 0);
 
 // Exposure + tonemap + quantize + pack of one view's linear colour planes;
 // frame_bytes stores each pixel as the bytes r, g, b, a.

@@ -10,7 +10,9 @@
 //   k_scan_*        device-wide exclusive scan (post-clip ids, bin offsets)
 //   k_setup         clipping with attributes, snap, cull, bbox, setup records,
 //                   per-tile bin counts     src/renderer.rs:668-846
-//   k_scatter       (triangle, tile) pairs into per-tile lists
+//   k_bin_expand    stable binning: the records' pairs in record order
+//   k_bin_hist, k_bin_rowscan, k_bin_scatter
+//                   stable binning: radix passes over the bin index
 //                                           src/bumpqueue.rs:96-112
 //   k_raster_shade  one workgroup per screen tile: coverage with the
 //                   reference's exact f32 edge-evaluation chains into a 64-bit
@@ -531,13 +533,12 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t &excl, 
 }
 
 // Each wave owns a contiguous segment of 64 x SCAN_ITEMS elements and reads
-// it in SCAN_ITEMS coalesced rows (element seg + k * 64 + lane); ex[k] gets
-// the exclusive prefix of its element within the segment.  Returns the
-// segment total.
+// it in SCAN_ITEMS coalesced rows (element seg + k * 64 + lane); v[k] gets
+// the element and ex[k] its exclusive prefix within the segment.  Returns
+// the segment total.
 __device__ __forceinline__ uint32_t wave_scan_segment(const uint32_t *in, size_t seg, size_t n,
-                                                      uint32_t ex[SCAN_ITEMS]) {
+                                                      uint32_t ex[SCAN_ITEMS], uint32_t v[SCAN_ITEMS]) {
     const int lane = threadIdx.x & 63;
-    uint32_t v[SCAN_ITEMS];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         size_t i = seg + (size_t)k * 64 + (size_t)lane;
@@ -553,15 +554,39 @@ __device__ __forceinline__ uint32_t wave_scan_segment(const uint32_t *in, size_t
     return carry;
 }
 
+// out gets the exclusive prefix; out2 (if not null) the same, or the
+// inclusive one with SCAN_OUT2_INCLUSIVE.
+#define SCAN_OUT2_INCLUSIVE 1u
+
 __device__ __forceinline__ void wave_store_segment(uint32_t *out, uint32_t *out2, size_t seg, size_t n, uint32_t base,
-                                                   const uint32_t ex[SCAN_ITEMS]) {
+                                                   const uint32_t ex[SCAN_ITEMS], const uint32_t v[SCAN_ITEMS],
+                                                   uint32_t flags) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         size_t i = seg + (size_t)k * 64 + (size_t)lane;
         if (i < n) {
             out[i] = base + ex[k];
-            if (out2) out2[i] = base + ex[k];
+            if (out2) out2[i] = base + ex[k] + ((flags & SCAN_OUT2_INCLUSIVE) ? v[k] : 0u);
+        }
+    }
+}
+
+// Digit histograms of a scanned bin-count array for the stable binning's
+// radix passes before the last: dh[p * 256 + d] = the number of pairs whose
+// bin has digit d in bits [8p, 8p + 8).  Element i of the array is bin i,
+// so digit 0 of its 64-aligned rows is the lane and digit 1 is uniform over
+// a row.
+__device__ __forceinline__ void digit_hist_segment(uint32_t *lh, int passes, size_t seg, const uint32_t v[SCAN_ITEMS]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        size_t row = seg + (size_t)k * 64;
+        if (v[k]) atomicAdd(&lh[(row + (size_t)lane) & 255u], v[k]);
+        if (passes > 1) {
+            uint32_t inc = wave_incl_scan(v[k]);
+            uint32_t sum = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+            if (lane == 0 && sum) atomicAdd(&lh[256 + ((row >> 8) & 255u)], sum);
         }
     }
 }
@@ -600,17 +625,27 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t 
 }
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, uint32_t n, const uint32_t *partials,
-                                                           uint32_t *out, uint32_t *out2) {
+                                                           uint32_t *out, uint32_t *out2, uint32_t flags,
+                                                           uint32_t *dh, int dh_passes) {
     __shared__ uint32_t sh[SCAN_BLOCK / 64];
+    __shared__ uint32_t lh[512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (dh)
+        for (int j = threadIdx.x; j < 512; j += SCAN_BLOCK) lh[j] = 0u;
     size_t seg = (size_t)blockIdx.x * SCAN_TILE + (size_t)wave * (64 * SCAN_ITEMS);
-    uint32_t ex[SCAN_ITEMS];
-    uint32_t tot = wave_scan_segment(in, seg, n, ex);
+    uint32_t ex[SCAN_ITEMS], v[SCAN_ITEMS];
+    uint32_t tot = wave_scan_segment(in, seg, n, ex, v);
     if (lane == 0) sh[wave] = tot;
     __syncthreads();
     uint32_t base = partials[blockIdx.x];
     for (int w = 0; w < wave; w++) base += sh[w];
-    wave_store_segment(out, out2, seg, n, base, ex);
+    wave_store_segment(out, out2, seg, n, base, ex, v, flags);
+    if (dh) {
+        digit_hist_segment(lh, dh_passes, seg, v);
+        __syncthreads();
+        for (int j = threadIdx.x; j < 256 * dh_passes; j += SCAN_BLOCK)
+            if (lh[j]) atomicAdd(&dh[j], lh[j]);
+    }
 }
 
 // One workgroup scans arrays of up to SCAN_SINGLE_MAX elements in a single
@@ -619,14 +654,20 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, u
 #define SCAN_SINGLE_MAX 16384u
 
 __global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2,
-                                                      uint32_t *total_out) {
+                                                      uint32_t *total_out, uint32_t flags, uint32_t *dh,
+                                                      int dh_passes) {
     __shared__ uint32_t sh[16];
+    __shared__ uint32_t lh[512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (dh) {
+        for (int j = threadIdx.x; j < 512; j += 1024) lh[j] = 0u;
+        __syncthreads();
+    }
     uint32_t carry = 0;
     for (uint32_t c = 0; c < n; c += 1024u * SCAN_ITEMS) {
         size_t seg = (size_t)c + (size_t)wave * (64 * SCAN_ITEMS);
-        uint32_t ex[SCAN_ITEMS];
-        uint32_t tot = wave_scan_segment(in, seg, n, ex);
+        uint32_t ex[SCAN_ITEMS], v[SCAN_ITEMS];
+        uint32_t tot = wave_scan_segment(in, seg, n, ex, v);
         if (lane == 0) sh[wave] = tot;
         __syncthreads();
         uint32_t base = carry, all = 0;
@@ -634,23 +675,29 @@ __global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32
             if (w < wave) base += sh[w];
             all += sh[w];
         }
-        wave_store_segment(out, out2, seg, n, base, ex);
+        wave_store_segment(out, out2, seg, n, base, ex, v, flags);
+        if (dh) digit_hist_segment(lh, dh_passes, seg, v);
         carry += all;
         __syncthreads();
     }
     if (threadIdx.x == 0 && total_out) *total_out = carry;
+    if (dh)
+        for (int j = threadIdx.x; j < 256 * dh_passes; j += 1024) dh[j] = lh[j];
 }
 
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
-                       uint32_t *total, hipStream_t st) {
+                       uint32_t *total, hipStream_t st, uint32_t flags, uint32_t *digit_hist, int digit_passes) {
+    if (digit_passes <= 0) digit_hist = nullptr;
     if (n <= SCAN_SINGLE_MAX) {
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, in, n, out, out2, total);
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, in, n, out, out2, total, flags, digit_hist,
+                           digit_passes);
         return hipGetLastError();
     }
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total, nullptr);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2, flags, digit_hist,
+                       digit_passes);
     return hipGetLastError();
 }
 
@@ -682,6 +729,7 @@ struct SetupOut {
     SR_GLOBAL uint4 *geo;
     SR_GLOBAL float4 *raster;
     SR_GLOBAL uint32_t *bin_count;
+    SR_GLOBAL uint32_t *chunk_pairs;      // (triangle, tile) pairs of each 256-triangle chunk's records
     uint32_t cap;
     SR_GLOBAL DevStatus *status;
     int T;
@@ -719,16 +767,6 @@ __device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t bin) {
     uint32_t rank, group;
     wave_groups(bin, leader, rank, group);
     if (group) atomicAdd(&counts[bin], group);
-}
-
-// Returns this lane's slot (bin != ~0u).
-__device__ __forceinline__ uint32_t wave_claim(uint32_t *cursor, uint32_t bin) {
-    int leader;
-    uint32_t rank, group;
-    wave_groups(bin, leader, rank, group);
-    uint32_t base = 0;
-    if (group) base = atomicAdd(&cursor[bin], group);
-    return (uint32_t)__shfl((int)base, leader, 64) + rank;
 }
 
 __device__ __forceinline__ int32_t top_left_bias(int32_t a, int32_t b) { return (a < 0 || (a == 0 && b > 0)) ? 0 : -1; }
@@ -991,7 +1029,8 @@ __device__ __forceinline__ int flat_owner(int incl, int k, int &before) {
     return o;
 }
 
-__device__ __forceinline__ void bin_rect_count(const DevView *views, int view, Rect r, uint32_t *counts) {
+// Returns the rectangle's bin count.
+__device__ __forceinline__ int bin_rect_count(const DevView *views, int view, Rect r, uint32_t *counts) {
     RectBins b = rect_bins(views, view, r);
     int lane = threadIdx.x & 63;
     int ns = b.nb <= SMALL_RECT ? b.nb : 0;
@@ -1011,38 +1050,7 @@ __device__ __forceinline__ void bin_rect_count(const DevView *views, int view, R
         int kk = k - before;
         if (k < total) atomicAdd(&counts[first + (uint32_t)((kk / rw) * stride + kk % rw)], 1u);
     }
-}
-
-__device__ __forceinline__ void bin_rect_scatter(const DevView *views, int view, Rect r, uint32_t id, uint32_t *cursor,
-                                                 uint32_t *list, uint32_t pair_cap, DevStatus *status) {
-    RectBins b = rect_bins(views, view, r);
-    int lane = threadIdx.x & 63;
-    int ns = b.nb <= SMALL_RECT ? b.nb : 0;
-    int maxs = wave_max(ns);
-    for (int k = 0; k < maxs; k++) {
-        uint32_t bin = 0xFFFFFFFFu;
-        if (k < ns) bin = b.first + (uint32_t)((k / b.rw) * b.stride + k % b.rw);
-        uint32_t pos = wave_claim(cursor, bin);
-        if (k < ns) {
-            if (pos < pair_cap) list[pos] = id;
-            else atomicOr(&status->overflow_pairs, 1u);
-        }
-    }
-    int incl = (int)wave_incl_scan((uint32_t)(b.nb > SMALL_RECT ? b.nb : 0));
-    int total = __builtin_amdgcn_readlane(incl, 63);
-    for (int k0 = 0; k0 < total; k0 += 64) {
-        int k = k0 + lane, before;
-        int o = flat_owner(incl, k, before);
-        uint32_t first = (uint32_t)__shfl((int)b.first, o, 64);
-        int rw = __shfl(b.rw, o, 64), stride = __shfl(b.stride, o, 64);
-        uint32_t oid = (uint32_t)__shfl((int)id, o, 64);
-        int kk = k - before;
-        if (k < total) {
-            uint32_t pos = atomicAdd(&cursor[first + (uint32_t)((kk / rw) * stride + kk % rw)], 1u);
-            if (pos < pair_cap) list[pos] = oid;
-            else atomicOr(&status->overflow_pairs, 1u);
-        }
-    }
+    return b.nb;
 }
 
 // Setup of input triangle t of draw D (index d), which the classification (k_front) found
@@ -1110,7 +1118,7 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
                 if (c == 1u) r = setup_input_triangle(draws[d], d, views, t, xs, id, O);
             }
         }
-        bin_rect_count(views, view, r, O.bin_count);
+        uint32_t np = (uint32_t)bin_rect_count(views, view, r, O.bin_count);
         int nclip = (c & SR_COUNT_CLIPPED) ? (int)nf : 0;
         int maxf = wave_max(nclip);
         for (int f = 0; f < maxf; f++) {
@@ -1118,177 +1126,291 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
             if (f < nclip)
                 rf = emit_clipped_fan(views, draws[d], clip_buf, (c & ~SR_COUNT_CLIPPED) >> 4, (uint32_t)f,
                                       id + (uint32_t)f, O);
-            bin_rect_count(views, view, rf, O.bin_count);
+            np += (uint32_t)bin_rect_count(views, view, rf, O.bin_count);
         }
+        uint32_t npx;
+        uint32_t chunk_np = block_excl_scan(np, npx, sh);
+        if (threadIdx.x == 0) O.chunk_pairs[base >> 8] = chunk_np;
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_scatter: (triangle, tile) pairs into the per-tile lists
+// Stable binning: every (triangle, tile) pair into its tile's list in
+// submission order
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_scatter(const DevView *views, const uint4 *geo, const DevStatus *status,
-                                                 uint32_t rec_cap, uint32_t *cursor, uint32_t *list, uint32_t pair_cap,
-                                                 DevStatus *status_w, int T) {
-    uint32_t total = status->total_records;
-    if (total > rec_cap) total = rec_cap;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
-        uint32_t id = base + threadIdx.x;
-        Rect r = {0, 0, 0, 0};
-        uint32_t view = 0;
-        if (id < total) {
-            uint4 g = geo[id];
-            view = g.w;
-            if (!(g.z & (SR_GEO_EMPTY | SR_GEO_TRANSLUCENT))) {
-                int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
-                r = bin_rect(views[view], mnx, mny, mxx, mxy, T);
-            }
-        }
-        bin_rect_scatter(views, (int)view, r, id, cursor, list, pair_cap, status_w);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// k_bin_sort: every bin's list in submission order
-// ---------------------------------------------------------------------------
-// The scatter claims list slots with atomics, so a bin's entries arrive in
-// any order.  Record ids are the canonical submission order (node distance,
-// opaque then translucent primitives, triangle, clip-fan index) and a record
-// appears at most once in a bin, so sorting each bin's segment ascending
-// restores exactly the order in which the reference's single-threaded front
-// end pushes packets into a tile queue (src/bumpqueue.rs:96-112).
+// The reference's single-threaded front end pushes packets into each tile's
+// queue in submission order (src/bumpqueue.rs:96-112).  Record ids are that
+// order (node distance, opaque then translucent primitives, triangle,
+// clip-fan index), and a record appears at most once in a bin, so a tile's
+// list is its records in ascending id.  k_bin_expand writes every record's
+// pairs (bin, record) in record order into one flat array, at offsets from
+// the per-chunk pair counts that k_setup leaves; then a
+// least-significant-digit radix sort on the global bin index, 8 bits per
+// pass, orders them by bin.  Every pass is stable, so a bin keeps ascending
+// record ids, and the last pass writes each record id straight to
+// bin_start[bin] + its rank in the bin.
 //
-// All sorts are the bitonic network in its all-ascending form: stage k
-// first compares i with i ^ (k - 1), then with i ^ j for j = k/4 .. 1, the
-// smaller id going to the lower index.  Positions past the list hold +inf
-// (no record has id 0xFFFFFFFF), so a power-of-two network sorts any length.
-#define BIN_SORT_LDS 4096
+// A pass is count, scan, scatter over tiles of 4096 elements (1024
+// contiguous ones per wave): k_bin_hist counts each tile's digits into a
+// digit-major matrix, k_bin_rowscan turns each digit's row into the tiles'
+// output offsets (from the digit's start: for the last pass bin_start of
+// the digit's first bin, for earlier ones the digit histograms that the
+// bin-count scan leaves in BinSortState), and k_bin_scatter writes each
+// tile.  Within a wave, elements with equal digits are ranked by lane with
+// ballots; the waves of a tile are ordered by their per-digit counts.  No
+// workgroup waits on another.
+#define BS_ITEMS 16
+#define BS_WAVE_ELEMS (64 * BS_ITEMS)
+#define BS_TILE (4 * BS_WAVE_ELEMS)
 
-// One wave sorts n <= 64 R ids held as v[t] = element t * 64 + lane.
-template <int R>
-__device__ __forceinline__ void wave_bitonic(uint32_t v[R]) {
-    const int lane = threadIdx.x & 63;
-    constexpr int P = 64 * R;
-#pragma unroll
-    for (int k = 2; k <= P; k <<= 1) {
-        // mirror step: partner i ^ (k - 1)
-        {
-            uint32_t nv[R];
-#pragma unroll
-            for (int t = 0; t < R; t++) {
-                int m = k - 1;
-                int tp = t ^ (m >> 6), lp = lane ^ (m & 63);
-                uint32_t p = (m & 63) ? (uint32_t)__shfl((int)v[tp], lp, 64) : v[tp];
-                bool lower = (t * 64 + lane) < (tp * 64 + lp);
-                nv[t] = lower ? min(v[t], p) : max(v[t], p);
-            }
-#pragma unroll
-            for (int t = 0; t < R; t++) v[t] = nv[t];
-        }
-#pragma unroll
-        for (int j = k >> 2; j > 0; j >>= 1) {
-            uint32_t nv[R];
-#pragma unroll
-            for (int t = 0; t < R; t++) {
-                int tp = t ^ (j >> 6), lp = lane ^ (j & 63);
-                uint32_t p = (j & 63) ? (uint32_t)__shfl_xor((int)v[tp], j & 63, 64) : v[tp];
-                bool lower = (t * 64 + lane) < (tp * 64 + lp);
-                nv[t] = lower ? min(v[t], p) : max(v[t], p);
-            }
-#pragma unroll
-            for (int t = 0; t < R; t++) v[t] = nv[t];
-        }
-    }
+struct BinPass {
+    const SR_GLOBAL DevView *views;
+    const SR_GLOBAL uint4 *geo;                     // expansion: the records
+    const SR_GLOBAL uint32_t *chunk_first;          // expansion: first record of each 256-triangle chunk
+    const SR_GLOBAL uint32_t *chunk_pairs;          // expansion: first pair of each chunk
+    uint32_t nchunks;
+    const SR_GLOBAL uint32_t *in_keys, *in_vals;    // radix passes: (bin, record) in the last pass's order
+    SR_GLOBAL uint32_t *out_keys, *out_vals;        // the expansion and radix passes before the last
+    SR_GLOBAL uint32_t *list;                       // the last pass: record ids per bin
+    SR_GLOBAL uint32_t *hist;                       // 256 rows of hist_cols tile counts, then offsets
+    uint32_t hist_cols;
+    const SR_GLOBAL uint32_t *bin_start;
+    SR_GLOBAL DevStatus *status;
+    const SR_GLOBAL BinSortState *bs;
+    uint32_t total_items, rec_cap, pair_cap;
+    int T, pass, last, shift, bits;
+};
+
+// Pair kk of a rectangle (row-major over its tiles).
+__device__ __forceinline__ uint32_t bs_pair_bin(uint32_t first, int rw, int stride, int kk) {
+    return first + (uint32_t)((kk / rw) * stride + kk % rw);
 }
 
-template <int R>
-__device__ __forceinline__ void wave_sort_list(uint32_t *list, uint32_t n) {
-    const int lane = threadIdx.x & 63;
-    uint32_t v[R];
-#pragma unroll
-    for (int t = 0; t < R; t++) {
-        uint32_t i = (uint32_t)(t * 64 + lane);
-        v[t] = i < n ? list[i] : 0xFFFFFFFFu;
-    }
-    wave_bitonic<R>(v);
-#pragma unroll
-    for (int t = 0; t < R; t++) {
-        uint32_t i = (uint32_t)(t * 64 + lane);
-        if (i < n) list[i] = v[t];
-    }
-}
+// Expansion: the pairs of chunk c's records [chunk_first[c], next chunk's
+// first) in record order.  A chunk's pairs are taken 256 at a time (a slab),
+// one per thread; BS_EXPAND_PARTS workgroups share a chunk, part q taking
+// slabs q, q + BS_EXPAND_PARTS, ..., so a chunk of large triangles (tens of
+// pairs per record) spreads over several workgroups.  Each part loads the
+// chunk's records 256 at a time and finds a pair's record by binary search
+// over their pair counts' prefix in LDS.
+#define BS_EXPAND_PARTS 8u
 
-// The whole workgroup sorts one list of n > 256 ids: in LDS up to
-// BIN_SORT_LDS, else in place in global memory (every access of a step is
-// ordered by the barrier between steps; the workgroup's waves share one CU).
-__device__ void group_sort_list(uint32_t *list, uint32_t n, uint32_t *sk) {
-    uint32_t P = 1;
-    while (P < n) P <<= 1;
-    const uint32_t nt = blockDim.x;
-    if (P <= BIN_SORT_LDS) {
-        for (uint32_t i = threadIdx.x; i < P; i += nt) sk[i] = i < n ? list[i] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= P; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                uint32_t m = j == (k >> 1) ? k - 1 : j;
-                for (uint32_t i = threadIdx.x; i < P; i += nt) {
-                    uint32_t l = i ^ m;
-                    if (l > i) {
-                        uint32_t a = sk[i], b = sk[l];
-                        if (a > b) {
-                            sk[i] = b;
-                            sk[l] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        for (uint32_t i = threadIdx.x; i < n; i += nt) list[i] = sk[i];
-        __syncthreads();
-        return;
-    }
-    for (uint32_t k = 2; k <= P; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            uint32_t m = j == (k >> 1) ? k - 1 : j;
-            for (uint32_t i = threadIdx.x; i < P; i += nt) {
-                uint32_t l = i ^ m;
-                if (l > i && l < n) {       // l >= n is +inf: no exchange
-                    uint32_t a = list[i], b = list[l];
-                    if (a > b) {
-                        list[i] = b;
-                        list[l] = a;
-                    }
+__global__ void __launch_bounds__(256) k_bin_expand(BinPass P) {
+    __shared__ uint32_t sh[17];
+    __shared__ uint32_t pex[257];
+    __shared__ uint32_t rfirst[256];
+    __shared__ int rrw[256], rstride[256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t total = min(P.status->total_records, P.rec_cap);
+    const uint32_t items = P.nchunks * BS_EXPAND_PARTS;
+    // part-major order: the first parts of all chunks come first, so the
+    // parts with work spread over the grid
+    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+        const uint32_t c = w % P.nchunks, q = w / P.nchunks;
+        const uint32_t cbase = P.chunk_pairs[c];
+        const uint32_t cend = c + 1 < P.nchunks ? P.chunk_pairs[c + 1] : P.bs->chunk_total;
+        // past the capacity the frame runs again with larger buffers (the
+        // host sees total_pairs > pair_cap)
+        if (q * 256u >= cend - cbase || cbase >= P.pair_cap) continue;
+        const uint32_t first = P.chunk_first[c];
+        const uint32_t last = min(c + 1 < P.nchunks ? P.chunk_first[c + 1] : P.status->total_records, total);
+        uint32_t sbase = 0;     // the sub-batch's first pair, counted from the chunk's
+        for (uint32_t r0 = first; r0 < last; r0 += 256u) {
+            const uint32_t id = r0 + tid;
+            Rect r = {0, 0, 0, 0};
+            int view = 0;
+            if (id < last) {
+                uint4 g = P.geo[id];
+                view = (int)g.w;
+                if (!(g.z & (SR_GEO_EMPTY | SR_GEO_TRANSLUCENT))) {
+                    int mnx = (int)(g.x & 0xFFFFu), mny = (int)(g.x >> 16), mxx = (int)(g.y & 0xFFFFu), mxy = (int)(g.y >> 16);
+                    r = bin_rect(P.views[view], mnx, mny, mxx, mxy, P.T);
                 }
             }
+            const RectBins b = rect_bins((const DevView *)P.views, view, r);
+            uint32_t ex;
+            const uint32_t sub = block_excl_scan((uint32_t)b.nb, ex, sh);
+            pex[tid] = ex;
+            rfirst[tid] = b.first;
+            rrw[tid] = b.rw;
+            rstride[tid] = b.stride;
+            if (tid == 0) pex[256] = sub;
+            __syncthreads();
+            for (uint32_t j = q; j * 256u < sbase + sub; j += BS_EXPAND_PARTS) {
+                const uint32_t p = j * 256u + tid;
+                if (p >= sbase && p < sbase + sub) {
+                    const uint32_t lp = p - sbase;
+                    // the last record whose first pair is at or before lp
+                    // (it has pairs: a record without any shares its
+                    // prefix with the next)
+                    uint32_t o = 0;
+#pragma unroll
+                    for (uint32_t step = 128; step > 0; step >>= 1)
+                        if (pex[o + step] <= lp) o += step;
+                    const int kk = (int)(lp - pex[o]);
+                    const uint32_t pos = cbase + p;
+                    if (pos < P.pair_cap) {
+                        P.out_keys[pos] = bs_pair_bin(rfirst[o], rrw[o], rstride[o], kk);
+                        P.out_vals[pos] = r0 + o;
+                    }
+                }
+            }
+            sbase += sub;
             __syncthreads();
         }
+    }
 }
 
-// Workgroups of 256 take four consecutive bins at a time, one per wave;
-// lists longer than 256 are then sorted by the whole workgroup in turn.
-__global__ void __launch_bounds__(256) k_bin_sort(const uint32_t *bin_start, const uint32_t *bin_end, uint32_t *list,
-                                                  uint32_t items, uint32_t pair_cap) {
-    __shared__ uint32_t sk[BIN_SORT_LDS];
-    __shared__ uint32_t big_s[4], big_n[4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t g = blockIdx.x; g * 4u < items; g += gridDim.x) {
-        uint32_t b = g * 4u + (uint32_t)wave;
-        uint32_t s = 0, n = 0;
-        if (b < items) {
-            s = min(bin_start[b], pair_cap);
-            n = min(bin_end[b], pair_cap) - s;
+// Valid lanes of the wave whose digit equals this lane's.
+__device__ __forceinline__ uint64_t bs_match(bool valid, uint32_t d, int bits) {
+    uint64_t m = __ballot(valid);
+    for (int b = 0; b < bits; b++) {
+        bool bit = (d >> b) & 1u;
+        uint64_t v = __ballot(bit);
+        m &= bit ? v : ~v;
+    }
+    return m;
+}
+
+__device__ __forceinline__ uint32_t bs_elements(const BinPass &P) { return min(P.status->total_pairs, P.pair_cap); }
+
+// This wave's 1024 keys of tile t (zero past the end), and their digits
+// counted per wave into cnt[wave][digit] (cleared first; LDS adds, whose
+// order does not matter here).
+__device__ __forceinline__ void bs_load_count(const BinPass &P, uint32_t t, uint32_t n, uint32_t key[BS_ITEMS],
+                                              uint32_t (*cnt)[256]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t wbase = t * BS_TILE + (uint32_t)wave * BS_WAVE_ELEMS;
+    const uint32_t mask = (1u << P.bits) - 1u;
+#pragma unroll
+    for (int s = 0; s < BS_ITEMS; s++) {
+        uint32_t i = wbase + (uint32_t)(s * 64 + lane);
+        key[s] = i < n ? P.in_keys[i] : 0u;
+    }
+    for (int k = lane; k < 256; k += 64) cnt[wave][k] = 0u;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int s = 0; s < BS_ITEMS; s++)
+        if (wbase + (uint32_t)(s * 64 + lane) < n) atomicAdd(&cnt[wave][(key[s] >> P.shift) & mask], 1u);
+}
+
+// Count: digit counts of each tile into hist[digit][tile].
+__global__ void __launch_bounds__(256) k_bin_hist(BinPass P) {
+    __shared__ uint32_t cnt[4][256];
+    const uint32_t n = bs_elements(P);
+    const uint32_t t = blockIdx.x;
+    if (t * BS_TILE >= n) return;
+    uint32_t key[BS_ITEMS];
+    bs_load_count(P, t, n, key, cnt);
+    __syncthreads();
+    const uint32_t d = threadIdx.x;
+    P.hist[(size_t)d * P.hist_cols + t] = cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
+}
+
+// Scan: workgroup d turns row d into the tiles' first output positions for
+// digit d.
+__global__ void __launch_bounds__(256) k_bin_rowscan(BinPass P) {
+    __shared__ uint32_t sh[17];
+    const uint32_t d = blockIdx.x;
+    const uint32_t ntiles = (bs_elements(P) + BS_TILE - 1) / BS_TILE;
+    uint32_t base;
+    if (P.last) {
+        uint32_t b = d << P.shift;
+        base = b < P.total_items ? P.bin_start[b] : 0u;   // digits past the last bin hold no element
+    } else {
+        uint32_t ex;
+        uint32_t v = threadIdx.x < d ? P.bs->digit_hist[P.pass][threadIdx.x] : 0u;
+        base = block_excl_scan(v, ex, sh);
+    }
+    SR_GLOBAL uint32_t *row = P.hist + (size_t)d * P.hist_cols;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += 256u) {
+        uint32_t t = t0 + threadIdx.x;
+        uint32_t v = t < ntiles ? row[t] : 0u, ex;
+        uint32_t tot = block_excl_scan(v, ex, sh);
+        if (t < ntiles) row[t] = base + ex;
+        base += tot;
+    }
+}
+
+// Scatter: each tile's elements to their positions, in input order within
+// each digit.  The tile is first reordered in LDS by digit, so the global
+// writes go out as one contiguous run per digit.  Workgroups are dispatched
+// to the 8 XCDs round-robin; XCD x takes the consecutive tiles
+// [x * per, (x + 1) * per), so the runs that consecutive tiles write into
+// one digit's range meet in that XCD's L2.
+__global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
+    __shared__ uint32_t cnt[4][256];
+    __shared__ uint32_t gofs[256];
+    __shared__ uint32_t sh[17];
+    __shared__ uint32_t skey[BS_TILE], sval[BS_TILE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t n = bs_elements(P);
+    const uint32_t ntiles = (n + BS_TILE - 1) / BS_TILE;
+    const uint32_t per = (ntiles + 7u) / 8u, k = blockIdx.x >> 3;
+    const uint32_t t = (blockIdx.x & 7u) * per + k;
+    if (k >= per || t >= ntiles) return;
+    uint32_t key[BS_ITEMS];
+    bs_load_count(P, t, n, key, cnt);
+    const uint32_t wbase = t * BS_TILE + (uint32_t)wave * BS_WAVE_ELEMS;
+    uint32_t val[BS_ITEMS];
+#pragma unroll
+    for (int s = 0; s < BS_ITEMS; s++) {
+        uint32_t i = wbase + (uint32_t)(s * 64 + lane);
+        val[s] = i < n ? P.in_vals[i] : 0u;
+    }
+    __syncthreads();
+    {
+        // digit d's run in the tile starts at local index lstart; wave w's
+        // elements of digit d follow those of the waves before it
+        const uint32_t d = threadIdx.x;
+        uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
+        uint32_t lstart;
+        block_excl_scan(c0 + c1 + c2 + c3, lstart, sh);
+        gofs[d] = P.hist[(size_t)d * P.hist_cols + t] - lstart;
+        cnt[0][d] = lstart;
+        cnt[1][d] = lstart + c0;
+        cnt[2][d] = lstart + c0 + c1;
+        cnt[3][d] = lstart + c0 + c1 + c2;
+    }
+    __syncthreads();
+    // ranks: for each row s of the wave's elements, the group of lanes with
+    // one digit takes its slots with one LDS fetch-add by its last lane;
+    // the wave's fetch-adds run in issue order, so rows keep their order
+    // and the adds need no wait between rows
+    const uint32_t mask = (1u << P.bits) - 1u;
+    uint32_t got[BS_ITEMS], rk[BS_ITEMS];
+    int ld[BS_ITEMS];
+#pragma unroll
+    for (int s = 0; s < BS_ITEMS; s++) {
+        bool valid = wbase + (uint32_t)(s * 64 + lane) < n;
+        uint32_t d = (key[s] >> P.shift) & mask;
+        uint64_t m = bs_match(valid, d, P.bits);
+        rk[s] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        ld[s] = valid ? 63 - __clzll(m) : lane;
+        got[s] = 0u;
+        if (valid && ld[s] == lane) got[s] = atomicAdd(&cnt[wave][d], (uint32_t)__popcll(m));
+    }
+#pragma unroll
+    for (int s = 0; s < BS_ITEMS; s++) {
+        uint32_t li = (uint32_t)__shfl((int)got[s], ld[s], 64) + rk[s];
+        if (wbase + (uint32_t)(s * 64 + lane) < n) {
+            skey[li] = key[s];
+            sval[li] = val[s];
         }
-        if (n > 1 && n <= 64) wave_sort_list<1>(list + s, n);
-        else if (n > 64 && n <= 128) wave_sort_list<2>(list + s, n);
-        else if (n > 128 && n <= 256) wave_sort_list<4>(list + s, n);
-        if (lane == 0) {
-            big_s[wave] = s;
-            big_n[wave] = n > 256 ? n : 0u;
+    }
+    __syncthreads();
+    const uint32_t m_tile = min(n - t * BS_TILE, (uint32_t)BS_TILE);
+    for (uint32_t i = threadIdx.x; i < m_tile; i += 256u) {
+        uint32_t kv = skey[i];
+        uint32_t pos = gofs[(kv >> P.shift) & mask] + i;
+        if (pos < P.pair_cap) {      // else the host sees total_pairs > pair_cap and runs the frame again
+            if (P.last) {
+                P.list[pos] = sval[i];
+            } else {
+                P.out_keys[pos] = kv;
+                P.out_vals[pos] = sval[i];
+            }
         }
-        __syncthreads();
-        for (int w = 0; w < 4; w++)
-            if (big_n[w]) group_sort_list(list + big_s[w], big_n[w], sk);
-        __syncthreads();
     }
 }
 
@@ -1724,7 +1846,7 @@ struct RasterArgs {
     const SR_GLOBAL uint32_t *bin_start;
     const SR_GLOBAL uint32_t *bin_end;
     const SR_GLOBAL uint32_t *list;
-    uint32_t pair_cap;
+    uint32_t pair_cap, rec_cap;
     const SR_GLOBAL uint4 *geo;
     const SR_GLOBAL float4 *raster;
     TriSource src;
@@ -2084,13 +2206,19 @@ __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long lo
     // triangles, and tiles with few or many entries, keep every lane busy.
     __shared__ int wtot[NW];
     __shared__ uint32_t zmin_s[HIZ ? NT : 1];
+    // A frame whose records or list entries outgrew their buffers is run
+    // again with larger ones; its lists are incomplete (the binning keeps
+    // only entries that fit), so it rasterizes nothing.  Every id read is
+    // also checked against the records.
+    const uint32_t nrec = P.status->total_records, npairs = P.status->total_pairs;
     uint32_t start = P.bin_start[item];
     uint32_t end = P.bin_end[item];
+    if (nrec > P.rec_cap || npairs > P.pair_cap) end = start;
     if (end > P.pair_cap) end = P.pair_cap;
     for (uint32_t c0 = start; c0 < end; c0 += (uint32_t)NT) {
         uint32_t e = c0 + (uint32_t)tid;
         int nq = 0;
-        if (e < end) {
+        if (e < end && P.list[e] < nrec) {
             uint32_t id = P.list[e];
             uint4 g = P.geo[id];
             TriSetup ts;
@@ -2940,6 +3068,19 @@ static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
     return (uint32_t)g;
 }
 
+int bin_sort_bits(uint32_t bins) {
+    int nbits = 1;
+    while (nbits < 32 && (1ull << nbits) < (unsigned long long)bins) nbits++;
+    return nbits;
+}
+
+int bin_sort_passes(uint32_t bins) {
+    int n = (bin_sort_bits(bins) + 7) / 8;
+    return n <= 3 ? n : 0;
+}
+
+uint32_t bin_sort_tiles(uint32_t pair_cap) { return std::max((pair_cap + BS_TILE - 1) / BS_TILE, 1u); }
+
 hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const uint32_t GRID_CAP = 256 * 16;
     if (a.events[0]) hipEventRecord(a.events[0], st);
@@ -2980,18 +3121,60 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     so.geo = (SR_GLOBAL uint4 *)a.rec_geo; so.raster = (SR_GLOBAL float4 *)a.rec_raster;
     so.bin_count = (SR_GLOBAL uint32_t *)a.bin_count; so.cap = a.rec_cap; so.status = (SR_GLOBAL DevStatus *)a.status;
     so.T = a.tile_size;
+    so.chunk_pairs = (SR_GLOBAL uint32_t *)a.chunk_pairs;
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
                            a.num_draws, a.total_in, a.xv + 3 * (size_t)a.total_xv, a.tri_counts, a.tri_chunks,
                            a.clip_buf, a.tri_base, so);
     }
     if (a.events[1]) hipEventRecord(a.events[1], st);
-    e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st);
+    // bin offsets (bin_end inclusive) with the radix digit histograms, the
+    // chunks' first pairs, then the stable binning
+    const int npass = bin_sort_passes(a.total_items);
+    if (npass < 1) return hipErrorInvalidValue;
+    const int nbits = bin_sort_bits(a.total_items);
+    e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st,
+                    SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(a.rec_cap, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.rec_geo, a.status, a.rec_cap,
-                       a.bin_end, a.list, a.pair_cap, a.status, a.tile_size);
-    hipLaunchKernelGGL(k_bin_sort, dim3(grid_for((a.total_items + 3) / 4, 1, GRID_CAP)), dim3(256), 0, st, a.bin_start,
-                       a.bin_end, a.list, a.total_items, a.pair_cap);
+    if (a.total_in > 0) {
+        e = launch_scan(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, a.scan_partials, &a.sort->chunk_total, st);
+        if (e != hipSuccess) return e;
+        BinPass bp = {};
+        bp.views = (const SR_GLOBAL DevView *)a.views;
+        bp.geo = (const SR_GLOBAL uint4 *)a.rec_geo;
+        bp.chunk_first = (const SR_GLOBAL uint32_t *)a.tri_chunks;
+        bp.chunk_pairs = (const SR_GLOBAL uint32_t *)a.chunk_pairs;
+        bp.nchunks = nchunks;
+        bp.list = (SR_GLOBAL uint32_t *)a.list;
+        bp.hist = (SR_GLOBAL uint32_t *)a.sort_hist;
+        bp.hist_cols = bin_sort_tiles(a.pair_cap);
+        bp.bin_start = (const SR_GLOBAL uint32_t *)a.bin_start;
+        bp.status = (SR_GLOBAL DevStatus *)a.status;
+        bp.bs = (const SR_GLOBAL BinSortState *)a.sort;
+        bp.total_items = a.total_items;
+        bp.rec_cap = a.rec_cap;
+        bp.pair_cap = a.pair_cap;
+        bp.T = a.tile_size;
+        // expansion into E (pair_keys/vals); the radix passes go E -> A ->
+        // E ... and the last one into the lists
+        uint32_t *keys[2] = {a.pair_keys, a.pair_keys2}, *vals[2] = {a.pair_vals, a.pair_vals2};
+        bp.out_keys = (SR_GLOBAL uint32_t *)keys[0];
+        bp.out_vals = (SR_GLOBAL uint32_t *)vals[0];
+        hipLaunchKernelGGL(k_bin_expand, dim3(std::min(nchunks * BS_EXPAND_PARTS, 8192u)), dim3(256), 0, st, bp);
+        for (int p = 0; p < npass; p++) {
+            bp.pass = p;
+            bp.last = p == npass - 1;
+            bp.shift = 8 * p;
+            bp.bits = std::min(8, nbits - 8 * p);
+            bp.in_keys = (const SR_GLOBAL uint32_t *)keys[p & 1];
+            bp.in_vals = (const SR_GLOBAL uint32_t *)vals[p & 1];
+            bp.out_keys = (SR_GLOBAL uint32_t *)(bp.last ? nullptr : keys[(p + 1) & 1]);
+            bp.out_vals = (SR_GLOBAL uint32_t *)(bp.last ? nullptr : vals[(p + 1) & 1]);
+            hipLaunchKernelGGL(k_bin_hist, dim3(bp.hist_cols), dim3(256), 0, st, bp);
+            hipLaunchKernelGGL(k_bin_rowscan, dim3(256), dim3(256), 0, st, bp);
+            hipLaunchKernelGGL(k_bin_scatter, dim3(bp.hist_cols + 8u), dim3(256), 0, st, bp);
+        }
+    }
     TriSource src;
     src.views = (const SR_GLOBAL DevView *)a.views;
     src.draws = (const SR_GLOBAL DevDraw *)a.draws;
@@ -3015,7 +3198,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     RasterArgs P;
     P.views = (const SR_GLOBAL DevView *)a.views; P.num_views = a.num_views; P.total_items = a.total_items;
     P.bin_start = (const SR_GLOBAL uint32_t *)a.bin_start; P.bin_end = (const SR_GLOBAL uint32_t *)a.bin_end;
-    P.list = (const SR_GLOBAL uint32_t *)a.list; P.pair_cap = a.pair_cap;
+    P.list = (const SR_GLOBAL uint32_t *)a.list; P.pair_cap = a.pair_cap; P.rec_cap = a.rec_cap;
     P.geo = (const SR_GLOBAL uint4 *)a.rec_geo; P.raster = (const SR_GLOBAL float4 *)a.rec_raster;
     P.src = src;
     P.tri_base = (const SR_GLOBAL uint32_t *)a.tri_base; P.out_ids = (SR_GLOBAL uint32_t *)a.out_ids;

@@ -4,8 +4,8 @@
 // The per-frame pipeline keeps transient buffers that the renderer grows on
 // demand: the view and draw tables with the frame's status, transformed
 // vertices, per-triangle counts and post-clip ids, setup records, bins and
-// their lists, the clip queue and clipped polygons, tile centres and
-// luminance.  This
+// their lists with the binning's pair arrays and tile counts, the clip
+// queue and clipped polygons, tile centres and luminance.  This
 // entry point carves all of them out of one file-scope device array,
 // `scratch`, so it calls no hipMalloc or hipFree; the caller passes the
 // frame's outputs.  The array's host-side name is only a stand-in for the
@@ -13,8 +13,8 @@
 // device, and only that address is handed to copies and kernels.  Every
 // call on a device shares that device's array and leaves the last call's
 // contents in it; the frame clears what must start at zero itself (the
-// status by its upload, bin counts in the first kernel), and calls that
-// overlap on one device race on it.
+// status and binning state by its upload, bin counts in the first kernel),
+// and calls that overlap on one device race on it.
 
 #include <hip/hip_runtime.h>
 
@@ -95,12 +95,15 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     // layout: fixed parts sized by the frame, then clip room, records and
     // list entries from what is left
     size_t vbytes = 256, dbytes = (std::max<size_t>(draws.size(), 1) * sizeof(DevDraw) + 255) & ~(size_t)255;
-    size_t upload = vbytes + dbytes + sizeof(DevStatus);
+    size_t upload = vbytes + dbytes + sizeof(DevStatus) + sizeof(BinSortState);
+    const int sort_passes = bin_sort_passes(items);
+    if (sort_passes < 1) return fail(SR_ERR_CAPACITY, "%u screen tiles; at most 2^24", items);
     Carve c;
     uint64_t o_views = c.take(upload);
     uint64_t o_xv = c.take((uint64_t)std::max<uint32_t>(total_xv, 1) * 64);
     uint64_t o_counts = c.take(((uint64_t)total_in + 1) * 4);
     uint64_t o_chunks = c.take(((uint64_t)total_in / 256 + 2) * 4);
+    uint64_t o_chunk_pairs = c.take(((uint64_t)total_in / 256 + 2) * 4);
     uint64_t o_base = c.take(((uint64_t)total_in + 1) * 4);
     uint64_t o_bins = c.take((uint64_t)std::max<uint32_t>(items, 1) * 12);
     uint64_t maxn = std::max<uint64_t>((uint64_t)total_in + 1, items);
@@ -111,23 +114,30 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     uint64_t o_clipq = c.take((uint64_t)clip_cap * 4);
     uint64_t o_clipb = c.take((uint64_t)clip_cap * 16 * 16 * 4);
     const uint64_t cap = SR_NOALLOC_SCRATCH_BYTES;
-    // records (80 B each) and list entries (4 B each, at least one per record)
-    uint64_t need_min = c.used + 256 * 3 + ((uint64_t)total_in + 1024) * (80 + 4);
+    // records (80 B each: geometry and raster) and list entries (4 B each,
+    // 8 B of expanded pairs, 8 B more when the radix sort takes more than
+    // one pass, 1/4 B of tile digit counts; at least one entry per record)
+    const uint64_t rec_bytes = 80, pair_bytes = 4 + 8 + (sort_passes > 1 ? 8 : 0) + 1;
+    uint64_t need_min = c.used + 256 * 8 + ((uint64_t)total_in + 1024) * (rec_bytes + pair_bytes);
     if (need_min > cap)
         return fail(SR_ERR_CAPACITY,
                     "a %d x %d frame of %u triangles needs at least %llu bytes of scratch; the no-allocation entry "
                     "point holds %llu",
                     width, height, total_in, (unsigned long long)need_min, (unsigned long long)cap);
-    uint64_t rest = cap - c.used - 256 * 3;
+    uint64_t rest = cap - c.used - 256 * 8 - 1024;
     uint64_t rec_want = (uint64_t)total_in + total_in / 8 + 1024;
-    uint32_t rec_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(rec_want, rest * 4 / 5 / 80), 0xFFFFFFF0ull);
-    uint32_t pair_cap = (uint32_t)std::min<uint64_t>((rest - (uint64_t)rec_cap * 80) / 4, 0xFFFFFFF0ull);
+    uint32_t rec_cap =
+        (uint32_t)std::min<uint64_t>(std::min<uint64_t>(rec_want, rest * 4 / 5 / rec_bytes), 0xFFFFFFF0ull);
+    uint32_t pair_cap = (uint32_t)std::min<uint64_t>((rest - (uint64_t)rec_cap * rec_bytes) / pair_bytes, 0xFFFFFFF0ull);
     uint64_t o_geo = c.take((uint64_t)rec_cap * 16);
     uint64_t o_ras = c.take((uint64_t)rec_cap * 64);
     uint64_t o_list = c.take((uint64_t)pair_cap * 4);
+    uint64_t o_pairs[4] = {0, 0, 0, 0};
+    for (int k = 0; k < (sort_passes > 1 ? 4 : 2); k++) o_pairs[k] = c.take((uint64_t)pair_cap * 4);
+    uint64_t o_hist = c.take((uint64_t)bin_sort_tiles(pair_cap) * 256 * 4);
     if (c.used > cap) return fail(SR_ERR_CAPACITY, "scratch layout exceeds %llu bytes", (unsigned long long)cap);
 
-    // view and draw tables and the zeroed status: one upload
+    // view and draw tables, the zeroed status and binning state: one upload
     std::vector<uint8_t> stage(upload, 0);
     memcpy(stage.data(), &view, sizeof(DevView));
     if (!draws.empty()) memcpy(stage.data() + vbytes, draws.data(), draws.size() * sizeof(DevDraw));
@@ -155,6 +165,14 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     a.bin_end = a.bin_start + items;
     a.scan_partials = (uint32_t *)(base + o_part);
     a.list = (uint32_t *)(base + o_list);
+    a.pair_keys = (uint32_t *)(base + o_pairs[0]);
+    a.pair_vals = (uint32_t *)(base + o_pairs[1]);
+    if (sort_passes > 1) {
+        a.pair_keys2 = (uint32_t *)(base + o_pairs[2]);
+        a.pair_vals2 = (uint32_t *)(base + o_pairs[3]);
+    }
+    a.sort_hist = (uint32_t *)(base + o_hist);
+    a.chunk_pairs = (uint32_t *)(base + o_chunk_pairs);
     a.rec_geo = (uint4 *)(base + o_geo);
     a.rec_raster = (float4 *)(base + o_ras);
     a.rec_cap = rec_cap;
@@ -163,6 +181,7 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     a.clip_buf = (float *)(base + o_clipb);
     a.clip_cap = clip_cap;
     a.status = (DevStatus *)(base + o_views + vbytes + dbytes);
+    a.sort = (BinSortState *)(base + o_views + vbytes + dbytes + sizeof(DevStatus));
     a.status_out = nullptr;
     a.out_ids = tri_id;
     a.out_depth = depth;

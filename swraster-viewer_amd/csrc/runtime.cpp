@@ -205,7 +205,8 @@ void Pipeline::destroy() {
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
     DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_chunks, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
-                      &list, &rec_geo, &rec_raster, &center, &clip_queue, &clip_buf, &tr_count,
+                      &list, &rec_geo, &rec_raster, &center, &clip_queue, &clip_buf, &chunk_pairs, &pair_keys,
+                      &pair_vals, &pair_keys2, &pair_vals2, &sort_hist, &tr_count,
                       &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color};
     for (DevBuf *b : bufs) b->release();
     stage.release();
@@ -239,7 +240,7 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t upload) {
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
                           a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
                           status_host.ptr, a.tr_count, a.tr_start, a.tr_end, a.tr_list, a.tr_scratch,
-                          a.center_color};
+                          a.center_color, a.chunk_pairs, a.pair_keys, a.pair_vals, a.pair_keys2, a.pair_vals2, a.sort_hist};
     static_assert(sizeof(ptrs) / sizeof(ptrs[0]) <= sizeof(k.p) / sizeof(k.p[0]), "graph key too small");
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) k.p[i] = (uint64_t)(uintptr_t)ptrs[i];
     if (!graph_exec || memcmp(&k, &graph_key, sizeof(k)) != 0) {
@@ -274,21 +275,21 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     // tables and clears the status before the first kernel
     size_t vbytes = (hv.size() * sizeof(DevView) + 255) & ~(size_t)255;
     size_t dbytes = (std::max<size_t>(hd.size(), 1) * sizeof(DevDraw) + 255) & ~(size_t)255;
-    size_t upload = vbytes + dbytes + sizeof(DevStatus);
+    size_t upload = vbytes + dbytes + sizeof(DevStatus) + sizeof(BinSortState);
+    const int sort_passes = bin_sort_passes(total_items);
+    if (sort_passes < 1) return fail(SR_ERR_CAPACITY, "%u screen tiles in one frame; at most 2^24", total_items);
     int r;
     if ((r = stage.ensure(upload))) return r;
     memcpy(stage.ptr, hv.data(), hv.size() * sizeof(DevView));
     if (!hd.empty()) memcpy((char *)stage.ptr + vbytes, hd.data(), hd.size() * sizeof(DevDraw));
-    DevStatus init;
-    memset(&init, 0, sizeof(init));
-    init.seq = ++seq;
-    memcpy((char *)stage.ptr + vbytes + dbytes, &init, sizeof(init));
+    memset((char *)stage.ptr + vbytes + dbytes + sizeof(DevStatus), 0, sizeof(BinSortState));
     if ((r = views.ensure(upload))) return r;
     if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 64))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
     if ((r = tri_counts.ensure(((size_t)total_in + 1) * 4))) return r;
     if ((r = tri_chunks.ensure(((size_t)total_in / 256 + 2) * 4))) return r;
     if ((r = tri_base.ensure(((size_t)total_in + 1) * 4))) return r;
+    if ((r = chunk_pairs.ensure(((size_t)total_in / 256 + 2) * 4))) return r;
     if ((r = bin_count.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_start.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_end.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
@@ -311,11 +312,21 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     if (pair_cap < rec_cap * 2) pair_cap = rec_cap * 2;
     if (clip_cap < 4096) clip_cap = 4096;
     for (int attempt = 0; attempt < 4; attempt++) {
+        // every attempt is a frame of its own number
+        DevStatus init;
+        memset(&init, 0, sizeof(init));
+        init.seq = ++seq;
+        memcpy((char *)stage.ptr + vbytes + dbytes, &init, sizeof(init));
         if ((r = rec_geo.ensure((size_t)rec_cap * 16))) return r;
         if ((r = rec_raster.ensure((size_t)rec_cap * 64))) return r;
         if ((r = list.ensure((size_t)pair_cap * 4))) return r;
         if ((r = clip_queue.ensure((size_t)clip_cap * 4))) return r;
         if ((r = clip_buf.ensure((size_t)clip_cap * 16 * 16 * 4))) return r;
+        if ((r = pair_keys.ensure((size_t)pair_cap * 4)) || (r = pair_vals.ensure((size_t)pair_cap * 4))) return r;
+        if (sort_passes > 1 &&
+            ((r = pair_keys2.ensure((size_t)pair_cap * 4)) || (r = pair_vals2.ensure((size_t)pair_cap * 4))))
+            return r;
+        if ((r = sort_hist.ensure((size_t)bin_sort_tiles(pair_cap) * 256 * 4))) return r;
         if (has_tr && (r = tr_list.ensure((size_t)tr_cap * 4))) return r;
         if (has_tr && (r = tr_scratch.ensure((size_t)tr_cap * 8))) return r;
         FrameArgs a;
@@ -347,6 +358,13 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.clip_buf = (float *)clip_buf.ptr;
         a.clip_cap = clip_cap;
         a.status = (DevStatus *)((char *)views.ptr + vbytes + dbytes);
+        a.sort = (BinSortState *)((char *)a.status + sizeof(DevStatus));
+        a.pair_keys = (uint32_t *)pair_keys.ptr;
+        a.pair_vals = (uint32_t *)pair_vals.ptr;
+        a.pair_keys2 = (uint32_t *)pair_keys2.ptr;
+        a.pair_vals2 = (uint32_t *)pair_vals2.ptr;
+        a.chunk_pairs = (uint32_t *)chunk_pairs.ptr;
+        a.sort_hist = (uint32_t *)sort_hist.ptr;
         a.status_out = (DevStatus *)status_host.ptr;
         status_dev = a.status;
         a.out_ids = out_ids;

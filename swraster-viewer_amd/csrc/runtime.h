@@ -61,6 +61,7 @@ struct Pipeline {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     DevBuf views, xv, xt, tri_counts, tri_chunks, tri_base, bin_count, bin_start, bin_end, partials, list;
     DevBuf rec_geo, rec_raster, center, clip_queue, clip_buf;
+    DevBuf chunk_pairs, pair_keys, pair_vals, pair_keys2, pair_vals2, sort_hist;
     DevBuf tr_count, tr_start, tr_end, tr_list, tr_scratch, center_color;
     PinBuf stage, status_host;
     uint32_t rec_cap = 0, pair_cap = 0, clip_cap = 0, tr_cap = 0;
@@ -68,7 +69,7 @@ struct Pipeline {
     bool last_hiz = false;            // the last frame's raster used depth rejection
     struct GraphKey {
         uint64_t n[16];
-        uint64_t p[32];
+        uint64_t p[40];
     } graph_key = {};
     hipGraphExec_t graph_exec = nullptr;
     int init(int device, int tile);
