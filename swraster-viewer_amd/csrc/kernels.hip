@@ -32,9 +32,10 @@
 // fragment, in submission order, among those with the minimal depth; NaN
 // depths never pass and +0 == -0.  The kernel computes exactly that as an
 // order-independent minimum over 64-bit keys (orderable depth << 32 |
-// ~triangle id), so the per-tile lists may be in any order and the result
-// is still deterministic.  The winner's barycentrics and depth bits are then
-// recomputed with the same evaluation chain.
+// ~triangle id), so the result does not depend on the order of a tile's
+// list (the lists are in submission order anyway).  The winner's
+// barycentrics and depth bits are then recomputed with the same evaluation
+// chain.
 
 #include <hip/hip_runtime.h>
 
@@ -649,15 +650,30 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, u
 }
 
 // One workgroup scans arrays of up to SCAN_SINGLE_MAX elements in a single
-// launch; larger arrays use the
-// partials / top / apply sequence above.
-#define SCAN_SINGLE_MAX 16384u
+// launch (workgroup b takes job b, so independent small scans share one
+// launch); larger arrays use the partials / top / apply sequence above.
+#define SCAN_SINGLE_MAX 65536u
 
-__global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2,
-                                                      uint32_t *total_out, uint32_t flags, uint32_t *dh,
-                                                      int dh_passes) {
+struct ScanJob {
+    const uint32_t *in;
+    uint32_t n;
+    uint32_t *out, *out2, *total;
+    uint32_t flags;
+    uint32_t *dh;
+    int dh_passes;
+};
+
+struct ScanJobs {
+    ScanJob job[2];
+};
+
+__global__ void __launch_bounds__(1024) k_scan_single(ScanJobs J) {
     __shared__ uint32_t sh[16];
     __shared__ uint32_t lh[512];
+    const ScanJob &q = J.job[blockIdx.x];
+    const uint32_t *in = q.in;
+    const uint32_t n = q.n;
+    uint32_t *dh = q.dh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (dh) {
         for (int j = threadIdx.x; j < 512; j += 1024) lh[j] = 0u;
@@ -675,22 +691,38 @@ __global__ void __launch_bounds__(1024) k_scan_single(const uint32_t *in, uint32
             if (w < wave) base += sh[w];
             all += sh[w];
         }
-        wave_store_segment(out, out2, seg, n, base, ex, v, flags);
-        if (dh) digit_hist_segment(lh, dh_passes, seg, v);
+        wave_store_segment(q.out, q.out2, seg, n, base, ex, v, q.flags);
+        if (dh) digit_hist_segment(lh, q.dh_passes, seg, v);
         carry += all;
         __syncthreads();
     }
-    if (threadIdx.x == 0 && total_out) *total_out = carry;
+    if (threadIdx.x == 0 && q.total) *q.total = carry;
     if (dh)
-        for (int j = threadIdx.x; j < 256 * dh_passes; j += 1024) dh[j] = lh[j];
+        for (int j = threadIdx.x; j < 256 * q.dh_passes; j += 1024) dh[j] = lh[j];
+}
+
+static ScanJob scan_job(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *total, uint32_t flags,
+                        uint32_t *digit_hist, int digit_passes) {
+    ScanJob j;
+    j.in = in;
+    j.n = n;
+    j.out = out;
+    j.out2 = out2;
+    j.total = total;
+    j.flags = flags;
+    j.dh = digit_passes > 0 ? digit_hist : nullptr;
+    j.dh_passes = digit_passes > 0 ? digit_passes : 0;
+    return j;
 }
 
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
                        uint32_t *total, hipStream_t st, uint32_t flags, uint32_t *digit_hist, int digit_passes) {
     if (digit_passes <= 0) digit_hist = nullptr;
     if (n <= SCAN_SINGLE_MAX) {
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, in, n, out, out2, total, flags, digit_hist,
-                           digit_passes);
+        ScanJobs J;
+        J.job[0] = scan_job(in, n, out, out2, total, flags, digit_hist, digit_passes);
+        J.job[1] = J.job[0];
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, J);
         return hipGetLastError();
     }
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -1150,8 +1182,8 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
 // record ids, and the last pass writes each record id straight to
 // bin_start[bin] + its rank in the bin.
 //
-// A pass is count, scan, scatter over tiles of 4096 elements (1024
-// contiguous ones per wave): k_bin_hist counts each tile's digits into a
+// A pass is count, scan, scatter over tiles of 256 x IT elements (a
+// contiguous quarter per wave): k_bin_hist counts each tile's digits into a
 // digit-major matrix, k_bin_rowscan turns each digit's row into the tiles'
 // output offsets (from the digit's start: for the last pass bin_start of
 // the digit's first bin, for earlier ones the digit histograms that the
@@ -1159,9 +1191,10 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
 // tile.  Within a wave, elements with equal digits are ranked by lane with
 // ballots; the waves of a tile are ordered by their per-digit counts.  No
 // workgroup waits on another.
-#define BS_ITEMS 16
-#define BS_WAVE_ELEMS (64 * BS_ITEMS)
-#define BS_TILE (4 * BS_WAVE_ELEMS)
+// Tiles hold 256 x IT elements (IT per thread): 4 up to BS_SMALL_PAIRS
+// pairs of capacity, so a frame's few hundred thousand pairs still spread
+// over several hundred workgroups, else 16.
+#define BS_SMALL_PAIRS (4u << 20)
 
 struct BinPass {
     const SR_GLOBAL DevView *views;
@@ -1178,6 +1211,7 @@ struct BinPass {
     SR_GLOBAL DevStatus *status;
     const SR_GLOBAL BinSortState *bs;
     uint32_t total_items, rec_cap, pair_cap;
+    uint32_t tile_elems;                            // elements per radix tile (256 x IT)
     int T, pass, last, shift, bits;
 };
 
@@ -1188,11 +1222,11 @@ __device__ __forceinline__ uint32_t bs_pair_bin(uint32_t first, int rw, int stri
 
 // Expansion: the pairs of chunk c's records [chunk_first[c], next chunk's
 // first) in record order.  A chunk's pairs are taken 256 at a time (a slab),
-// one per thread; BS_EXPAND_PARTS workgroups share a chunk, part q taking
-// slabs q, q + BS_EXPAND_PARTS, ..., so a chunk of large triangles (tens of
-// pairs per record) spreads over several workgroups.  Each part loads the
-// chunk's records 256 at a time and finds a pair's record by binary search
-// over their pair counts' prefix in LDS.
+// one per thread; up to BS_EXPAND_PARTS workgroups share a chunk, each
+// taking a contiguous range of its slabs, so a chunk of large triangles
+// (tens of pairs per record) spreads over several workgroups.  Each part
+// loads the chunk's records 256 at a time and finds a pair's record by
+// binary search over their pair counts' prefix in LDS.
 #define BS_EXPAND_PARTS 8u
 
 __global__ void __launch_bounds__(256) k_bin_expand(BinPass P) {
@@ -1209,13 +1243,16 @@ __global__ void __launch_bounds__(256) k_bin_expand(BinPass P) {
         const uint32_t c = w % P.nchunks, q = w / P.nchunks;
         const uint32_t cbase = P.chunk_pairs[c];
         const uint32_t cend = c + 1 < P.nchunks ? P.chunk_pairs[c + 1] : P.bs->chunk_total;
+        const uint32_t nslabs = (cend - cbase + 255u) / 256u;
+        const uint32_t per = (nslabs + BS_EXPAND_PARTS - 1) / BS_EXPAND_PARTS;
+        const uint32_t jlo = q * per, jhi = min(jlo + per, nslabs);
         // past the capacity the frame runs again with larger buffers (the
         // host sees total_pairs > pair_cap)
-        if (q * 256u >= cend - cbase || cbase >= P.pair_cap) continue;
+        if (jlo >= jhi || cbase + jlo * 256u >= P.pair_cap) continue;
         const uint32_t first = P.chunk_first[c];
         const uint32_t last = min(c + 1 < P.nchunks ? P.chunk_first[c + 1] : P.status->total_records, total);
         uint32_t sbase = 0;     // the sub-batch's first pair, counted from the chunk's
-        for (uint32_t r0 = first; r0 < last; r0 += 256u) {
+        for (uint32_t r0 = first; r0 < last && sbase < jhi * 256u; r0 += 256u) {
             const uint32_t id = r0 + tid;
             Rect r = {0, 0, 0, 0};
             int view = 0;
@@ -1236,7 +1273,8 @@ __global__ void __launch_bounds__(256) k_bin_expand(BinPass P) {
             rstride[tid] = b.stride;
             if (tid == 0) pex[256] = sub;
             __syncthreads();
-            for (uint32_t j = q; j * 256u < sbase + sub; j += BS_EXPAND_PARTS) {
+            const uint32_t ja = max(jlo, sbase / 256u);
+            for (uint32_t j = ja; j < jhi && j * 256u < sbase + sub; j++) {
                 const uint32_t p = j * 256u + tid;
                 if (p >= sbase && p < sbase + sub) {
                     const uint32_t lp = p - sbase;
@@ -1277,31 +1315,33 @@ __device__ __forceinline__ uint32_t bs_elements(const BinPass &P) { return min(P
 // This wave's 1024 keys of tile t (zero past the end), and their digits
 // counted per wave into cnt[wave][digit] (cleared first; LDS adds, whose
 // order does not matter here).
-__device__ __forceinline__ void bs_load_count(const BinPass &P, uint32_t t, uint32_t n, uint32_t key[BS_ITEMS],
+template <int IT>
+__device__ __forceinline__ void bs_load_count(const BinPass &P, uint32_t t, uint32_t n, uint32_t key[IT],
                                               uint32_t (*cnt)[256]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t wbase = t * BS_TILE + (uint32_t)wave * BS_WAVE_ELEMS;
+    const uint32_t wbase = t * (256u * IT) + (uint32_t)wave * (64u * IT);
     const uint32_t mask = (1u << P.bits) - 1u;
 #pragma unroll
-    for (int s = 0; s < BS_ITEMS; s++) {
+    for (int s = 0; s < IT; s++) {
         uint32_t i = wbase + (uint32_t)(s * 64 + lane);
         key[s] = i < n ? P.in_keys[i] : 0u;
     }
     for (int k = lane; k < 256; k += 64) cnt[wave][k] = 0u;
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int s = 0; s < BS_ITEMS; s++)
+    for (int s = 0; s < IT; s++)
         if (wbase + (uint32_t)(s * 64 + lane) < n) atomicAdd(&cnt[wave][(key[s] >> P.shift) & mask], 1u);
 }
 
 // Count: digit counts of each tile into hist[digit][tile].
+template <int IT>
 __global__ void __launch_bounds__(256) k_bin_hist(BinPass P) {
     __shared__ uint32_t cnt[4][256];
     const uint32_t n = bs_elements(P);
     const uint32_t t = blockIdx.x;
-    if (t * BS_TILE >= n) return;
-    uint32_t key[BS_ITEMS];
-    bs_load_count(P, t, n, key, cnt);
+    if (t * (256u * IT) >= n) return;
+    uint32_t key[IT];
+    bs_load_count<IT>(P, t, n, key, cnt);
     __syncthreads();
     const uint32_t d = threadIdx.x;
     P.hist[(size_t)d * P.hist_cols + t] = cnt[0][d] + cnt[1][d] + cnt[2][d] + cnt[3][d];
@@ -1312,7 +1352,7 @@ __global__ void __launch_bounds__(256) k_bin_hist(BinPass P) {
 __global__ void __launch_bounds__(256) k_bin_rowscan(BinPass P) {
     __shared__ uint32_t sh[17];
     const uint32_t d = blockIdx.x;
-    const uint32_t ntiles = (bs_elements(P) + BS_TILE - 1) / BS_TILE;
+    const uint32_t ntiles = (bs_elements(P) + P.tile_elems - 1) / P.tile_elems;
     uint32_t base;
     if (P.last) {
         uint32_t b = d << P.shift;
@@ -1338,23 +1378,25 @@ __global__ void __launch_bounds__(256) k_bin_rowscan(BinPass P) {
 // to the 8 XCDs round-robin; XCD x takes the consecutive tiles
 // [x * per, (x + 1) * per), so the runs that consecutive tiles write into
 // one digit's range meet in that XCD's L2.
+template <int IT>
 __global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
+    constexpr uint32_t TILE = 256u * IT;
     __shared__ uint32_t cnt[4][256];
     __shared__ uint32_t gofs[256];
     __shared__ uint32_t sh[17];
-    __shared__ uint32_t skey[BS_TILE], sval[BS_TILE];
+    __shared__ uint32_t skey[TILE], sval[TILE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t n = bs_elements(P);
-    const uint32_t ntiles = (n + BS_TILE - 1) / BS_TILE;
+    const uint32_t ntiles = (n + TILE - 1) / TILE;
     const uint32_t per = (ntiles + 7u) / 8u, k = blockIdx.x >> 3;
     const uint32_t t = (blockIdx.x & 7u) * per + k;
     if (k >= per || t >= ntiles) return;
-    uint32_t key[BS_ITEMS];
-    bs_load_count(P, t, n, key, cnt);
-    const uint32_t wbase = t * BS_TILE + (uint32_t)wave * BS_WAVE_ELEMS;
-    uint32_t val[BS_ITEMS];
+    uint32_t key[IT];
+    bs_load_count<IT>(P, t, n, key, cnt);
+    const uint32_t wbase = t * TILE + (uint32_t)wave * (64u * IT);
+    uint32_t val[IT];
 #pragma unroll
-    for (int s = 0; s < BS_ITEMS; s++) {
+    for (int s = 0; s < IT; s++) {
         uint32_t i = wbase + (uint32_t)(s * 64 + lane);
         val[s] = i < n ? P.in_vals[i] : 0u;
     }
@@ -1378,10 +1420,10 @@ __global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
     // the wave's fetch-adds run in issue order, so rows keep their order
     // and the adds need no wait between rows
     const uint32_t mask = (1u << P.bits) - 1u;
-    uint32_t got[BS_ITEMS], rk[BS_ITEMS];
-    int ld[BS_ITEMS];
+    uint32_t got[IT], rk[IT];
+    int ld[IT];
 #pragma unroll
-    for (int s = 0; s < BS_ITEMS; s++) {
+    for (int s = 0; s < IT; s++) {
         bool valid = wbase + (uint32_t)(s * 64 + lane) < n;
         uint32_t d = (key[s] >> P.shift) & mask;
         uint64_t m = bs_match(valid, d, P.bits);
@@ -1391,7 +1433,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
         if (valid && ld[s] == lane) got[s] = atomicAdd(&cnt[wave][d], (uint32_t)__popcll(m));
     }
 #pragma unroll
-    for (int s = 0; s < BS_ITEMS; s++) {
+    for (int s = 0; s < IT; s++) {
         uint32_t li = (uint32_t)__shfl((int)got[s], ld[s], 64) + rk[s];
         if (wbase + (uint32_t)(s * 64 + lane) < n) {
             skey[li] = key[s];
@@ -1399,7 +1441,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
         }
     }
     __syncthreads();
-    const uint32_t m_tile = min(n - t * BS_TILE, (uint32_t)BS_TILE);
+    const uint32_t m_tile = min(n - t * TILE, TILE);
     for (uint32_t i = threadIdx.x; i < m_tile; i += 256u) {
         uint32_t kv = skey[i];
         uint32_t pos = gofs[(kv >> P.shift) & mask] + i;
@@ -3079,7 +3121,12 @@ int bin_sort_passes(uint32_t bins) {
     return n <= 3 ? n : 0;
 }
 
-uint32_t bin_sort_tiles(uint32_t pair_cap) { return std::max((pair_cap + BS_TILE - 1) / BS_TILE, 1u); }
+static uint32_t bin_sort_items(uint32_t pair_cap) { return pair_cap <= BS_SMALL_PAIRS ? 4u : 16u; }
+
+uint32_t bin_sort_tiles(uint32_t pair_cap) {
+    uint32_t te = 256u * bin_sort_items(pair_cap);
+    return std::max((uint32_t)(((uint64_t)pair_cap + te - 1) / te), 1u);
+}
 
 hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const uint32_t GRID_CAP = 256 * 16;
@@ -3133,12 +3180,22 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const int npass = bin_sort_passes(a.total_items);
     if (npass < 1) return hipErrorInvalidValue;
     const int nbits = bin_sort_bits(a.total_items);
-    e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs, st,
-                    SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
+    // (one launch when both arrays fit one workgroup's scan)
+    if (a.total_in > 0 && a.total_items <= SCAN_SINGLE_MAX && nchunks <= SCAN_SINGLE_MAX) {
+        ScanJobs J;
+        J.job[0] = scan_job(a.bin_count, a.total_items, a.bin_start, a.bin_end, &a.status->total_pairs,
+                            SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
+        J.job[1] = scan_job(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, &a.sort->chunk_total, 0, nullptr, 0);
+        hipLaunchKernelGGL(k_scan_single, dim3(2), dim3(1024), 0, st, J);
+        e = hipGetLastError();
+    } else {
+        e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs,
+                        st, SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
+        if (e == hipSuccess && a.total_in > 0)
+            e = launch_scan(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, a.scan_partials, &a.sort->chunk_total, st);
+    }
     if (e != hipSuccess) return e;
     if (a.total_in > 0) {
-        e = launch_scan(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, a.scan_partials, &a.sort->chunk_total, st);
-        if (e != hipSuccess) return e;
         BinPass bp = {};
         bp.views = (const SR_GLOBAL DevView *)a.views;
         bp.geo = (const SR_GLOBAL uint4 *)a.rec_geo;
@@ -3148,6 +3205,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         bp.list = (SR_GLOBAL uint32_t *)a.list;
         bp.hist = (SR_GLOBAL uint32_t *)a.sort_hist;
         bp.hist_cols = bin_sort_tiles(a.pair_cap);
+        bp.tile_elems = 256u * bin_sort_items(a.pair_cap);
         bp.bin_start = (const SR_GLOBAL uint32_t *)a.bin_start;
         bp.status = (SR_GLOBAL DevStatus *)a.status;
         bp.bs = (const SR_GLOBAL BinSortState *)a.sort;
@@ -3170,9 +3228,12 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
             bp.in_vals = (const SR_GLOBAL uint32_t *)vals[p & 1];
             bp.out_keys = (SR_GLOBAL uint32_t *)(bp.last ? nullptr : keys[(p + 1) & 1]);
             bp.out_vals = (SR_GLOBAL uint32_t *)(bp.last ? nullptr : vals[(p + 1) & 1]);
-            hipLaunchKernelGGL(k_bin_hist, dim3(bp.hist_cols), dim3(256), 0, st, bp);
+            if (bp.tile_elems == 1024u) hipLaunchKernelGGL(k_bin_hist<4>, dim3(bp.hist_cols), dim3(256), 0, st, bp);
+            else hipLaunchKernelGGL(k_bin_hist<16>, dim3(bp.hist_cols), dim3(256), 0, st, bp);
             hipLaunchKernelGGL(k_bin_rowscan, dim3(256), dim3(256), 0, st, bp);
-            hipLaunchKernelGGL(k_bin_scatter, dim3(bp.hist_cols + 8u), dim3(256), 0, st, bp);
+            if (bp.tile_elems == 1024u)
+                hipLaunchKernelGGL(k_bin_scatter<4>, dim3(bp.hist_cols + 8u), dim3(256), 0, st, bp);
+            else hipLaunchKernelGGL(k_bin_scatter<16>, dim3(bp.hist_cols + 8u), dim3(256), 0, st, bp);
         }
     }
     TriSource src;
