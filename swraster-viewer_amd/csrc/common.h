@@ -55,7 +55,7 @@ struct DevTexture {
     float whv[4];                 // width, width, height, height
     uint32_t max_mip;
     uint32_t num_mips;
-    uint32_t mip_off[SR_MAX_MIPS];
+    uint32_t mip_off[SR_MAX_MIPS];    // each level in 4x4-texel blocks, row-major over the padded level
     uint32_t mip_w[SR_MAX_MIPS];
     uint32_t mip_h[SR_MAX_MIPS];
 };

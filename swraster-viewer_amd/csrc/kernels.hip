@@ -1201,7 +1201,7 @@ struct BinPass {
     const SR_GLOBAL uint4 *geo;                     // expansion: the records
     const SR_GLOBAL uint32_t *chunk_first;          // expansion: first record of each 256-triangle chunk
     const SR_GLOBAL uint32_t *chunk_pairs;          // expansion: first pair of each chunk
-    uint32_t nchunks;
+    uint32_t nchunks, parts;
     const SR_GLOBAL uint32_t *in_keys, *in_vals;    // radix passes: (bin, record) in the last pass's order
     SR_GLOBAL uint32_t *out_keys, *out_vals;        // the expansion and radix passes before the last
     SR_GLOBAL uint32_t *list;                       // the last pass: record ids per bin
@@ -1222,9 +1222,11 @@ __device__ __forceinline__ uint32_t bs_pair_bin(uint32_t first, int rw, int stri
 
 // Expansion: the pairs of chunk c's records [chunk_first[c], next chunk's
 // first) in record order.  A chunk's pairs are taken 256 at a time (a slab),
-// one per thread; up to BS_EXPAND_PARTS workgroups share a chunk, each
-// taking a contiguous range of its slabs, so a chunk of large triangles
-// (tens of pairs per record) spreads over several workgroups.  Each part
+// one per thread; `parts` workgroups share a chunk, each taking a
+// contiguous range of its slabs, so a chunk of large triangles (tens of
+// pairs per record) spreads over several workgroups.  The host sets parts
+// from the ratio of the pair and record capacities (1 for small triangles,
+// up to BS_EXPAND_PARTS), so frames of small triangles launch no idle parts.  Each part
 // loads the chunk's records 256 at a time and finds a pair's record by
 // binary search over their pair counts' prefix in LDS.
 #define BS_EXPAND_PARTS 8u
@@ -1236,7 +1238,8 @@ __global__ void __launch_bounds__(256) k_bin_expand(BinPass P) {
     __shared__ int rrw[256], rstride[256];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = min(P.status->total_records, P.rec_cap);
-    const uint32_t items = P.nchunks * BS_EXPAND_PARTS;
+    const uint32_t parts = P.parts;
+    const uint32_t items = P.nchunks * parts;
     // part-major order: the first parts of all chunks come first, so the
     // parts with work spread over the grid
     for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
@@ -1244,7 +1247,7 @@ __global__ void __launch_bounds__(256) k_bin_expand(BinPass P) {
         const uint32_t cbase = P.chunk_pairs[c];
         const uint32_t cend = c + 1 < P.nchunks ? P.chunk_pairs[c + 1] : P.bs->chunk_total;
         const uint32_t nslabs = (cend - cbase + 255u) / 256u;
-        const uint32_t per = (nslabs + BS_EXPAND_PARTS - 1) / BS_EXPAND_PARTS;
+        const uint32_t per = (nslabs + parts - 1) / parts;
         const uint32_t jlo = q * per, jhi = min(jlo + per, nslabs);
         // past the capacity the frame runs again with larger buffers (the
         // host sees total_pairs > pair_cap)
@@ -1603,7 +1606,10 @@ __device__ __forceinline__ uint32_t texel_nearest(const DevScene &S, const DevTe
     float wf = (float)T.mip_w[mip], hf = (float)T.mip_h[mip];
     uint32_t x = sat_u32(apply_wrap(floorf(u * wf), wf, ref.wrap_s));
     uint32_t y = sat_u32(apply_wrap(floorf(v * hf), hf, ref.wrap_t));
-    uint32_t idx = T.mip_off[mip] + y * T.mip_w[mip] + x;
+    // levels are stored in 4x4-texel blocks, row-major over the level
+    // padded to multiples of 4
+    uint32_t bw = (T.mip_w[mip] + 3u) >> 2;
+    uint32_t idx = T.mip_off[mip] + ((((y >> 2) * bw) + (x >> 2)) << 4) + ((y & 3u) << 2) + (x & 3u);
     return S.texels[T.base + idx];
 }
 
@@ -3121,6 +3127,14 @@ int bin_sort_passes(uint32_t bins) {
     return n <= 3 ? n : 0;
 }
 
+// Workgroups per chunk in the expansion: about one per 1024 pairs a chunk
+// of 256 triangles holds, judged from the capacities (the pair capacity
+// follows the frames' pair counts).
+static uint32_t bin_expand_parts(uint32_t rec_cap, uint32_t pair_cap) {
+    uint64_t per_chunk = (uint64_t)pair_cap * 256u / std::max(rec_cap, 1u);
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(per_chunk / 1024u, 1u), BS_EXPAND_PARTS);
+}
+
 static uint32_t bin_sort_items(uint32_t pair_cap) { return pair_cap <= BS_SMALL_PAIRS ? 4u : 16u; }
 
 uint32_t bin_sort_tiles(uint32_t pair_cap) {
@@ -3218,7 +3232,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         uint32_t *keys[2] = {a.pair_keys, a.pair_keys2}, *vals[2] = {a.pair_vals, a.pair_vals2};
         bp.out_keys = (SR_GLOBAL uint32_t *)keys[0];
         bp.out_vals = (SR_GLOBAL uint32_t *)vals[0];
-        hipLaunchKernelGGL(k_bin_expand, dim3(std::min(nchunks * BS_EXPAND_PARTS, 8192u)), dim3(256), 0, st, bp);
+        bp.parts = bin_expand_parts(a.rec_cap, a.pair_cap);
+        hipLaunchKernelGGL(k_bin_expand, dim3(std::min(nchunks * bp.parts, 65536u)), dim3(256), 0, st, bp);
         for (int p = 0; p < npass; p++) {
             bp.pass = p;
             bp.last = p == npass - 1;

@@ -234,7 +234,8 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
                      build_mip_chain(t.width, t.height, t.type, t.texels, c.data, c.off, c.w, c.h);
                  });
     size_t pool_size = 0;
-    for (const Chain &c : chains) pool_size += c.data.size();
+    for (const Chain &c : chains)
+        for (size_t k = 0; k < c.w.size(); k++) pool_size += (size_t)((c.w[k] + 3) / 4) * ((c.h[k] + 3) / 4) * 16;
     pool.reserve(pool_size);
     for (int i = 0; i < d->num_textures; i++) {
         const sr_texture_desc &t = d->textures[i];
@@ -249,12 +250,24 @@ SR_API int sr_scene_create(const sr_scene_desc *d, int32_t device, sr_scene **ou
         o.whv[2] = (float)t.height; o.whv[3] = (float)t.height;
         o.num_mips = (uint32_t)off.size();
         o.max_mip = o.num_mips - 1;
+        // each level stored in 4x4-texel blocks (64 B), blocks row-major
+        // over the level padded to multiples of 4: a pixel's nearest texel
+        // shares its cache line with the texels of the pixels around it
+        uint32_t at = 0;
         for (size_t k = 0; k < off.size(); k++) {
-            o.mip_off[k] = off[k];
-            o.mip_w[k] = w[k];
-            o.mip_h[k] = h[k];
+            const uint32_t lw = w[k], lh = h[k], bw = (lw + 3) / 4, bh = (lh + 3) / 4;
+            o.mip_off[k] = at;
+            o.mip_w[k] = lw;
+            o.mip_h[k] = lh;
+            size_t first = pool.size();
+            pool.resize(first + (size_t)bw * bh * 16, 0u);
+            const uint32_t *src = data.data() + off[k];
+            for (uint32_t y = 0; y < lh; y++)
+                for (uint32_t x = 0; x < lw; x++)
+                    pool[first + (((size_t)(y >> 2) * bw + (x >> 2)) << 4) + ((y & 3) << 2) + (x & 3)] =
+                        src[(size_t)y * lw + x];
+            at += bw * bh * 16;
         }
-        pool.insert(pool.end(), data.begin(), data.end());
         std::vector<uint32_t>().swap(data);
     }
 
