@@ -1766,9 +1766,17 @@ __device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, i
     float ndh = maxps(dot3(nw, hv), 0.0f);
     float ndv = maxps(dot3(nw, vn), 1.0e-4f);
     float vdh = maxps(dot3(vn, hv), 0.0f);
-    f3 gi[4];
+    f3 irr;
     float gw0, gw1;
-    voxel_lookup(S, pos, gi, gw0, gw1);
+    {
+        // the irradiance right away: four SH coefficients need not stay live
+        f3 gi[4];
+        voxel_lookup(S, pos, gi, gw0, gw1);
+        irr.x = ((gi[0].x + gi[1].x * nw.y) + gi[2].x * nw.z) + gi[3].x * nw.x;
+        irr.y = ((gi[0].y + gi[1].y * nw.y) + gi[2].y * nw.z) + gi[3].y * nw.x;
+        irr.z = ((gi[0].z + gi[1].z * nw.y) + gi[2].z * nw.z) + gi[3].z * nw.x;
+        irr = max3s(irr, 0.0f);
+    }
     float vli = clampv(gw0, 0.0f, 1.0f);
     float skyv = clampv(gw1, 0.0f, 1.0f);
     f3 base = mk3(M.base[0], M.base[1], M.base[2]);
@@ -1812,11 +1820,6 @@ __device__ __forceinline__ f3 shade_pixel(const DevScene &S, const DevView &V, i
     f3 cdd = muls(muls(mul3(mul3(LC, kdd), lam), ndl), vli);
     f3 cds = muls(muls(muls(mul3(LC, bf), sdg), ndl), vli);
     f3 kdi = mk3((1.0f - f0.x) * om, (1.0f - f0.y) * om, (1.0f - f0.z) * om);
-    f3 irr;
-    irr.x = ((gi[0].x + gi[1].x * nw.y) + gi[2].x * nw.z) + gi[3].x * nw.x;
-    irr.y = ((gi[0].y + gi[1].y * nw.y) + gi[2].y * nw.z) + gi[3].y * nw.x;
-    irr.z = ((gi[0].z + gi[1].z * nw.y) + gi[2].z * nw.z) + gi[3].z * nw.x;
-    irr = max3s(irr, 0.0f);
     f3 cid = muls(mul3(mul3(irr, base), kdi), INV_PI);
     f3 iv = muls(vn, -1.0f);
     f3 refl = sub3(iv, muls(muls(nw, dot3(iv, nw)), 2.0f));
