@@ -37,6 +37,7 @@
 // barycentrics and depth bits are then recomputed with the same evaluation
 // chain.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -3145,9 +3146,18 @@ uint32_t bin_sort_tiles(uint32_t pair_cap) {
     return std::max((uint32_t)(((uint64_t)pair_cap + te - 1) / te), 1u);
 }
 
+// A launch that stamps the stage-timing events from its own dispatch (start
+// when the kernel begins, stop when it ends) where the frame asks for them:
+// a separate event record between two kernels holds the queue for several
+// microseconds.
+#define SR_LAUNCH_TIMED(e0, e1, kern, grid, block, st, ...)                                      \
+    do {                                                                                        \
+        if ((e0) || (e1)) hipExtLaunchKernelGGL(kern, grid, block, 0, st, e0, e1, 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL(kern, grid, block, 0, st, __VA_ARGS__);                          \
+    } while (0)
+
 hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const uint32_t GRID_CAP = 256 * 16;
-    if (a.events[0]) hipEventRecord(a.events[0], st);
     FrameBegin fb;
     fb.bin_count = a.bin_count;
     fb.items = a.total_items;
@@ -3155,13 +3165,15 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     fb.tr_slots = a.tr_slots;
     uint32_t tri_blocks = a.total_in > 0 ? grid_for(a.total_in, 256, GRID_CAP) : 0u;
     uint32_t vtx_blocks = a.total_xv > 0 ? grid_for(a.total_xv, 256, GRID_CAP) : 0u;
+    hipEvent_t ev_none = nullptr;
     if (tri_blocks + vtx_blocks > 0)
-        hipLaunchKernelGGL(k_front, dim3(tri_blocks + vtx_blocks), dim3(256), 0, st, a.views, a.draws, a.num_draws,
-                           a.total_in, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr,
-                           a.xv + 3 * (size_t)a.total_xv, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
-                           a.status, fb, tri_blocks);
+        SR_LAUNCH_TIMED(a.events[0], ev_none, k_front, dim3(tri_blocks + vtx_blocks), dim3(256), st, a.views, a.draws,
+                        a.num_draws, a.total_in, a.total_xv, a.xv, a.xv + a.total_xv, a.any_tangents ? a.xt : nullptr,
+                        a.xv + 3 * (size_t)a.total_xv, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
+                        a.status, fb, tri_blocks);
     else
-        hipLaunchKernelGGL(k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256), 0, st, fb);
+        SR_LAUNCH_TIMED(a.events[0], ev_none, k_frame_begin, dim3(grid_for(a.total_items, 256, GRID_CAP)), dim3(256),
+                        st, fb);
     if (a.total_in > 0) {
         hipLaunchKernelGGL(k_clip, dim3(512), dim3(256), 0, st, a.views, a.draws, a.num_draws, a.xv, a.xv + a.total_xv,
                            a.any_tangents ? a.xt : nullptr, a.tri_counts, a.tri_chunks, a.clip_queue, a.clip_cap,
@@ -3187,11 +3199,12 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     so.T = a.tile_size;
     so.chunk_pairs = (SR_GLOBAL uint32_t *)a.chunk_pairs;
     if (a.total_in > 0) {
-        hipLaunchKernelGGL(k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), 0, st, a.views, a.draws,
-                           a.num_draws, a.total_in, a.xv + 3 * (size_t)a.total_xv, a.tri_counts, a.tri_chunks,
-                           a.clip_buf, a.tri_base, so);
+        SR_LAUNCH_TIMED(ev_none, a.events[1], k_setup, dim3(grid_for(a.total_in, 256, GRID_CAP)), dim3(256), st,
+                        a.views, a.draws, a.num_draws, a.total_in, a.xv + 3 * (size_t)a.total_xv, a.tri_counts,
+                        a.tri_chunks, a.clip_buf, a.tri_base, so);
+    } else if (a.events[1]) {
+        hipEventRecord(a.events[1], st);
     }
-    if (a.events[1]) hipEventRecord(a.events[1], st);
     // bin offsets (bin_end inclusive) with the radix digit histograms, the
     // chunks' first pairs, then the stable binning
     const int npass = bin_sort_passes(a.total_items);
@@ -3273,7 +3286,6 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         hipLaunchKernelGGL(k_tr_sort, dim3(a.tr_slots), dim3(256), 0, st, a.tr_start, a.tr_end, a.tr_list, a.tr_cap,
                            a.tr_scratch, a.rec_raster, src);
     }
-    if (a.events[2]) hipEventRecord(a.events[2], st);
     RasterArgs P;
     P.views = (const SR_GLOBAL DevView *)a.views; P.num_views = a.num_views; P.total_items = a.total_items;
     P.bin_start = (const SR_GLOBAL uint32_t *)a.bin_start; P.bin_end = (const SR_GLOBAL uint32_t *)a.bin_end;
@@ -3289,27 +3301,28 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     P.center_color = (SR_GLOBAL float4 *)(a.has_translucent ? a.center_color : nullptr);
     P.persistent = a.persistent_blocks > 0;
     uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : a.total_items;
+    // stage events: bins done = the raster's start, the raster's end, and
+    // the frame's end = the shading's end (or the translucent pass's)
+    hipEvent_t e2 = a.events[2], e4 = a.events[4], e3 = a.has_translucent ? ev_none : a.events[3];
     switch (a.tile_size) {
-        case 8: hipLaunchKernelGGL((k_raster<8, 64>), dim3(grid), dim3(64), 0, st, P); break;
-        case 32: hipLaunchKernelGGL((k_raster<32, 512>), dim3(grid), dim3(512), 0, st, P); break;
+        case 8: SR_LAUNCH_TIMED(e2, e4, (k_raster<8, 64>), dim3(grid), dim3(64), st, P); break;
+        case 32: SR_LAUNCH_TIMED(e2, e4, (k_raster<32, 512>), dim3(grid), dim3(512), st, P); break;
         default:
-            if (a.hiz) hipLaunchKernelGGL((k_raster<16, 256, true>), dim3(grid), dim3(256), 0, st, P);
-            else hipLaunchKernelGGL((k_raster<16, 256>), dim3(grid), dim3(256), 0, st, P);
+            if (a.hiz) SR_LAUNCH_TIMED(e2, e4, (k_raster<16, 256, true>), dim3(grid), dim3(256), st, P);
+            else SR_LAUNCH_TIMED(e2, e4, (k_raster<16, 256>), dim3(grid), dim3(256), st, P);
             break;
     }
-    if (a.events[4]) hipEventRecord(a.events[4], st);
     switch (a.tile_size) {
-        case 8: hipLaunchKernelGGL((k_shade<8, 64>), dim3(a.total_items), dim3(64), 0, st, P); break;
-        case 32: hipLaunchKernelGGL((k_shade<32, 1024>), dim3(a.total_items), dim3(1024), 0, st, P); break;
-        default: hipLaunchKernelGGL((k_shade<16, 256>), dim3(a.total_items), dim3(256), 0, st, P); break;
+        case 8: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<8, 64>), dim3(a.total_items), dim3(64), st, P); break;
+        case 32: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<32, 1024>), dim3(a.total_items), dim3(1024), st, P); break;
+        default: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<16, 256>), dim3(a.total_items), dim3(256), st, P); break;
     }
     if (a.has_translucent) {
         TrArgs Q;
         Q.start = (const SR_GLOBAL uint32_t *)a.tr_start; Q.end = (const SR_GLOBAL uint32_t *)a.tr_end;
         Q.list = (const SR_GLOBAL uint32_t *)a.tr_list; Q.cap = a.tr_cap;
-        hipLaunchKernelGGL(k_translucent, dim3(a.tr_slots * 16u), dim3(256), 0, st, P, Q);
+        SR_LAUNCH_TIMED(ev_none, a.events[3], k_translucent, dim3(a.tr_slots * 16u), dim3(256), st, P, Q);
     }
-    if (a.events[3]) hipEventRecord(a.events[3], st);
     return hipGetLastError();
 }
 
