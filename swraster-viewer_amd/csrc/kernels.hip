@@ -2237,6 +2237,20 @@ __device__ __forceinline__ bool centre_tile(int tx0, int ty0) { return ((tx0 & 6
 // entry into an LDS depth buffer, then the visibility buffer (triangle id per
 // pixel, or none).
 // ---------------------------------------------------------------------------
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8).  The per-tile kernels take their items in runs of XCD_RUN
+// consecutive ones per XCD (run r on XCD r % 8): neighbouring tiles, which
+// read the same triangles, texels and voxels, then share one XCD's L2,
+// while the runs still interleave over the whole chip.  The grid is a
+// multiple of 8 x XCD_RUN; workgroups past the last item do nothing.
+#define XCD_RUN 8u
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b) {
+    const uint32_t x = b & 7u, k = b >> 3;
+    return ((k / XCD_RUN) * 8u + x) * XCD_RUN + (k % XCD_RUN);
+}
+
+static inline uint32_t xcd_grid(uint32_t items) { return (items + 8u * XCD_RUN - 1) / (8u * XCD_RUN) * (8u * XCD_RUN); }
+
 template <int T, int NT, bool HIZ>
 __device__ void raster_item(const RasterArgs &P, uint32_t item, unsigned long long *keys, uint32_t *stage) {
     constexpr int NW = NT / 64;               // waves
@@ -2338,7 +2352,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32
     __shared__ unsigned long long keys[T * T];
     __shared__ uint32_t stage[Stage<T>::COUNT * NT];
     if (!P.persistent) {
-        raster_item<T, NT, HIZ>(P, blockIdx.x, keys, stage);
+        const uint32_t item = xcd_item(blockIdx.x);
+        if (item < P.total_items) raster_item<T, NT, HIZ>(P, item, keys, stage);
         return;
     }
     __shared__ uint32_t next;
@@ -2612,7 +2627,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T * T >
     // writes it): the first workgroup stores it in the host's pinned copy
     // with system-scope stores (after its pixels, where no register is
     // live), so no read-back copy follows the frame
-    shade_item<T, NT>(P, blockIdx.x);
+    const uint32_t item = xcd_item(blockIdx.x);
+    if (item < P.total_items) shade_item<T, NT>(P, item);
     constexpr uint32_t STATUS_WORDS = sizeof(DevStatus) / 4;
     static_assert(STATUS_WORDS <= NT, "one thread per status word");
     if (blockIdx.x == 0 && threadIdx.x < STATUS_WORDS && P.status_out)
@@ -3300,7 +3316,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     P.center_ids = (SR_GLOBAL uint32_t *)a.center_ids;
     P.center_color = (SR_GLOBAL float4 *)(a.has_translucent ? a.center_color : nullptr);
     P.persistent = a.persistent_blocks > 0;
-    uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : a.total_items;
+    uint32_t grid = P.persistent ? (uint32_t)a.persistent_blocks : xcd_grid(a.total_items);
     // stage events: bins done = the raster's start, the raster's end, and
     // the frame's end = the shading's end (or the translucent pass's)
     hipEvent_t e2 = a.events[2], e4 = a.events[4], e3 = a.has_translucent ? ev_none : a.events[3];
@@ -3313,9 +3329,13 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
             break;
     }
     switch (a.tile_size) {
-        case 8: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<8, 64>), dim3(a.total_items), dim3(64), st, P); break;
-        case 32: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<32, 1024>), dim3(a.total_items), dim3(1024), st, P); break;
-        default: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<16, 256>), dim3(a.total_items), dim3(256), st, P); break;
+        case 8: SR_LAUNCH_TIMED(ev_none, e3, (k_shade<8, 64>), dim3(xcd_grid(a.total_items)), dim3(64), st, P); break;
+        case 32:
+            SR_LAUNCH_TIMED(ev_none, e3, (k_shade<32, 1024>), dim3(xcd_grid(a.total_items)), dim3(1024), st, P);
+            break;
+        default:
+            SR_LAUNCH_TIMED(ev_none, e3, (k_shade<16, 256>), dim3(xcd_grid(a.total_items)), dim3(256), st, P);
+            break;
     }
     if (a.has_translucent) {
         TrArgs Q;
