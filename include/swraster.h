@@ -440,6 +440,19 @@ SR_API int sr_renderer_read_color(sr_renderer *renderer, float *rgb);
 /* Centre-quad luminance of every 64px tile (src/tilerasterizer.rs:103-106). */
 SR_API int sr_renderer_read_tile_luminance(sr_renderer *renderer, float *lum, int32_t *count);
 SR_API int sr_renderer_get_stats(const sr_renderer *renderer, sr_frame_stats *stats);
+/* How frames time their stages for sr_frame_stats (the reference's clip/bin
+ * and raster timers, src/renderer.rs:206-219).  SR_STAGE_TIMING_CLOCK (the
+ * default): the kernels stamp the stage boundaries on the GPU's
+ * constant-rate wall clock, read back only when stats are asked for; it
+ * fills ms_setup, ms_bin, ms_raster_shade and ms_total (about 2 us per
+ * frame).  SR_STAGE_TIMING_EVENTS: HIP events carried by the stage kernels'
+ * own launches, which also split ms_raster from ms_shade (about 18 us per
+ * frame).  SR_STAGE_TIMING_OFF: no stage times (the ms fields read 0).
+ * Applies from the next frame. */
+#define SR_STAGE_TIMING_CLOCK 0
+#define SR_STAGE_TIMING_EVENTS 1
+#define SR_STAGE_TIMING_OFF 2
+SR_API int sr_renderer_set_stage_timing(sr_renderer *renderer, int32_t mode);
 
 /* One frame without allocating device memory.  No counterpart in the
  * reference: it is for callers that cannot allocate (code under HIP graph

@@ -28,6 +28,7 @@ SR_TEX_SRGB, SR_TEX_NORMAL, SR_TEX_METALLIC_ROUGHNESS, SR_TEX_LINEAR = 0, 1, 2, 
 SR_WRAP_REPEAT, SR_WRAP_MIRRORED_REPEAT, SR_WRAP_CLAMP_TO_EDGE = 0, 1, 2
 SR_ALPHA_OPAQUE, SR_ALPHA_MASK, SR_ALPHA_BLEND = 0, 1, 2
 NO_ID = 0xFFFFFFFF
+STAGE_TIMING = {"clock": 0, "events": 1, "off": 2}   # SR_STAGE_TIMING_*
 
 
 class SwrasterError(RuntimeError):
@@ -211,6 +212,7 @@ _SIGS = {
     "sr_renderer_read_color": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "sr_renderer_read_tile_luminance": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "sr_renderer_get_stats": (C.c_int, [_P, C.POINTER(sr_frame_stats)]),
+    "sr_renderer_set_stage_timing": (C.c_int, [_P, C.c_int32]),
     "sr_renderer_device_buffers": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "sr_batch_create": (C.c_int, [C.POINTER(sr_batch_opts), C.POINTER(_P)]),
     "sr_batch_destroy": (C.c_int, [_P]),
@@ -796,6 +798,12 @@ class Renderer:
         s = sr_frame_stats()
         check(lib().sr_renderer_get_stats(self.handle, C.byref(s)))
         return {k: getattr(s, k) for k, _ in sr_frame_stats._fields_}
+
+    def set_stage_timing(self, mode):
+        """How later frames time their stages: "clock" (default; the kernels'
+        wall-clock stamps), "events" (HIP events on the stage launches) or
+        "off"."""
+        check(lib().sr_renderer_set_stage_timing(self.handle, STAGE_TIMING[mode]))
 
     def close(self):
         if self.handle:

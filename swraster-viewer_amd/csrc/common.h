@@ -155,3 +155,19 @@ struct BinSortState {
     uint32_t chunk_total;         // pairs of all records (the expansion's size)
     uint32_t pad[3];
 };
+
+// Stage boundaries of one frame on the GPU's constant-rate wall clock
+// (wall_clock64, which ticks hipDeviceAttributeWallClockRate times per ms).
+// A kernel's first workgroup stamps its start: the transform's, the bin
+// scan's (the end of setup) and the raster's.  For the frame's end every
+// workgroup of the shading (and translucent) pass raises one of
+// CLOCK_SPREAD end slots to the time it finished (an atomic max that
+// returns nothing, spread so that few workgroups share an address), and the
+// end is the largest slot; the transform zeroes the end slots.
+#define CLOCK_SPREAD 256
+enum { CLOCK_FRONT = 0, CLOCK_SETUP_END, CLOCK_RASTER, CLOCK_STARTS };
+enum { CLOCK_SHADE_END = 0, CLOCK_TR_END, CLOCK_ENDS };
+struct StageClock {
+    uint64_t start[4];                          // CLOCK_STARTS used
+    uint64_t end[CLOCK_ENDS][CLOCK_SPREAD];     // shading, translucent shading
+};

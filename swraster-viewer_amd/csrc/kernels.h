@@ -35,6 +35,7 @@ struct FrameArgs {
     uint32_t *pair_keys2, *pair_vals2;  // pair_cap each: other radix pass's output (two passes or more)
     uint32_t *sort_hist;                // 256 * bin_sort_tiles(pair_cap): per-tile digit counts, then offsets
     BinSortState *sort;                 // zeroed by the upload, right after the status
+    StageClock *clock;                  // stage stamps (end slots zeroed by the transform; null: none)
     uint4 *rec_geo;               // rec_cap
     float4 *rec_raster;           // rec_cap * 4
     uint32_t rec_cap, pair_cap;
@@ -63,7 +64,8 @@ struct FrameArgs {
     float4 *center_color;         // 4 (colour, depth) per 64px tile
     int32_t persistent_blocks;    // 0 = one block per item
     int32_t hiz;                  // 16-px raster with depth rejection between task rounds
-    hipEvent_t events[5];         // optional stage markers: start, setup done, bins done, shade done, raster done
+    hipEvent_t events[5];         // optional stage events (the launches that carry them cost about 4 us each):
+                                  // start, setup done, bins done, shade done, raster done
 };
 
 // Stable binning sizes: bits of the bin index, radix passes (8 bits each;
@@ -82,10 +84,11 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t stream);
 // partials holds (n + 4095) / 4096 + 1 words.  With digit_passes > 0, the
 // input is a bin-count array and digit_hist (256 * digit_passes words,
 // zeroed beforehand) receives its radix digit histograms (bits 8p..8p+7 of
-// the bin index for pass p).
+// the bin index for pass p).  start_stamp (if not null) receives the wall
+// clock at the scan's start.
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
                        uint32_t *total, hipStream_t stream, uint32_t flags = 0, uint32_t *digit_hist = nullptr,
-                       int digit_passes = 0);
+                       int digit_passes = 0, uint64_t *start_stamp = nullptr);
 
 // Exposure + tonemap + quantize + pack of one view's linear colour planes;
 // frame_bytes stores each pixel as the bytes r, g, b, a.

@@ -196,14 +196,36 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
 // Clears the frame's bin counts over the whole grid, in k_front (or in
 // k_frame_begin when there are no vertices); the frame's status is cleared
 // by the upload that precedes the first kernel.
+// Stage stamps on the wall clock (StageClock): the kernel's first workgroup
+// stamps its start (workgroups are dispatched in index order).
+__device__ __forceinline__ void stamp_start(uint64_t *t) {
+    if (t && blockIdx.x == 0 && threadIdx.x == 0) *t = (uint64_t)wall_clock64();
+}
+__device__ __forceinline__ void clock_start(StageClock *c, int which) {
+    stamp_start(c ? &c->start[which] : nullptr);
+}
+// Every thread of the workgroup calls it after its work; the workgroup's
+// end raises its end slot.
+__device__ __forceinline__ void clock_end(StageClock *c, int which) {
+    if (!c) return;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(&c->end[which][blockIdx.x & (CLOCK_SPREAD - 1)]),
+                  (unsigned long long)wall_clock64());
+}
+
 struct FrameBegin {
     uint32_t *bin_count, *tr_count;
     uint32_t items, tr_slots;
+    StageClock *clock;
 };
 
 __device__ __forceinline__ void frame_begin(const FrameBegin &F) {
+    clock_start(F.clock, CLOCK_FRONT);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t k = i; k < F.items; k += gridDim.x * blockDim.x) F.bin_count[k] = 0u;
+    if (F.clock)
+        for (uint32_t k = i; k < CLOCK_ENDS * CLOCK_SPREAD; k += gridDim.x * blockDim.x) (&F.clock->end[0][0])[k] = 0ull;
     if (F.tr_count)
         for (uint32_t k = i; k < F.tr_slots; k += gridDim.x * blockDim.x) F.tr_count[k] = 0u;
 }
@@ -593,8 +615,10 @@ __device__ __forceinline__ void digit_hist_segment(uint32_t *lh, int passes, siz
     }
 }
 
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_partials(const uint32_t *in, uint32_t n, uint32_t *partials) {
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_partials(const uint32_t *in, uint32_t n, uint32_t *partials,
+                                                               uint64_t *stamp) {
     __shared__ uint32_t sh[17];
+    stamp_start(stamp);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     size_t seg = (size_t)blockIdx.x * SCAN_TILE + (size_t)wave * (64 * SCAN_ITEMS);
     uint32_t s = 0;
@@ -666,11 +690,13 @@ struct ScanJob {
 
 struct ScanJobs {
     ScanJob job[2];
+    uint64_t *stamp;              // start stamp (or null)
 };
 
 __global__ void __launch_bounds__(1024) k_scan_single(ScanJobs J) {
     __shared__ uint32_t sh[16];
     __shared__ uint32_t lh[512];
+    stamp_start(J.stamp);
     const ScanJob &q = J.job[blockIdx.x];
     const uint32_t *in = q.in;
     const uint32_t n = q.n;
@@ -717,17 +743,19 @@ static ScanJob scan_job(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t 
 }
 
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
-                       uint32_t *total, hipStream_t st, uint32_t flags, uint32_t *digit_hist, int digit_passes) {
+                       uint32_t *total, hipStream_t st, uint32_t flags, uint32_t *digit_hist, int digit_passes,
+                       uint64_t *start_stamp) {
     if (digit_passes <= 0) digit_hist = nullptr;
     if (n <= SCAN_SINGLE_MAX) {
         ScanJobs J;
         J.job[0] = scan_job(in, n, out, out2, total, flags, digit_hist, digit_passes);
         J.job[1] = J.job[0];
+        J.stamp = start_stamp;
         hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, J);
         return hipGetLastError();
     }
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, start_stamp);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total, nullptr);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2, flags, digit_hist,
                        digit_passes);
@@ -1913,6 +1941,7 @@ struct RasterArgs {
     SR_GLOBAL float4 *center_color;         // 4 (colour, depth) per 64px tile for the translucent pass (or null)
     SR_GLOBAL DevStatus *status;
     DevStatus *status_out;                  // pinned host memory for the frame's status
+    StageClock *clock;                      // stage stamps (or null)
     int32_t persistent;
 };
 
@@ -2351,20 +2380,21 @@ template <int T, int NT, bool HIZ = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T == 32 ? 4 : 5))) k_raster(RasterArgs P) {
     __shared__ unsigned long long keys[T * T];
     __shared__ uint32_t stage[Stage<T>::COUNT * NT];
+    __shared__ uint32_t next;
+    clock_start(P.clock, CLOCK_RASTER);
     if (!P.persistent) {
         const uint32_t item = xcd_item(blockIdx.x);
         if (item < P.total_items) raster_item<T, NT, HIZ>(P, item, keys, stage);
-        return;
-    }
-    __shared__ uint32_t next;
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(&P.status->work_counter, 1u);
-        __syncthreads();
-        uint32_t item = next;
-        __syncthreads();
-        if (item >= P.total_items) break;
-        raster_item<T, NT, HIZ>(P, item, keys, stage);
-        __syncthreads();
+    } else {
+        for (;;) {
+            if (threadIdx.x == 0) next = atomicAdd(&P.status->work_counter, 1u);
+            __syncthreads();
+            uint32_t item = next;
+            __syncthreads();
+            if (item >= P.total_items) break;
+            raster_item<T, NT, HIZ>(P, item, keys, stage);
+            __syncthreads();
+        }
     }
 }
 
@@ -2635,6 +2665,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(T * T >
         __hip_atomic_store(reinterpret_cast<uint32_t *>(P.status_out) + threadIdx.x,
                            reinterpret_cast<const SR_GLOBAL uint32_t *>(P.status)[threadIdx.x], __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_SYSTEM);
+    clock_end(P.clock, CLOCK_SHADE_END);
 }
 
 // ---------------------------------------------------------------------------
@@ -2899,8 +2930,7 @@ __device__ __forceinline__ int find_view_tile(const RasterArgs &P, uint32_t slot
 // barycentrics, so threads never wait on each other.  The block holding the
 // tile's centre quad also updates it (on screen or not) and the tile
 // luminance.
-__global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
-    __shared__ uint32_t st[G_COUNT * 256];
+__device__ __forceinline__ void translucent_block(const RasterArgs &P, const TrArgs &Q, uint32_t *st) {
     uint32_t slot = blockIdx.x >> 4, sub = blockIdx.x & 15u;
     uint32_t s = Q.start[slot], e = min(Q.end[slot], Q.cap);
     if (e <= s) return;
@@ -3030,6 +3060,12 @@ __global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
     float l0 = __shfl(lum, qbase, 64), l1 = __shfl(lum, qbase + 1, 64);
     float l2 = __shfl(lum, qbase + 2, 64), l3 = __shfl(lum, qbase + 3, 64);
     if (centre && sl == 0) P.out_lum[slot] = (((l0 + l1) + l2) + l3) * 0.25f;
+}
+
+__global__ void __launch_bounds__(256) k_translucent(RasterArgs P, TrArgs Q) {
+    __shared__ uint32_t st[G_COUNT * 256];
+    translucent_block(P, Q, st);
+    clock_end(P.clock, CLOCK_TR_END);
 }
 
 // ---------------------------------------------------------------------------
@@ -3179,6 +3215,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     fb.items = a.total_items;
     fb.tr_count = a.has_translucent ? a.tr_count : nullptr;
     fb.tr_slots = a.tr_slots;
+    fb.clock = a.clock;
     uint32_t tri_blocks = a.total_in > 0 ? grid_for(a.total_in, 256, GRID_CAP) : 0u;
     uint32_t vtx_blocks = a.total_xv > 0 ? grid_for(a.total_xv, 256, GRID_CAP) : 0u;
     hipEvent_t ev_none = nullptr;
@@ -3222,7 +3259,9 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         hipEventRecord(a.events[1], st);
     }
     // bin offsets (bin_end inclusive) with the radix digit histograms, the
-    // chunks' first pairs, then the stable binning
+    // chunks' first pairs, then the stable binning; the scan's start is the
+    // end of setup on the stage clock
+    uint64_t *setup_end = a.clock ? &a.clock->start[CLOCK_SETUP_END] : nullptr;
     const int npass = bin_sort_passes(a.total_items);
     if (npass < 1) return hipErrorInvalidValue;
     const int nbits = bin_sort_bits(a.total_items);
@@ -3232,11 +3271,12 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         J.job[0] = scan_job(a.bin_count, a.total_items, a.bin_start, a.bin_end, &a.status->total_pairs,
                             SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
         J.job[1] = scan_job(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, &a.sort->chunk_total, 0, nullptr, 0);
+        J.stamp = setup_end;
         hipLaunchKernelGGL(k_scan_single, dim3(2), dim3(1024), 0, st, J);
         e = hipGetLastError();
     } else {
         e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs,
-                        st, SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
+                        st, SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1, setup_end);
         if (e == hipSuccess && a.total_in > 0)
             e = launch_scan(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, a.scan_partials, &a.sort->chunk_total, st);
     }
@@ -3313,6 +3353,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     P.color_plane = a.color_plane; P.out_rgba = (SR_GLOBAL uint32_t *)a.out_rgba;
     P.out_lum = (SR_GLOBAL float *)a.out_lum; P.status = (SR_GLOBAL DevStatus *)a.status;
     P.status_out = a.status_out;
+    P.clock = a.clock;
     P.center_ids = (SR_GLOBAL uint32_t *)a.center_ids;
     P.center_color = (SR_GLOBAL float4 *)(a.has_translucent ? a.center_color : nullptr);
     P.persistent = a.persistent_blocks > 0;

@@ -33,6 +33,7 @@ struct sr_renderer {
     int ref_tiles = 0;
     float auto_exposure = DEFAULT_EXPOSURE, auto_exposure_target = DEFAULT_EXPOSURE, auto_exposure_ev = 1.0f;
     sr_frame_stats stats;
+    bool times_pending = false;  // the checked frame's stage times are read on request
     bool has_frame = false;
     // The last frame is queued and not yet checked for overflow; what a rerun
     // needs is kept until then (the scene counts it in pending_frames).
@@ -115,12 +116,50 @@ static void fill_stats(sr_renderer *r) {
     r->stats.tiles = r->p_items;
     r->stats.draws = (uint32_t)r->p_draws.size();
     r->stats.depth_rejection = r->pipe.last_hiz ? 1u : 0u;
-    r->stats.ms_setup = ev_ms(r->pipe.ev[0], r->pipe.ev[1]);
-    r->stats.ms_bin = ev_ms(r->pipe.ev[1], r->pipe.ev[2]);
-    r->stats.ms_raster_shade = ev_ms(r->pipe.ev[2], r->pipe.ev[3]);
-    r->stats.ms_raster = ev_ms(r->pipe.ev[2], r->pipe.ev[4]);
-    r->stats.ms_shade = ev_ms(r->pipe.ev[4], r->pipe.ev[3]);
-    r->stats.ms_total = ev_ms(r->pipe.ev[0], r->pipe.ev[3]);
+    r->times_pending = true;
+}
+
+// The checked frame's stage times, from its events or its wall-clock
+// stamps (which stay in the pipeline's clock buffer until the next frame).
+static void stage_times(sr_renderer *r) {
+    if (!r->times_pending) return;
+    r->times_pending = false;
+    sr_frame_stats &s = r->stats;
+    s.ms_setup = s.ms_bin = s.ms_raster_shade = s.ms_raster = s.ms_shade = s.ms_total = 0.0f;
+    const Pipeline &p = r->pipe;
+    if (p.timing == SR_STAGE_TIMING_EVENTS) {
+        s.ms_setup = ev_ms(p.ev[0], p.ev[1]);
+        s.ms_bin = ev_ms(p.ev[1], p.ev[2]);
+        s.ms_raster_shade = ev_ms(p.ev[2], p.ev[3]);
+        s.ms_raster = ev_ms(p.ev[2], p.ev[4]);
+        s.ms_shade = ev_ms(p.ev[4], p.ev[3]);
+        s.ms_total = ev_ms(p.ev[0], p.ev[3]);
+        return;
+    }
+    if (p.timing != SR_STAGE_TIMING_CLOCK || !p.clock_dev || p.clock_ticks_per_ms <= 0.0) return;
+    StageClock c;
+    hipSetDevice(r->device);
+    if (hipMemcpyAsync(&c, p.clock_dev, sizeof(c), hipMemcpyDeviceToHost, p.stream) != hipSuccess ||
+        hipStreamSynchronize(p.stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    auto end_of = [&](int k) {
+        uint64_t m = 0;
+        for (int i = 0; i < CLOCK_SPREAD; i++) m = std::max<uint64_t>(m, c.end[k][i]);
+        return m;
+    };
+    uint64_t t0 = c.start[CLOCK_FRONT], t1 = c.start[CLOCK_SETUP_END], t2 = c.start[CLOCK_RASTER];
+    uint64_t te = std::max(end_of(CLOCK_SHADE_END), end_of(CLOCK_TR_END));
+    auto ms = [&](uint64_t a, uint64_t b) {
+        return a && b && b >= a ? (float)((double)(b - a) / p.clock_ticks_per_ms) : 0.0f;
+    };
+    // the stamps split the frame as the reference's timers do (clip/bin,
+    // then raster with shading); the raster / shading split takes events
+    s.ms_setup = ms(t0, t1);
+    s.ms_bin = ms(t1, t2);
+    s.ms_raster_shade = ms(t2, te);
+    s.ms_total = ms(t0, te);
 }
 
 static hipError_t queue_exposure(sr_renderer *r) {
@@ -422,7 +461,19 @@ SR_API int sr_renderer_read_tile_luminance(sr_renderer *r, float *lum, int32_t *
 SR_API int sr_renderer_get_stats(const sr_renderer *r, sr_frame_stats *stats) {
     if (!r || !stats) return fail(SR_ERR_INVALID_ARGUMENT, "null argument");
     if (int rc = finish(const_cast<sr_renderer *>(r))) return rc;
+    stage_times(const_cast<sr_renderer *>(r));
     *stats = r->stats;
+    return SR_OK;
+}
+
+SR_API int sr_renderer_set_stage_timing(sr_renderer *r, int32_t mode) {
+    if (!r) return fail(SR_ERR_INVALID_ARGUMENT, "null renderer");
+    if (mode != SR_STAGE_TIMING_CLOCK && mode != SR_STAGE_TIMING_EVENTS && mode != SR_STAGE_TIMING_OFF)
+        return fail(SR_ERR_INVALID_ARGUMENT, "stage timing mode %d (0 clock, 1 events, 2 off)", mode);
+    // the queued frame keeps the mode it was launched with
+    if (int rc = finish(r)) return rc;
+    stage_times(r);
+    r->pipe.timing = mode;
     return SR_OK;
 }
 
@@ -467,6 +518,9 @@ SR_API int sr_batch_create(const sr_batch_opts *opts, sr_batch **out) {
         delete b;
         return rc;
     }
+    // a batch reports its whole time (events around the batch, or carried by
+    // its first and last launches when it runs without a graph)
+    b->pipe.timing = SR_STAGE_TIMING_EVENTS;
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;

@@ -196,6 +196,10 @@ int Pipeline::init(int dev, int tile) {
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    int khz = 0;
+    HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    clock_ticks_per_ms = (double)khz;
+    if (int r = clock_buf.ensure(sizeof(StageClock))) return r;
     // coherent: the shading kernel stores the frame's status here
     return status_host.ensure(sizeof(DevStatus), hipHostMallocCoherent);
 }
@@ -207,7 +211,7 @@ void Pipeline::destroy() {
     DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_chunks, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
                       &list, &rec_geo, &rec_raster, &center, &clip_queue, &clip_buf, &chunk_pairs, &pair_keys,
                       &pair_vals, &pair_keys2, &pair_vals2, &sort_hist, &tr_count,
-                      &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color};
+                      &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color, &clock_buf};
     for (DevBuf *b : bufs) b->release();
     stage.release();
     status_host.release();
@@ -240,7 +244,8 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t upload) {
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
                           a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
                           status_host.ptr, a.tr_count, a.tr_start, a.tr_end, a.tr_list, a.tr_scratch,
-                          a.center_color, a.chunk_pairs, a.pair_keys, a.pair_vals, a.pair_keys2, a.pair_vals2, a.sort_hist};
+                          a.center_color, a.chunk_pairs, a.pair_keys, a.pair_vals, a.pair_keys2, a.pair_vals2, a.sort_hist,
+                          a.clock};
     static_assert(sizeof(ptrs) / sizeof(ptrs[0]) <= sizeof(k.p) / sizeof(k.p[0]), "graph key too small");
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) k.p[i] = (uint64_t)(uintptr_t)ptrs[i];
     if (!graph_exec || memcmp(&k, &graph_key, sizeof(k)) != 0) {
@@ -359,6 +364,8 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.clip_cap = clip_cap;
         a.status = (DevStatus *)((char *)views.ptr + vbytes + dbytes);
         a.sort = (BinSortState *)((char *)a.status + sizeof(DevStatus));
+        a.clock = timing == SR_STAGE_TIMING_CLOCK ? (StageClock *)clock_buf.ptr : nullptr;
+        clock_dev = a.clock;
         a.pair_keys = (uint32_t *)pair_keys.ptr;
         a.pair_vals = (uint32_t *)pair_vals.ptr;
         a.pair_keys2 = (uint32_t *)pair_keys2.ptr;
@@ -394,7 +401,7 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
             if ((r = launch_graph(a, vbytes, upload))) return r;
         } else {
             HIP_TRY(hipMemcpyAsync(views.ptr, stage.ptr, upload, hipMemcpyHostToDevice, stream));
-            for (int i = 0; i < 5; i++) a.events[i] = ev[i];
+            for (int i = 0; i < 5; i++) a.events[i] = timing == SR_STAGE_TIMING_EVENTS ? ev[i] : nullptr;
             HIP_TRY(launch_frame(a, stream));
         }
         if (!sync) return SR_OK;

@@ -437,6 +437,7 @@ SR_API int sr_server_create(const sr_server_opts *opts, sr_server **out) {
             delete s;
             return rc;
         }
+        sl.pipe.timing = SR_STAGE_TIMING_OFF;    // request latency is taken on the host
     }
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread([s] { s->run(); });
