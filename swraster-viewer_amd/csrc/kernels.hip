@@ -2014,10 +2014,16 @@ __device__ __forceinline__ bool block_out(const TriSetup &ts, int bx, int by) {
 // origin evaluation, at most 16 chain additions), each off by at most 2^-24
 // of a magnitude bounded by P = |A| Xm + |B| Ym + |C| + 512 (|A| + |B|), so
 // it differs from E by less than M = 2^-19 P; a sample with E < -M is never
-// covered.  Per row, each edge with A != 0 bounds X on one side, taken at
-// the looser of the row's two sample lines; the bounds are evaluated in
-// double with a slack far above its rounding error.  The argument needs
-// SX = 32 A and SY = 32 B (no i32 wrap), else the full rows are kept.
+// covered.  The spans work in f32 relative to the tile's origin (Xt, Yt):
+// E = A X' + B Y' + C' with C' = C + A Xt + B Yt, whose three roundings are
+// off by at most 2^-22 P, so E < -M2 with M2 = 2^-18 P (P as computed, at
+// most 5 roundings low) excludes the sample.  Per row, each edge with A != 0
+// bounds X' on one side, -(M2 + C') / A - (B / A) Y', taken at the looser of
+// the row's two sample lines; its operands and products are each off by a
+// few 2^-24 (the reciprocal by one ulp), in all less than 2^-21 (|base| +
+// |slope| Y'max), and the slack is twice that plus 16 (one pixel) for the
+// rounding of the column bounds.  The argument needs SX = 32 A and SY = 32 B
+// (no i32 wrap), else the full rows are kept.
 template <int T, int SW>
 __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint32_t gflags, int tx0, int ty0,
                                            uint32_t *st, int lane) {
@@ -2062,25 +2068,30 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
     bool spans = nqx * nqy > 4;
 #pragma unroll
     for (int e = 0; e < 3; e++) spans = spans && fabsf(ts.A[e]) < 67108864.0f && fabsf(ts.B[e]) < 67108864.0f;
-    double base[3], slope[3], slack[3];
+    float base[3], slope[3], slack[3];
     int kind[3];
+    const int qxr = qx0 - tx0, qyr = qy0 - ty0;     // first quad relative to the tile (pixels)
     if (spans) {
-        double Xm = (double)((tx0 + T) * 16 + 24), Ym = (double)((ty0 + T) * 16 + 24);
+        const float Xt = (float)(tx0 * 16), Yt = (float)(ty0 * 16);
+        const float Xm = (float)((tx0 + T) * 16 + 24), Ym = (float)((ty0 + T) * 16 + 24);
+        const float Ymr = (float)(T * 16 + 24);
 #pragma unroll
         for (int e = 0; e < 3; e++) {
-            double A = ts.A[e], B = ts.B[e], C = ts.C[e];
-            double M = 0x1p-19 * (fabs(A) * Xm + fabs(B) * Ym + fabs(C) + 512.0 * (fabs(A) + fabs(B)));
-            kind[e] = A > 0.0 ? 1 : (A < 0.0 ? -1 : 0);
+            float A = ts.A[e], B = ts.B[e], C = ts.C[e];
+            float P = ((fabsf(A) * Xm + fabsf(B) * Ym) + fabsf(C)) + 512.0f * (fabsf(A) + fabsf(B));
+            float M2 = 0x1p-18f * P;
+            float Cr = (C + A * Xt) + B * Yt;
+            kind[e] = A > 0.0f ? 1 : (A < 0.0f ? -1 : 0);
             if (kind[e] != 0) {
-                // E >= -M  <=>  A X >= -(M + C + B Y)
-                double inv = 1.0 / A;
-                base[e] = -(M + C) * inv;
+                // E >= -M2  <=>  A X' >= -(M2 + C' + B Y')
+                float inv = __builtin_amdgcn_rcpf(A);
+                base[e] = -(M2 + Cr) * inv;
                 slope[e] = -B * inv;
             } else {
-                base[e] = M + C;      // feasible where B Y + C + M >= 0
+                base[e] = M2 + Cr;    // feasible where B Y' + C' + M2 >= 0
                 slope[e] = B;
             }
-            slack[e] = 16.0 + 0x1p-40 * (fabs(base[e]) + fabs(slope[e]) * Ym);
+            slack[e] = 16.0f + 0x1p-20f * (fabsf(base[e]) + fabsf(slope[e]) * Ymr);
         }
     }
     int cum = 0;
@@ -2092,19 +2103,19 @@ __device__ __forceinline__ int stage_entry(const TriSetup &ts, uint32_t id, uint
             if (!spans) {
                 n = nqx;
             } else {
-                double Y0 = (double)((qy0 + 2 * r) * 16 + 8), Y1 = Y0 + 16.0;
-                double xlo = -1.0e300, xhi = 1.0e300;
+                float Y0 = (float)((qyr + 2 * r) * 16 + 8), Y1 = Y0 + 16.0f;
+                float xlo = -INFINITY, xhi = INFINITY;
                 bool feas = true;
 #pragma unroll
                 for (int e = 0; e < 3; e++) {
-                    double b0 = fma(slope[e], Y0, base[e]), b1 = fma(slope[e], Y1, base[e]);
-                    if (kind[e] > 0) xlo = fmax(xlo, fmin(b0, b1) - slack[e]);
-                    else if (kind[e] < 0) xhi = fmin(xhi, fmax(b0, b1) + slack[e]);
-                    else feas = feas && fmax(b0, b1) >= -slack[e];
+                    float b0 = fmaf(slope[e], Y0, base[e]), b1 = fmaf(slope[e], Y1, base[e]);
+                    if (kind[e] > 0) xlo = fmaxf(xlo, fminf(b0, b1) - slack[e]);
+                    else if (kind[e] < 0) xhi = fminf(xhi, fmaxf(b0, b1) + slack[e]);
+                    else feas = feas && fmaxf(b0, b1) >= -slack[e];
                 }
-                // quad column c has samples at X = (qx0 + 2c) * 16 + 8 and + 24
-                double clo = fmax(ceil((xlo - 24.0 - 16.0 * qx0) * (1.0 / 32.0)), 0.0);
-                double chi = fmin(floor((xhi - 8.0 - 16.0 * qx0) * (1.0 / 32.0)), (double)(nqx - 1));
+                // quad column c has samples at X' = (qxr + 2c) * 16 + 8 and + 24
+                float clo = fmaxf(ceilf((xlo - 24.0f - 16.0f * (float)qxr) * (1.0f / 32.0f)), 0.0f);
+                float chi = fminf(floorf((xhi - 8.0f - 16.0f * (float)qxr) * (1.0f / 32.0f)), (float)(nqx - 1));
                 if (feas && clo <= chi) {
                     l = (int)clo;
                     n = (int)chi - l + 1;
