@@ -473,6 +473,14 @@ __global__ void __launch_bounds__(256) k_clip(const DevView *views, const DevDra
         int len = 3;
         bool over = false;
         for (int p = 0; p < 6 && len > 0; p++) {
+            // a plane that every vertex is inside leaves the polygon as it is
+            // (the pass would copy it in order): the group's lanes test one
+            // vertex each and skip the pass (a 16-vertex polygon still takes
+            // it, which fails the frame as the pass would)
+            float dv = 0.0f;
+            if (c < len) dv = plane_dot(p, P[c][0], P[c][1], P[c][2], P[c][3]);
+            uint64_t outm = __ballot(c < len && !(dv >= 0.0f));
+            if (((outm >> (threadIdx.x & 48u)) & 0xFFFFull) == 0ull && len < CLIP_MAXV) continue;
             int nl = 0;
             for (int i = 0; i < len; i++) {
                 int ip = i == 0 ? len - 1 : i - 1;
