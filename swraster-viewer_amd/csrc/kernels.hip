@@ -644,12 +644,24 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t 
                                                   uint32_t *total_out2) {
     __shared__ uint32_t sh[17];
     uint32_t carry = 0;
-    for (uint32_t c = 0; c < nparts; c += 1024) {
-        uint32_t i = c + threadIdx.x;
-        uint32_t v = i < nparts ? partials[i] : 0u;
+    // four consecutive elements per thread, all loaded before one workgroup
+    // scan (4096 per round)
+    for (uint32_t c = 0; c < nparts; c += 4096u) {
+        const uint32_t i = c + 4u * threadIdx.x;
+        uint32_t v[4], sum = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = i + k < nparts ? partials[i + k] : 0u;
+            sum += v[k];
+        }
         uint32_t ex;
-        uint32_t tot = block_excl_scan(v, ex, sh);
-        if (i < nparts) partials[i] = carry + ex;
+        uint32_t tot = block_excl_scan(sum, ex, sh);
+        uint32_t run = carry + ex;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i + k < nparts) partials[i + k] = run;
+            run += v[k];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) {
@@ -1402,12 +1414,25 @@ __global__ void __launch_bounds__(256) k_bin_rowscan(BinPass P) {
         uint32_t v = threadIdx.x < d ? P.bs->digit_hist[P.pass][threadIdx.x] : 0u;
         base = block_excl_scan(v, ex, sh);
     }
+    // four consecutive tiles per thread, all loaded before one workgroup
+    // scan (1024 tiles per round)
     SR_GLOBAL uint32_t *row = P.hist + (size_t)d * P.hist_cols;
-    for (uint32_t t0 = 0; t0 < ntiles; t0 += 256u) {
-        uint32_t t = t0 + threadIdx.x;
-        uint32_t v = t < ntiles ? row[t] : 0u, ex;
-        uint32_t tot = block_excl_scan(v, ex, sh);
-        if (t < ntiles) row[t] = base + ex;
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += 1024u) {
+        const uint32_t t = t0 + 4u * threadIdx.x;
+        uint32_t v[4], sum = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            v[k] = t + k < ntiles ? row[t + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t ex;
+        uint32_t tot = block_excl_scan(sum, ex, sh);
+        uint32_t run = base + ex;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (t + k < ntiles) row[t + k] = run;
+            run += v[k];
+        }
         base += tot;
     }
 }
