@@ -240,18 +240,19 @@ def test_gpu_zero_depth_ties(sr, orc, scenes):
 # ---------------------------------------------------------------------------
 # bins in submission order
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("config", ["c1", "c2", "c3_small", "c1_tiny"])
+@pytest.mark.parametrize("config", ["c1", "c2", "c3_small", "c1_tiny", "c1_8k"])
 def test_bins_hold_submission_order(sr, scenes, env, config):
     """Every tile's list holds ascending set-up triangle ids (the order the
     reference's single-threaded front end queues packets), each once, and
     two renders of a frame give the same lists bit for bit.  The frames
-    cover one radix pass over the bin index (c1_tiny: 96 bins) and two
-    (the others; batches of more than 65,536 bins take three, covered by
-    the C4 batch tests)."""
+    cover one radix pass over the bin index (c1_tiny: 96 bins), two (c1,
+    c2, c3_small) and three (c1_8k: 7680x4320, 129,600 bins)."""
     if config == "c3_small":
         scene, spec = scenes.make_c3(width=1280, height=720, triangles=20000, primitives=64)
     elif config == "c1_tiny":
         scene, spec = scenes.make_c1(width=160, height=90)
+    elif config == "c1_8k":
+        scene, spec = scenes.make_c1(width=7680, height=4320)
     else:
         scene, spec = scenes.CONFIGS[config]()
     scene.prepare(env, device=0)
