@@ -148,10 +148,12 @@ struct DevStatus {
 };
 
 // Working state of the stable binning (radix passes over the bin index),
-// uploaded zeroed right after the status: the digit histograms of the
-// passes before the last and the expansion's total.
+// right after the status (every field is written in the frame before it is
+// read): each pass's element count per digit (its row scan writes them, its
+// scatter scans them into the digits' first positions) and the expansion's
+// total.
 struct BinSortState {
-    uint32_t digit_hist[2][256];
+    uint32_t digit_total[3][256];
     uint32_t chunk_total;         // pairs of all records (the expansion's size)
     uint32_t pad[3];
 };

@@ -25,16 +25,15 @@ struct FrameArgs {
     uint32_t *tri_counts;         // total_in
     uint32_t *tri_chunks;         // (total_in + 255) / 256 fan-count sums, then first ids
     uint32_t *tri_base;           // total_in (+1)
-    uint32_t *bin_count;          // total_items
-    uint32_t *bin_start;          // total_items
-    uint32_t *bin_end;            // total_items (cursor, then end)
+    uint32_t *bin_start;          // total_items: each bin's first list entry (zeroed at the frame's start)
+    uint32_t *bin_end;            // total_items: one past its last (0 for a bin without entries)
     uint32_t *scan_partials;      // scratch for scans
     uint32_t *list;               // pair_cap record ids, each bin's in submission order
     uint32_t *chunk_pairs;              // (total_in + 255) / 256 + 1: pairs of each setup chunk, then first pair
     uint32_t *pair_keys, *pair_vals;    // pair_cap each: (bin, record) pairs in record order, then every
-    uint32_t *pair_keys2, *pair_vals2;  // pair_cap each: other radix pass's output (two passes or more)
+    uint32_t *pair_keys2, *pair_vals2;  // pair_cap each: the other pass output (vals: two passes or more)
     uint32_t *sort_hist;                // 256 * bin_sort_tiles(pair_cap): per-tile digit counts, then offsets
-    BinSortState *sort;                 // zeroed by the upload, right after the status
+    BinSortState *sort;                 // right after the status
     StageClock *clock;                  // stage stamps (end slots zeroed by the transform; null: none)
     uint4 *rec_geo;               // rec_cap
     float4 *rec_raster;           // rec_cap * 4
@@ -80,15 +79,12 @@ uint32_t bin_sort_tiles(uint32_t pair_cap);
 hipError_t launch_frame(const FrameArgs &a, hipStream_t stream);
 
 // Exclusive scan of n u32 (out, and out2 if not null: the same, or the
-// inclusive scan when flags has bit 0); the total goes to *total (device).
-// partials holds (n + 4095) / 4096 + 1 words.  With digit_passes > 0, the
-// input is a bin-count array and digit_hist (256 * digit_passes words,
-// zeroed beforehand) receives its radix digit histograms (bits 8p..8p+7 of
-// the bin index for pass p).  start_stamp (if not null) receives the wall
-// clock at the scan's start.
+// inclusive scan when flags has bit 0); the total goes to *total (device)
+// and to *total2 if not null.  partials holds (n + 4095) / 4096 + 1 words.
+// start_stamp (if not null) receives the wall clock at the scan's start.
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
-                       uint32_t *total, hipStream_t stream, uint32_t flags = 0, uint32_t *digit_hist = nullptr,
-                       int digit_passes = 0, uint64_t *start_stamp = nullptr);
+                       uint32_t *total, hipStream_t stream, uint32_t flags = 0, uint32_t *total2 = nullptr,
+                       uint64_t *start_stamp = nullptr);
 
 // Exposure + tonemap + quantize + pack of one view's linear colour planes;
 // frame_bytes stores each pixel as the bytes r, g, b, a.

@@ -215,7 +215,7 @@ __device__ __forceinline__ void clock_end(StageClock *c, int which) {
 }
 
 struct FrameBegin {
-    uint32_t *bin_count, *tr_count;
+    uint32_t *bin_start, *bin_end, *tr_count;
     uint32_t items, tr_slots;
     StageClock *clock;
 };
@@ -223,7 +223,10 @@ struct FrameBegin {
 __device__ __forceinline__ void frame_begin(const FrameBegin &F) {
     clock_start(F.clock, CLOCK_FRONT);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t k = i; k < F.items; k += gridDim.x * blockDim.x) F.bin_count[k] = 0u;
+    for (uint32_t k = i; k < F.items; k += gridDim.x * blockDim.x) {
+        F.bin_start[k] = 0u;
+        F.bin_end[k] = 0u;
+    }
     if (F.clock)
         for (uint32_t k = i; k < CLOCK_ENDS * CLOCK_SPREAD; k += gridDim.x * blockDim.x) (&F.clock->end[0][0])[k] = 0ull;
     if (F.tr_count)
@@ -604,25 +607,6 @@ __device__ __forceinline__ void wave_store_segment(uint32_t *out, uint32_t *out2
     }
 }
 
-// Digit histograms of a scanned bin-count array for the stable binning's
-// radix passes before the last: dh[p * 256 + d] = the number of pairs whose
-// bin has digit d in bits [8p, 8p + 8).  Element i of the array is bin i,
-// so digit 0 of its 64-aligned rows is the lane and digit 1 is uniform over
-// a row.
-__device__ __forceinline__ void digit_hist_segment(uint32_t *lh, int passes, size_t seg, const uint32_t v[SCAN_ITEMS]) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        size_t row = seg + (size_t)k * 64;
-        if (v[k]) atomicAdd(&lh[(row + (size_t)lane) & 255u], v[k]);
-        if (passes > 1) {
-            uint32_t inc = wave_incl_scan(v[k]);
-            uint32_t sum = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-            if (lane == 0 && sum) atomicAdd(&lh[256 + ((row >> 8) & 255u)], sum);
-        }
-    }
-}
-
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_partials(const uint32_t *in, uint32_t n, uint32_t *partials,
                                                                uint64_t *stamp) {
     __shared__ uint32_t sh[17];
@@ -671,13 +655,9 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t *partials, uint32_t 
 }
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, uint32_t n, const uint32_t *partials,
-                                                           uint32_t *out, uint32_t *out2, uint32_t flags,
-                                                           uint32_t *dh, int dh_passes) {
+                                                           uint32_t *out, uint32_t *out2, uint32_t flags) {
     __shared__ uint32_t sh[SCAN_BLOCK / 64];
-    __shared__ uint32_t lh[512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (dh)
-        for (int j = threadIdx.x; j < 512; j += SCAN_BLOCK) lh[j] = 0u;
     size_t seg = (size_t)blockIdx.x * SCAN_TILE + (size_t)wave * (64 * SCAN_ITEMS);
     uint32_t ex[SCAN_ITEMS], v[SCAN_ITEMS];
     uint32_t tot = wave_scan_segment(in, seg, n, ex, v);
@@ -686,12 +666,6 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, u
     uint32_t base = partials[blockIdx.x];
     for (int w = 0; w < wave; w++) base += sh[w];
     wave_store_segment(out, out2, seg, n, base, ex, v, flags);
-    if (dh) {
-        digit_hist_segment(lh, dh_passes, seg, v);
-        __syncthreads();
-        for (int j = threadIdx.x; j < 256 * dh_passes; j += SCAN_BLOCK)
-            if (lh[j]) atomicAdd(&dh[j], lh[j]);
-    }
 }
 
 // One workgroup scans arrays of up to SCAN_SINGLE_MAX elements in a single
@@ -702,10 +676,8 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t *in, u
 struct ScanJob {
     const uint32_t *in;
     uint32_t n;
-    uint32_t *out, *out2, *total;
+    uint32_t *out, *out2, *total, *total2;
     uint32_t flags;
-    uint32_t *dh;
-    int dh_passes;
 };
 
 struct ScanJobs {
@@ -715,17 +687,11 @@ struct ScanJobs {
 
 __global__ void __launch_bounds__(1024) k_scan_single(ScanJobs J) {
     __shared__ uint32_t sh[16];
-    __shared__ uint32_t lh[512];
     stamp_start(J.stamp);
     const ScanJob &q = J.job[blockIdx.x];
     const uint32_t *in = q.in;
     const uint32_t n = q.n;
-    uint32_t *dh = q.dh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (dh) {
-        for (int j = threadIdx.x; j < 512; j += 1024) lh[j] = 0u;
-        __syncthreads();
-    }
     uint32_t carry = 0;
     for (uint32_t c = 0; c < n; c += 1024u * SCAN_ITEMS) {
         size_t seg = (size_t)c + (size_t)wave * (64 * SCAN_ITEMS);
@@ -739,36 +705,33 @@ __global__ void __launch_bounds__(1024) k_scan_single(ScanJobs J) {
             all += sh[w];
         }
         wave_store_segment(q.out, q.out2, seg, n, base, ex, v, q.flags);
-        if (dh) digit_hist_segment(lh, q.dh_passes, seg, v);
         carry += all;
         __syncthreads();
     }
-    if (threadIdx.x == 0 && q.total) *q.total = carry;
-    if (dh)
-        for (int j = threadIdx.x; j < 256 * q.dh_passes; j += 1024) dh[j] = lh[j];
+    if (threadIdx.x == 0) {
+        if (q.total) *q.total = carry;
+        if (q.total2) *q.total2 = carry;
+    }
 }
 
-static ScanJob scan_job(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *total, uint32_t flags,
-                        uint32_t *digit_hist, int digit_passes) {
+static ScanJob scan_job(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *total,
+                        uint32_t *total2, uint32_t flags) {
     ScanJob j;
     j.in = in;
     j.n = n;
     j.out = out;
     j.out2 = out2;
     j.total = total;
+    j.total2 = total2;
     j.flags = flags;
-    j.dh = digit_passes > 0 ? digit_hist : nullptr;
-    j.dh_passes = digit_passes > 0 ? digit_passes : 0;
     return j;
 }
 
 hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *out2, uint32_t *partials,
-                       uint32_t *total, hipStream_t st, uint32_t flags, uint32_t *digit_hist, int digit_passes,
-                       uint64_t *start_stamp) {
-    if (digit_passes <= 0) digit_hist = nullptr;
+                       uint32_t *total, hipStream_t st, uint32_t flags, uint32_t *total2, uint64_t *start_stamp) {
     if (n <= SCAN_SINGLE_MAX) {
         ScanJobs J;
-        J.job[0] = scan_job(in, n, out, out2, total, flags, digit_hist, digit_passes);
+        J.job[0] = scan_job(in, n, out, out2, total, total2, flags);
         J.job[1] = J.job[0];
         J.stamp = start_stamp;
         hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, J);
@@ -776,9 +739,8 @@ hipError_t launch_scan(const uint32_t *in, uint32_t n, uint32_t *out, uint32_t *
     }
     uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, start_stamp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total, nullptr);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2, flags, digit_hist,
-                       digit_passes);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partials, nb, total, total2);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, n, partials, out, out2, flags);
     return hipGetLastError();
 }
 
@@ -809,46 +771,11 @@ __device__ __forceinline__ Rect bin_rect(const DevView &V, int mnx, int mny, int
 struct SetupOut {
     SR_GLOBAL uint4 *geo;
     SR_GLOBAL float4 *raster;
-    SR_GLOBAL uint32_t *bin_count;
     SR_GLOBAL uint32_t *chunk_pairs;      // (triangle, tile) pairs of each 256-triangle chunk's records
     uint32_t cap;
     SR_GLOBAL DevStatus *status;
     int T;
 };
-
-// Wave-aggregated bin atomics: every lane of the wave calls these together;
-// lanes with bin == ~0u take no part.  Lanes are grouped by bin first (ballot
-// rounds, no memory traffic), then each group's first lane adds the group
-// size with one atomic instruction for all groups, so the wave waits for at
-// most one round trip.
-__device__ __forceinline__ void wave_groups(uint32_t bin, int &leader, uint32_t &rank, uint32_t &group) {
-    bool pending = bin != 0xFFFFFFFFu;
-    int lane = threadIdx.x & 63;
-    leader = lane;
-    rank = 0;
-    group = 0;
-    for (;;) {
-        uint64_t pm = __ballot(pending);
-        if (!pm) break;
-        int first = __ffsll((unsigned long long)pm) - 1;
-        uint32_t lb = (uint32_t)__builtin_amdgcn_readlane((int)bin, first);
-        bool mine = pending && bin == lb;
-        uint64_t m = __ballot(mine);
-        if (mine) {
-            leader = first;
-            rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        }
-        if (lane == first) group = (uint32_t)__popcll(m);
-        pending = pending && !mine;
-    }
-}
-
-__device__ __forceinline__ void wave_count(uint32_t *counts, uint32_t bin) {
-    int leader;
-    uint32_t rank, group;
-    wave_groups(bin, leader, rank, group);
-    if (group) atomicAdd(&counts[bin], group);
-}
 
 __device__ __forceinline__ int32_t top_left_bias(int32_t a, int32_t b) { return (a < 0 || (a == 0 && b > 0)) ? 0 : -1; }
 
@@ -1066,14 +993,7 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 
-// Bin counting / scattering for one rectangle per lane, called by the whole
-// wave.  Rectangles of at most SMALL_RECT bins (the common small-triangle
-// case, where neighbouring lanes hit the same bins) use wave-aggregated
-// atomics.  The bins of larger ones are laid out as one list over the wave
-// (a prefix sum of their sizes) and taken 64 at a time, one bin per lane, so
-// every step issues 64 independent atomics whatever the mix of sizes.
-#define SMALL_RECT 4
-
+// The bins of a rectangle of GPU tiles (row-major over its view's tiles).
 struct RectBins {
     int nb;            // bins in the rectangle (0 if empty)
     int rw;            // rectangle width in tiles
@@ -1094,44 +1014,6 @@ __device__ __forceinline__ RectBins rect_bins(const DevView *views, int view, Re
         b.first = V.item_base + (uint32_t)(r.gy0 * V.tiles_x + r.gx0);
     }
     return b;
-}
-
-// Owner lane of flattened element k (all lanes call): the first lane whose
-// inclusive count exceeds k; before = elements of the lanes ahead of it.
-__device__ __forceinline__ int flat_owner(int incl, int k, int &before) {
-    int o = 0;
-#pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        int v = __shfl(incl, o + step - 1, 64);
-        if (v <= k) o += step;
-    }
-    int prev = __shfl(incl, o > 0 ? o - 1 : 0, 64);
-    before = o > 0 ? prev : 0;
-    return o;
-}
-
-// Returns the rectangle's bin count.
-__device__ __forceinline__ int bin_rect_count(const DevView *views, int view, Rect r, uint32_t *counts) {
-    RectBins b = rect_bins(views, view, r);
-    int lane = threadIdx.x & 63;
-    int ns = b.nb <= SMALL_RECT ? b.nb : 0;
-    int maxs = wave_max(ns);
-    for (int k = 0; k < maxs; k++) {
-        uint32_t bin = 0xFFFFFFFFu;
-        if (k < ns) bin = b.first + (uint32_t)((k / b.rw) * b.stride + k % b.rw);
-        wave_count(counts, bin);
-    }
-    int incl = (int)wave_incl_scan((uint32_t)(b.nb > SMALL_RECT ? b.nb : 0));
-    int total = __builtin_amdgcn_readlane(incl, 63);
-    for (int k0 = 0; k0 < total; k0 += 64) {
-        int k = k0 + lane, before;
-        int o = flat_owner(incl, k, before);
-        uint32_t first = (uint32_t)__shfl((int)b.first, o, 64);
-        int rw = __shfl(b.rw, o, 64), stride = __shfl(b.stride, o, 64);
-        int kk = k - before;
-        if (k < total) atomicAdd(&counts[first + (uint32_t)((kk / rw) * stride + kk % rw)], 1u);
-    }
-    return b.nb;
 }
 
 // Setup of input triangle t of draw D (index d), which the classification (k_front) found
@@ -1199,7 +1081,7 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
                 if (c == 1u) r = setup_input_triangle(draws[d], d, views, t, xs, id, O);
             }
         }
-        uint32_t np = (uint32_t)bin_rect_count(views, view, r, O.bin_count);
+        uint32_t np = (uint32_t)rect_bins(views, view, r).nb;
         int nclip = (c & SR_COUNT_CLIPPED) ? (int)nf : 0;
         int maxf = wave_max(nclip);
         for (int f = 0; f < maxf; f++) {
@@ -1207,7 +1089,7 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
             if (f < nclip)
                 rf = emit_clipped_fan(views, draws[d], clip_buf, (c & ~SR_COUNT_CLIPPED) >> 4, (uint32_t)f,
                                       id + (uint32_t)f, O);
-            np += (uint32_t)bin_rect_count(views, view, rf, O.bin_count);
+            np += (uint32_t)rect_bins(views, view, rf).nb;
         }
         uint32_t npx;
         uint32_t chunk_np = block_excl_scan(np, npx, sh);
@@ -1228,16 +1110,17 @@ __global__ void __launch_bounds__(256) k_setup(const DevView *views, const DevDr
 // the per-chunk pair counts that k_setup leaves; then a
 // least-significant-digit radix sort on the global bin index, 8 bits per
 // pass, orders them by bin.  Every pass is stable, so a bin keeps ascending
-// record ids, and the last pass writes each record id straight to
-// bin_start[bin] + its rank in the bin.
+// record ids; the last pass writes the record ids (the lists) and their
+// bins, and k_bin_bounds marks where each bin's run starts and ends.  No
+// per-bin counts are taken beforehand (their atomics cost k_setup a third
+// of its time).
 //
 // A pass is count, scan, scatter over tiles of 256 x IT elements (a
 // contiguous quarter per wave): k_bin_hist counts each tile's digits into a
 // digit-major matrix, k_bin_rowscan turns each digit's row into the tiles'
-// output offsets (from the digit's start: for the last pass bin_start of
-// the digit's first bin, for earlier ones the digit histograms that the
-// bin-count scan leaves in BinSortState), and k_bin_scatter writes each
-// tile.  Within a wave, elements with equal digits are ranked by lane with
+// offsets within the digit and leaves the digit's total, and k_bin_scatter
+// scans the totals into the digits' first positions and writes each tile.
+// Within a wave, elements with equal digits are ranked by lane with
 // ballots; the waves of a tile are ordered by their per-digit counts.  No
 // workgroup waits on another.
 // Tiles hold 256 x IT elements (IT per thread): 4 up to BS_SMALL_PAIRS
@@ -1252,13 +1135,13 @@ struct BinPass {
     const SR_GLOBAL uint32_t *chunk_pairs;          // expansion: first pair of each chunk
     uint32_t nchunks, parts;
     const SR_GLOBAL uint32_t *in_keys, *in_vals;    // radix passes: (bin, record) in the last pass's order
-    SR_GLOBAL uint32_t *out_keys, *out_vals;        // the expansion and radix passes before the last
+    SR_GLOBAL uint32_t *out_keys, *out_vals;        // the expansion and every pass (the last: keys only)
     SR_GLOBAL uint32_t *list;                       // the last pass: record ids per bin
     SR_GLOBAL uint32_t *hist;                       // 256 rows of hist_cols tile counts, then offsets
     uint32_t hist_cols;
-    const SR_GLOBAL uint32_t *bin_start;
     SR_GLOBAL DevStatus *status;
     const SR_GLOBAL BinSortState *bs;
+    SR_GLOBAL uint32_t *dtotal;                     // this pass's count per digit (row scan -> scatter)
     uint32_t total_items, rec_cap, pair_cap;
     uint32_t tile_elems;                            // elements per radix tile (256 x IT)
     int T, pass, last, shift, bits;
@@ -1405,18 +1288,10 @@ __global__ void __launch_bounds__(256) k_bin_rowscan(BinPass P) {
     __shared__ uint32_t sh[17];
     const uint32_t d = blockIdx.x;
     const uint32_t ntiles = (bs_elements(P) + P.tile_elems - 1) / P.tile_elems;
-    uint32_t base;
-    if (P.last) {
-        uint32_t b = d << P.shift;
-        base = b < P.total_items ? P.bin_start[b] : 0u;   // digits past the last bin hold no element
-    } else {
-        uint32_t ex;
-        uint32_t v = threadIdx.x < d ? P.bs->digit_hist[P.pass][threadIdx.x] : 0u;
-        base = block_excl_scan(v, ex, sh);
-    }
     // four consecutive tiles per thread, all loaded before one workgroup
-    // scan (1024 tiles per round)
+    // scan (1024 tiles per round); the digit's total goes to dtotal
     SR_GLOBAL uint32_t *row = P.hist + (size_t)d * P.hist_cols;
+    uint32_t base = 0u;
     for (uint32_t t0 = 0; t0 < ntiles; t0 += 1024u) {
         const uint32_t t = t0 + 4u * threadIdx.x;
         uint32_t v[4], sum = 0u;
@@ -1435,6 +1310,7 @@ __global__ void __launch_bounds__(256) k_bin_rowscan(BinPass P) {
         }
         base += tot;
     }
+    if (threadIdx.x == 0) P.dtotal[d] = base;
 }
 
 // Scatter: each tile's elements to their positions, in input order within
@@ -1471,9 +1347,10 @@ __global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
         // elements of digit d follow those of the waves before it
         const uint32_t d = threadIdx.x;
         uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
-        uint32_t lstart;
+        uint32_t lstart, dbase;
         block_excl_scan(c0 + c1 + c2 + c3, lstart, sh);
-        gofs[d] = P.hist[(size_t)d * P.hist_cols + t] - lstart;
+        block_excl_scan(P.dtotal[d], dbase, sh);     // the digit's first position
+        gofs[d] = dbase + P.hist[(size_t)d * P.hist_cols + t] - lstart;
         cnt[0][d] = lstart;
         cnt[1][d] = lstart + c0;
         cnt[2][d] = lstart + c0 + c1;
@@ -1511,13 +1388,33 @@ __global__ void __launch_bounds__(256) k_bin_scatter(BinPass P) {
         uint32_t kv = skey[i];
         uint32_t pos = gofs[(kv >> P.shift) & mask] + i;
         if (pos < P.pair_cap) {      // else the host sees total_pairs > pair_cap and runs the frame again
-            if (P.last) {
-                P.list[pos] = sval[i];
-            } else {
-                P.out_keys[pos] = kv;
-                P.out_vals[pos] = sval[i];
-            }
+            if (P.last) P.list[pos] = sval[i];
+            else P.out_vals[pos] = sval[i];
+            P.out_keys[pos] = kv;
         }
+    }
+}
+
+// Each tile's list bounds from the sorted bins: bin b's entries are
+// [bin_start[b], bin_end[b]); both arrays are zeroed at the frame's start,
+// so a bin without entries stays empty.  (The arguments come in a struct:
+// a kernel parameter typed with an address space is mangled into the
+// device-side name but not the host's, and the launch would find no
+// kernel.)
+struct BinBounds {
+    const SR_GLOBAL uint32_t *key;      // the last pass's bins, in list order
+    const SR_GLOBAL DevStatus *status;
+    uint32_t pair_cap, items;
+    SR_GLOBAL uint32_t *bin_start, *bin_end;
+};
+
+__global__ void __launch_bounds__(256) k_bin_bounds(BinBounds B) {
+    const uint32_t n = min(B.status->total_pairs, B.pair_cap);
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint32_t b = B.key[i];
+        if (b >= B.items) continue;    // (only in a frame that overflowed and runs again)
+        if (i == 0u || B.key[i - 1u] != b) B.bin_start[b] = i;
+        if (i + 1u == n || B.key[i + 1u] != b) B.bin_end[b] = i + 1u;
     }
 }
 
@@ -3255,7 +3152,8 @@ uint32_t bin_sort_tiles(uint32_t pair_cap) {
 hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     const uint32_t GRID_CAP = 256 * 16;
     FrameBegin fb;
-    fb.bin_count = a.bin_count;
+    fb.bin_start = a.bin_start;
+    fb.bin_end = a.bin_end;
     fb.items = a.total_items;
     fb.tr_count = a.has_translucent ? a.tr_count : nullptr;
     fb.tr_slots = a.tr_slots;
@@ -3292,7 +3190,7 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     if (e != hipSuccess) return e;
     SetupOut so;
     so.geo = (SR_GLOBAL uint4 *)a.rec_geo; so.raster = (SR_GLOBAL float4 *)a.rec_raster;
-    so.bin_count = (SR_GLOBAL uint32_t *)a.bin_count; so.cap = a.rec_cap; so.status = (SR_GLOBAL DevStatus *)a.status;
+    so.cap = a.rec_cap; so.status = (SR_GLOBAL DevStatus *)a.status;
     so.T = a.tile_size;
     so.chunk_pairs = (SR_GLOBAL uint32_t *)a.chunk_pairs;
     if (a.total_in > 0) {
@@ -3302,30 +3200,27 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
     } else if (a.events[1]) {
         hipEventRecord(a.events[1], st);
     }
-    // bin offsets (bin_end inclusive) with the radix digit histograms, the
-    // chunks' first pairs, then the stable binning; the scan's start is the
-    // end of setup on the stage clock
+    // the chunks' first pairs (their total is the frame's pair count), then
+    // the stable binning and the lists' bounds; the scan's start is the end
+    // of setup on the stage clock
     uint64_t *setup_end = a.clock ? &a.clock->start[CLOCK_SETUP_END] : nullptr;
     const int npass = bin_sort_passes(a.total_items);
     if (npass < 1) return hipErrorInvalidValue;
     const int nbits = bin_sort_bits(a.total_items);
-    // (one launch when both arrays fit one workgroup's scan)
-    if (a.total_in > 0 && a.total_items <= SCAN_SINGLE_MAX && nchunks <= SCAN_SINGLE_MAX) {
-        ScanJobs J;
-        J.job[0] = scan_job(a.bin_count, a.total_items, a.bin_start, a.bin_end, &a.status->total_pairs,
-                            SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1);
-        J.job[1] = scan_job(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, &a.sort->chunk_total, 0, nullptr, 0);
-        J.stamp = setup_end;
-        hipLaunchKernelGGL(k_scan_single, dim3(2), dim3(1024), 0, st, J);
-        e = hipGetLastError();
-    } else {
-        e = launch_scan(a.bin_count, a.total_items, a.bin_start, a.bin_end, a.scan_partials, &a.status->total_pairs,
-                        st, SCAN_OUT2_INCLUSIVE, &a.sort->digit_hist[0][0], npass - 1, setup_end);
-        if (e == hipSuccess && a.total_in > 0)
-            e = launch_scan(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, a.scan_partials, &a.sort->chunk_total, st);
-    }
-    if (e != hipSuccess) return e;
     if (a.total_in > 0) {
+        if (nchunks <= SCAN_SINGLE_MAX) {
+            ScanJobs J;
+            J.job[0] = scan_job(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, &a.sort->chunk_total,
+                                &a.status->total_pairs, 0u);
+            J.job[1] = J.job[0];
+            J.stamp = setup_end;
+            hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, J);
+            e = hipGetLastError();
+        } else {
+            e = launch_scan(a.chunk_pairs, nchunks, a.chunk_pairs, nullptr, a.scan_partials, &a.sort->chunk_total, st,
+                            0u, &a.status->total_pairs, setup_end);
+        }
+        if (e != hipSuccess) return e;
         BinPass bp = {};
         bp.views = (const SR_GLOBAL DevView *)a.views;
         bp.geo = (const SR_GLOBAL uint4 *)a.rec_geo;
@@ -3336,7 +3231,6 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         bp.hist = (SR_GLOBAL uint32_t *)a.sort_hist;
         bp.hist_cols = bin_sort_tiles(a.pair_cap);
         bp.tile_elems = 256u * bin_sort_items(a.pair_cap);
-        bp.bin_start = (const SR_GLOBAL uint32_t *)a.bin_start;
         bp.status = (SR_GLOBAL DevStatus *)a.status;
         bp.bs = (const SR_GLOBAL BinSortState *)a.sort;
         bp.total_items = a.total_items;
@@ -3344,7 +3238,8 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
         bp.pair_cap = a.pair_cap;
         bp.T = a.tile_size;
         // expansion into E (pair_keys/vals); the radix passes go E -> A ->
-        // E ... and the last one into the lists
+        // E ..., the last one's record ids into the lists and its bins into
+        // the key array it did not read
         uint32_t *keys[2] = {a.pair_keys, a.pair_keys2}, *vals[2] = {a.pair_vals, a.pair_vals2};
         bp.out_keys = (SR_GLOBAL uint32_t *)keys[0];
         bp.out_vals = (SR_GLOBAL uint32_t *)vals[0];
@@ -3355,9 +3250,10 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
             bp.last = p == npass - 1;
             bp.shift = 8 * p;
             bp.bits = std::min(8, nbits - 8 * p);
+            bp.dtotal = (SR_GLOBAL uint32_t *)&a.sort->digit_total[p][0];
             bp.in_keys = (const SR_GLOBAL uint32_t *)keys[p & 1];
             bp.in_vals = (const SR_GLOBAL uint32_t *)vals[p & 1];
-            bp.out_keys = (SR_GLOBAL uint32_t *)(bp.last ? nullptr : keys[(p + 1) & 1]);
+            bp.out_keys = (SR_GLOBAL uint32_t *)keys[(p + 1) & 1];
             bp.out_vals = (SR_GLOBAL uint32_t *)(bp.last ? nullptr : vals[(p + 1) & 1]);
             if (bp.tile_elems == 1024u) hipLaunchKernelGGL(k_bin_hist<4>, dim3(bp.hist_cols), dim3(256), 0, st, bp);
             else hipLaunchKernelGGL(k_bin_hist<16>, dim3(bp.hist_cols), dim3(256), 0, st, bp);
@@ -3366,6 +3262,14 @@ hipError_t launch_frame(const FrameArgs &a, hipStream_t st) {
                 hipLaunchKernelGGL(k_bin_scatter<4>, dim3(bp.hist_cols + 8u), dim3(256), 0, st, bp);
             else hipLaunchKernelGGL(k_bin_scatter<16>, dim3(bp.hist_cols + 8u), dim3(256), 0, st, bp);
         }
+        BinBounds bb;
+        bb.key = (const SR_GLOBAL uint32_t *)keys[npass & 1];
+        bb.status = (const SR_GLOBAL DevStatus *)a.status;
+        bb.pair_cap = a.pair_cap;
+        bb.items = a.total_items;
+        bb.bin_start = (SR_GLOBAL uint32_t *)a.bin_start;
+        bb.bin_end = (SR_GLOBAL uint32_t *)a.bin_end;
+        hipLaunchKernelGGL(k_bin_bounds, dim3(grid_for(a.pair_cap, 256, GRID_CAP)), dim3(256), 0, st, bb);
     }
     TriSource src;
     src.views = (const SR_GLOBAL DevView *)a.views;

@@ -13,7 +13,7 @@
 // device, and only that address is handed to copies and kernels.  Every
 // call on a device shares that device's array and leaves the last call's
 // contents in it; the frame clears what must start at zero itself (the
-// status and binning state by its upload, bin counts in the first kernel),
+// status by its upload, the tiles' list bounds in the first kernel),
 // and calls that overlap on one device race on it.
 
 #include <hip/hip_runtime.h>
@@ -95,17 +95,17 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     // layout: fixed parts sized by the frame, then clip room, records and
     // list entries from what is left
     size_t vbytes = 256, dbytes = (std::max<size_t>(draws.size(), 1) * sizeof(DevDraw) + 255) & ~(size_t)255;
-    size_t upload = vbytes + dbytes + sizeof(DevStatus) + sizeof(BinSortState);
+    size_t upload = vbytes + dbytes + sizeof(DevStatus);    // the binning state follows, not uploaded
     const int sort_passes = bin_sort_passes(items);
     if (sort_passes < 1) return fail(SR_ERR_CAPACITY, "%u screen tiles; at most 2^24", items);
     Carve c;
-    uint64_t o_views = c.take(upload);
+    uint64_t o_views = c.take(upload + sizeof(BinSortState));
     uint64_t o_xv = c.take((uint64_t)std::max<uint32_t>(total_xv, 1) * 64);
     uint64_t o_counts = c.take(((uint64_t)total_in + 1) * 4);
     uint64_t o_chunks = c.take(((uint64_t)total_in / 256 + 2) * 4);
     uint64_t o_chunk_pairs = c.take(((uint64_t)total_in / 256 + 2) * 4);
     uint64_t o_base = c.take(((uint64_t)total_in + 1) * 4);
-    uint64_t o_bins = c.take((uint64_t)std::max<uint32_t>(items, 1) * 12);
+    uint64_t o_bins = c.take((uint64_t)std::max<uint32_t>(items, 1) * 8);
     uint64_t maxn = std::max<uint64_t>((uint64_t)total_in + 1, items);
     uint64_t o_part = c.take(((maxn + 4095) / 4096 + 1) * 4);
     uint64_t o_center = c.take((uint64_t)ref_tiles * 16);
@@ -115,9 +115,10 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     uint64_t o_clipb = c.take((uint64_t)clip_cap * 16 * 16 * 4);
     const uint64_t cap = SR_NOALLOC_SCRATCH_BYTES;
     // records (80 B each: geometry and raster) and list entries (4 B each,
-    // 8 B of expanded pairs, 8 B more when the radix sort takes more than
-    // one pass, 1/4 B of tile digit counts; at least one entry per record)
-    const uint64_t rec_bytes = 80, pair_bytes = 4 + 8 + (sort_passes > 1 ? 8 : 0) + 1;
+    // 8 B of expanded pairs, 4 B of the other pass's keys and 4 B more of its
+    // values when the radix sort takes more than one pass, 1/4 B of tile
+    // digit counts; at least one entry per record)
+    const uint64_t rec_bytes = 80, pair_bytes = 4 + 8 + 4 + (sort_passes > 1 ? 4 : 0) + 1;
     uint64_t need_min = c.used + 256 * 8 + ((uint64_t)total_in + 1024) * (rec_bytes + pair_bytes);
     if (need_min > cap)
         return fail(SR_ERR_CAPACITY,
@@ -133,11 +134,11 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     uint64_t o_ras = c.take((uint64_t)rec_cap * 64);
     uint64_t o_list = c.take((uint64_t)pair_cap * 4);
     uint64_t o_pairs[4] = {0, 0, 0, 0};
-    for (int k = 0; k < (sort_passes > 1 ? 4 : 2); k++) o_pairs[k] = c.take((uint64_t)pair_cap * 4);
+    for (int k = 0; k < (sort_passes > 1 ? 4 : 3); k++) o_pairs[k] = c.take((uint64_t)pair_cap * 4);
     uint64_t o_hist = c.take((uint64_t)bin_sort_tiles(pair_cap) * 256 * 4);
     if (c.used > cap) return fail(SR_ERR_CAPACITY, "scratch layout exceeds %llu bytes", (unsigned long long)cap);
 
-    // view and draw tables, the zeroed status and binning state: one upload
+    // view and draw tables and the zeroed status: one upload
     std::vector<uint8_t> stage(upload, 0);
     memcpy(stage.data(), &view, sizeof(DevView));
     if (!draws.empty()) memcpy(stage.data() + vbytes, draws.data(), draws.size() * sizeof(DevDraw));
@@ -160,17 +161,14 @@ SR_API int sr_render_frame_noalloc(const sr_scene *scene, const sr_camera *camer
     a.tri_counts = (uint32_t *)(base + o_counts);
     a.tri_chunks = (uint32_t *)(base + o_chunks);
     a.tri_base = (uint32_t *)(base + o_base);
-    a.bin_count = (uint32_t *)(base + o_bins);
-    a.bin_start = a.bin_count + items;
+    a.bin_start = (uint32_t *)(base + o_bins);
     a.bin_end = a.bin_start + items;
     a.scan_partials = (uint32_t *)(base + o_part);
     a.list = (uint32_t *)(base + o_list);
     a.pair_keys = (uint32_t *)(base + o_pairs[0]);
     a.pair_vals = (uint32_t *)(base + o_pairs[1]);
-    if (sort_passes > 1) {
-        a.pair_keys2 = (uint32_t *)(base + o_pairs[2]);
-        a.pair_vals2 = (uint32_t *)(base + o_pairs[3]);
-    }
+    a.pair_keys2 = (uint32_t *)(base + o_pairs[2]);
+    if (sort_passes > 1) a.pair_vals2 = (uint32_t *)(base + o_pairs[3]);
     a.sort_hist = (uint32_t *)(base + o_hist);
     a.chunk_pairs = (uint32_t *)(base + o_chunk_pairs);
     a.rec_geo = (uint4 *)(base + o_geo);

@@ -149,7 +149,9 @@ static void stage_times(sr_renderer *r) {
         for (int i = 0; i < CLOCK_SPREAD; i++) m = std::max<uint64_t>(m, c.end[k][i]);
         return m;
     };
+    // a frame without triangles runs no setup: its setup ends where it starts
     uint64_t t0 = c.start[CLOCK_FRONT], t1 = c.start[CLOCK_SETUP_END], t2 = c.start[CLOCK_RASTER];
+    if (!t1) t1 = t0;
     uint64_t te = std::max(end_of(CLOCK_SHADE_END), end_of(CLOCK_TR_END));
     auto ms = [&](uint64_t a, uint64_t b) {
         return a && b && b >= a ? (float)((double)(b - a) / p.clock_ticks_per_ms) : 0.0f;
@@ -485,8 +487,21 @@ SR_API int sr_renderer_read_bins(sr_renderer *r, uint32_t *start, uint32_t *end,
     hipSetDevice(r->device);
     size_t n = r->p_items;
     if (items) *items = (uint32_t)n;
-    if (start) HIP_TRY(hipMemcpy(start, r->pipe.bin_start.ptr, n * 4, hipMemcpyDeviceToHost));
-    if (end) HIP_TRY(hipMemcpy(end, r->pipe.bin_end.ptr, n * 4, hipMemcpyDeviceToHost));
+    if (start || end) {
+        // the device marks the bounds of tiles with entries and leaves the
+        // others (0, 0); the tiles' runs follow each other in tile order, so
+        // an empty tile starts and ends where the previous one ends
+        std::vector<uint32_t> s(n), e(n);
+        HIP_TRY(hipMemcpy(s.data(), r->pipe.bin_start.ptr, n * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(e.data(), r->pipe.bin_end.ptr, n * 4, hipMemcpyDeviceToHost));
+        uint32_t at = 0;
+        for (size_t i = 0; i < n; i++) {
+            if (e[i] == s[i]) s[i] = e[i] = at;
+            at = e[i];
+        }
+        if (start) memcpy(start, s.data(), n * 4);
+        if (end) memcpy(end, e.data(), n * 4);
+    }
     if (list && list_len) {
         uint64_t total = std::min<uint64_t>(r->pipe.last_pairs, r->pipe.pair_cap);
         HIP_TRY(hipMemcpy(list, r->pipe.list.ptr, std::min<uint64_t>(total, list_len) * 4, hipMemcpyDeviceToHost));

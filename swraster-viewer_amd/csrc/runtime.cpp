@@ -208,7 +208,7 @@ void Pipeline::destroy() {
     hipSetDevice(device);
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
-    DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_chunks, &tri_base, &bin_count, &bin_start, &bin_end, &partials,
+    DevBuf *bufs[] = {&views, &xv, &xt, &tri_counts, &tri_chunks, &tri_base, &bin_start, &bin_end, &partials,
                       &list, &rec_geo, &rec_raster, &center, &clip_queue, &clip_buf, &chunk_pairs, &pair_keys,
                       &pair_vals, &pair_keys2, &pair_vals2, &sort_hist, &tr_count,
                       &tr_start, &tr_end, &tr_list, &tr_scratch, &center_color, &clock_buf};
@@ -240,7 +240,7 @@ int Pipeline::launch_graph(const FrameArgs &a, size_t vbytes, size_t upload) {
     k.n[8] = a.pair_cap; k.n[9] = a.clip_cap; k.n[10] = (uint64_t)a.persistent_blocks; k.n[11] = vbytes;
     k.n[12] = upload; k.n[13] = a.color_plane; k.n[14] = (uint64_t)a.has_translucent;
     k.n[15] = ((uint64_t)a.tr_cap << 32) | a.tr_slots | ((uint64_t)a.hiz << 31);
-    const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_chunks, a.tri_base, a.bin_count, a.bin_start, a.bin_end,
+    const void *ptrs[] = {a.views, a.draws, a.xv, a.xt, a.tri_counts, a.tri_chunks, a.tri_base, a.bin_start, a.bin_end,
                           a.scan_partials, a.list, a.rec_geo, a.rec_raster, a.clip_queue, a.clip_buf, a.status,
                           a.out_ids, a.out_depth, a.out_color, a.out_rgba, a.out_lum, a.center_ids, stage.ptr,
                           status_host.ptr, a.tr_count, a.tr_start, a.tr_end, a.tr_list, a.tr_scratch,
@@ -277,25 +277,24 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
     HIP_TRY(hipSetDevice(device));
     // the view table, the draw table and the frame's status (zeroed) share
     // one device buffer at 256-byte aligned offsets, so one copy uploads the
-    // tables and clears the status before the first kernel
+    // tables and clears the status before the first kernel; the binning
+    // state follows the status (the frame writes it before reading it)
     size_t vbytes = (hv.size() * sizeof(DevView) + 255) & ~(size_t)255;
     size_t dbytes = (std::max<size_t>(hd.size(), 1) * sizeof(DevDraw) + 255) & ~(size_t)255;
-    size_t upload = vbytes + dbytes + sizeof(DevStatus) + sizeof(BinSortState);
+    size_t upload = vbytes + dbytes + sizeof(DevStatus);
     const int sort_passes = bin_sort_passes(total_items);
     if (sort_passes < 1) return fail(SR_ERR_CAPACITY, "%u screen tiles in one frame; at most 2^24", total_items);
     int r;
     if ((r = stage.ensure(upload))) return r;
     memcpy(stage.ptr, hv.data(), hv.size() * sizeof(DevView));
     if (!hd.empty()) memcpy((char *)stage.ptr + vbytes, hd.data(), hd.size() * sizeof(DevDraw));
-    memset((char *)stage.ptr + vbytes + dbytes + sizeof(DevStatus), 0, sizeof(BinSortState));
-    if ((r = views.ensure(upload))) return r;
+    if ((r = views.ensure(upload + sizeof(BinSortState)))) return r;
     if ((r = xv.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 64))) return r;
     if (any_tangents && (r = xt.ensure((size_t)std::max<uint32_t>(total_xv, 1) * 16))) return r;
     if ((r = tri_counts.ensure(((size_t)total_in + 1) * 4))) return r;
     if ((r = tri_chunks.ensure(((size_t)total_in / 256 + 2) * 4))) return r;
     if ((r = tri_base.ensure(((size_t)total_in + 1) * 4))) return r;
     if ((r = chunk_pairs.ensure(((size_t)total_in / 256 + 2) * 4))) return r;
-    if ((r = bin_count.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_start.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     if ((r = bin_end.ensure((size_t)std::max<uint32_t>(total_items, 1) * 4))) return r;
     size_t lum_slots = 1;
@@ -328,9 +327,9 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         if ((r = clip_queue.ensure((size_t)clip_cap * 4))) return r;
         if ((r = clip_buf.ensure((size_t)clip_cap * 16 * 16 * 4))) return r;
         if ((r = pair_keys.ensure((size_t)pair_cap * 4)) || (r = pair_vals.ensure((size_t)pair_cap * 4))) return r;
-        if (sort_passes > 1 &&
-            ((r = pair_keys2.ensure((size_t)pair_cap * 4)) || (r = pair_vals2.ensure((size_t)pair_cap * 4))))
-            return r;
+        // (the last pass writes its bins to the key array it does not read)
+        if ((r = pair_keys2.ensure((size_t)pair_cap * 4))) return r;
+        if (sort_passes > 1 && (r = pair_vals2.ensure((size_t)pair_cap * 4))) return r;
         if ((r = sort_hist.ensure((size_t)bin_sort_tiles(pair_cap) * 256 * 4))) return r;
         if (has_tr && (r = tr_list.ensure((size_t)tr_cap * 4))) return r;
         if (has_tr && (r = tr_scratch.ensure((size_t)tr_cap * 8))) return r;
@@ -350,7 +349,6 @@ int Pipeline::run(const std::vector<DevView> &hv, const std::vector<DevDraw> &hd
         a.tri_counts = (uint32_t *)tri_counts.ptr;
         a.tri_chunks = (uint32_t *)tri_chunks.ptr;
         a.tri_base = (uint32_t *)tri_base.ptr;
-        a.bin_count = (uint32_t *)bin_count.ptr;
         a.bin_start = (uint32_t *)bin_start.ptr;
         a.bin_end = (uint32_t *)bin_end.ptr;
         a.scan_partials = (uint32_t *)partials.ptr;

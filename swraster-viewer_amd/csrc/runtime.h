@@ -63,7 +63,7 @@ struct Pipeline {
     double clock_ticks_per_ms = 0.0;      // the device's wall clock (StageClock)
     StageClock *clock_dev = nullptr;      // the last eager frame's stamps (clock_buf)
     DevBuf clock_buf;
-    DevBuf views, xv, xt, tri_counts, tri_chunks, tri_base, bin_count, bin_start, bin_end, partials, list;
+    DevBuf views, xv, xt, tri_counts, tri_chunks, tri_base, bin_start, bin_end, partials, list;
     DevBuf rec_geo, rec_raster, center, clip_queue, clip_buf;
     DevBuf chunk_pairs, pair_keys, pair_vals, pair_keys2, pair_vals2, sort_hist;
     DevBuf tr_count, tr_start, tr_end, tr_list, tr_scratch, center_color;

@@ -26,6 +26,20 @@ def test_library_exports_every_declared_symbol(sr):
     assert set(names) == set(sr._SIGS), "ctypes signatures out of sync with the header"
 
 
+def test_kernel_names_match_between_host_and_device(sr):
+    """Every kernel the host registers exists under the same name in the
+    gfx950 code object.  A kernel parameter typed with an address space
+    (address_space(1) pointers, which the device pass uses) is mangled into
+    the device-side name only (`U3AS1`), so the host's launch would find no
+    kernel and the runtime aborts: kernels take such pointers inside
+    structs."""
+    data = open(sr.LIB_PATH, "rb").read()
+    kernels = set(re.findall(rb"_ZN2sr[0-9]+k_[A-Za-z0-9_]+", data))
+    assert len(kernels) >= 20
+    with_as = sorted(k.decode() for k in kernels if b"U3AS" in k)
+    assert not with_as, f"kernels with address-space parameters: {with_as}"
+
+
 def test_version_and_device_count(sr):
     a, b = C.c_int32(), C.c_int32()
     sr.check(sr.lib().sr_version(C.byref(a), C.byref(b)))
