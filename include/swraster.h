@@ -199,7 +199,7 @@ typedef struct sr_frame_stats {
     uint64_t binned_pairs;         /* (triangle, tile) references */
     uint32_t tiles;
     uint32_t draws;
-    float ms_setup;                /* transform+clip+setup+bin count */
+    float ms_setup;                /* transform+clip+setup */
     float ms_bin;                  /* scans + scatter */
     float ms_raster_shade;         /* per-tile raster + vbuffer shading */
     float ms_blit;

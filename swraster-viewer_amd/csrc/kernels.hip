@@ -7,13 +7,13 @@
 //                   and the frustum classification of every input triangle
 //                   (fan counts, clip queue) src/renderer.rs:579-665
 //   k_clip          clipping of the queued triangles with all attributes
-//   k_scan_*        device-wide exclusive scan (post-clip ids, bin offsets)
+//   k_scan_*        device-wide exclusive scan (post-clip ids, chunk pairs)
 //   k_setup         clipping with attributes, snap, cull, bbox, setup records,
-//                   per-tile bin counts     src/renderer.rs:668-846
+//                   pair counts per chunk   src/renderer.rs:668-846
 //   k_bin_expand    stable binning: the records' pairs in record order
-//   k_bin_hist, k_bin_rowscan, k_bin_scatter
-//                   stable binning: radix passes over the bin index
-//                                           src/bumpqueue.rs:96-112
+//   k_bin_hist, k_bin_rowscan, k_bin_scatter, k_bin_bounds
+//                   stable binning: radix passes over the bin index, then
+//                   each tile's list bounds src/bumpqueue.rs:96-112
 //   k_raster_shade  one workgroup per screen tile: coverage with the
 //                   reference's exact f32 edge-evaluation chains into a 64-bit
 //                   (depth, id) LDS buffer, then visibility-buffer PBR
@@ -193,7 +193,7 @@ __device__ __forceinline__ int clip_classify(const float c[3][4]) {
     return -1;
 }
 
-// Clears the frame's bin counts over the whole grid, in k_front (or in
+// Clears the tiles' list bounds over the whole grid, in k_front (or in
 // k_frame_begin when there are no vertices); the frame's status is cleared
 // by the upload that precedes the first kernel.
 // Stage stamps on the wall clock (StageClock): the kernel's first workgroup
@@ -418,7 +418,7 @@ __device__ __forceinline__ void classify_triangles(const DevView *views, const D
 // The frame's first kernel: workgroups [0, tri_blocks) classify the input
 // triangles, the others transform the vertices (no data passes between the
 // two: the classification computes its clip positions itself), and every
-// workgroup helps clear the bin counts.
+// workgroup helps clear the tiles' list bounds.
 __global__ void __launch_bounds__(256) k_front(const DevView *__restrict__ views, const DevDraw *__restrict__ draws,
                                                int ndraws, uint32_t total_in, uint32_t total_xv,
                                                float4 *__restrict__ xp, float4 *__restrict__ xa,
@@ -766,7 +766,7 @@ __device__ __forceinline__ Rect bin_rect(const DevView &V, int mnx, int mny, int
 }
 
 // ---------------------------------------------------------------------------
-// k_setup: clip with attributes, snap, cull, bbox, setup records, bin counts
+// k_setup: clip with attributes, snap, cull, bbox, setup records, pair counts
 // ---------------------------------------------------------------------------
 struct SetupOut {
     SR_GLOBAL uint4 *geo;
