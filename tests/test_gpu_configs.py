@@ -7,7 +7,8 @@ equal the plain renderer's frames bit for bit; two views equal the oracle.
 C5: the eight resident scenes (12.3M triangles, 240 distinct 2048^2
 textures) served at 720p-2160p give each request the same visibility buffer
 and pixels alone, in the mix and in the plain renderer; two requests equal
-the oracle.  Ties: duplicated coplanar triangles spanning many 16-px tiles,
+the oracle.  Frames whose tiles get no list entry at all.  Ties: duplicated
+coplanar triangles spanning many 16-px tiles,
 and a layer at exactly zero depth, resolve to the later triangle as the
 reference's `z <= current` test leaves them (src/tilerasterizer.rs:511-523).
 """
@@ -208,6 +209,37 @@ def test_gpu_equal_depth_later_wins(sr, orc, scenes):
     assert hit.sum() > 20000
     assert np.all(g["ids"][hit] >= len(base)), "an earlier duplicate won an equal-depth tie"
     compare(sr, g, o, "equal-depth duplicates")
+
+
+@pytest.mark.parametrize("case", ["backfacing", "looking_away"])
+def test_gpu_frames_without_list_entries(sr, orc, scenes, case):
+    """A frame that sets up triangles but bins none (every triangle faces
+    away and is culled at setup: the radix binning sorts zero entries and
+    every tile's list stays empty) and a frame whose camera sees no
+    triangle at all: sky everywhere, equal to the oracle, and empty lists."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    w, h = 640, 360
+    tris = big_front_triangles(rng, 12, w, h, 50, 250)
+    if case == "backfacing":
+        tris = tris[:, [0, 2, 1]]          # clockwise on screen
+    scene, cam_spec = screen_triangle_scene(sr, tris, np.full(len(tris), 7.0), w, h)
+    attach_dummy(sr, scene)
+    if case == "looking_away":
+        cam_spec = dict(cam_spec, yaw=float(np.pi))
+    cam = scenes.make_camera(cam_spec)
+    gscene = sr.Scene(scene)
+    g = render_gpu(sr, gscene, cam, w, h)
+    o = render_oracle(orc, scene, cam, w, h)
+    assert np.all(g["ids"] == sr.NO_ID)
+    compare(sr, g, o, f"no list entries ({case})")
+    r = sr.Renderer(w, h)
+    r.render_scene(gscene, cam)
+    start, end, lst = r.read_bins()
+    st = r.stats()
+    r.close()
+    assert lst.size == 0 and st["binned_pairs"] == 0
+    assert np.all(start == 0) and np.all(end == 0)
+    assert st["setup_triangles"] == (len(tris) if case == "backfacing" else 0)
 
 
 def test_gpu_zero_depth_ties(sr, orc, scenes):
