@@ -151,7 +151,8 @@ def run_batched(args):
 
     reps = max(1, min(args.steps, 5))
     times = {}
-    for name, fn in (("plain", plain), ("batched", batched)):
+    legs = (("batched", batched),) if args.batch_only else (("plain", plain), ("batched", batched))
+    for name, fn in legs:
         fn()                                       # warm-up: buffers sized, code paged in
         hip.hipDeviceSynchronize()
         ts = []
@@ -176,9 +177,9 @@ def run_batched(args):
                    "persistent_blocks": tuned["persistent_blocks"], "hip_graph": not args.no_graph,
                    "step": "one batch of all views (batched) vs the same views as plain frames"},
         "autotune": {k: tuned.get(k) for k in ("tile_size", "persistent_blocks", "ms", "cached", "table")},
-        "plain_frames_per_s": round(n / times["plain"], 2),
-        "plain_ms": round(times["plain"] * 1e3, 3),
-        "batched_vs_plain": round(times["plain"] / times["batched"], 3),
+        "plain_frames_per_s": round(n / times["plain"], 2) if "plain" in times else None,
+        "plain_ms": round(times["plain"] * 1e3, 3) if "plain" in times else None,
+        "batched_vs_plain": round(times["plain"] / times["batched"], 3) if "plain" in times else None,
         "mtri_per_s": round(tris / times["batched"] / 1e6, 1),
         "gpix_per_s": round(px / times["batched"] / 1e9, 3),
         "batch_stats": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
@@ -317,6 +318,8 @@ def main():
     ap.add_argument("--requests", type=int, default=240, help="c5: requests per rate")
     ap.add_argument("--views", type=int, default=256, help="c4: views per batch")
     ap.add_argument("--no-graph", action="store_true", help="c4: launch the batch without a hipGraph")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="c4: time the batch alone (no plain frames, no ratio), e.g. for a kernel profile")
     ap.add_argument("--tile", type=int, default=0,
                     help="tile size (c4: skips the autotuner, one raster workgroup per (view, tile); c5: the server's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
