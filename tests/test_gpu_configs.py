@@ -7,7 +7,8 @@ equal the plain renderer's frames bit for bit; two views equal the oracle.
 C5: the eight resident scenes (12.3M triangles, 240 distinct 2048^2
 textures) served at 720p-2160p give each request the same visibility buffer
 and pixels alone, in the mix and in the plain renderer; two requests equal
-the oracle.  Frames whose tiles get no list entry at all.  Ties: duplicated
+the oracle.  Frames whose tiles get no list entry at all; zero-area
+triangles.  Ties: duplicated
 coplanar triangles spanning many 16-px tiles,
 and a layer at exactly zero depth, resolve to the later triangle as the
 reference's `z <= current` test leaves them (src/tilerasterizer.rs:511-523).
@@ -209,6 +210,60 @@ def test_gpu_equal_depth_later_wins(sr, orc, scenes):
     assert hit.sum() > 20000
     assert np.all(g["ids"][hit] >= len(base)), "an earlier duplicate won an equal-depth tie"
     compare(sr, g, o, "equal-depth duplicates")
+
+
+def degenerate_triangles(rng, n, w, h):
+    """n zero-area triangles in subpixel units: three collinear vertices
+    (integer multiples of one integer step, some spanning many tiles), two
+    coincident vertices, or all three coincident."""
+    lo, hi = np.array([16.0, 16.0]), np.array([16.0 * (w - 1), 16.0 * (h - 1)])
+    out = []
+    while len(out) < n:
+        a = np.round(rng.uniform(lo, hi))
+        kind = len(out) % 3
+        if kind == 0:
+            d = rng.integers(-40, 41, 2).astype(float)
+            if not d.any():
+                continue
+            m = rng.choice(np.arange(-60, 61), 3, replace=False).astype(float)
+            tri = a[None, :] + m[:, None] * d[None, :]
+        elif kind == 1:
+            b = np.round(rng.uniform(lo, hi))
+            tri = np.stack([a, a, b])[rng.permutation(3)]
+        else:
+            tri = np.stack([a, a, a])
+        if np.all(tri >= lo) and np.all(tri <= hi):      # inside the viewport: no clipping
+            out.append(tri)
+    return np.array(out)
+
+
+def test_gpu_zero_area_triangles(sr, orc, scenes):
+    """Zero-area triangles are kept by setup, as the reference keeps them
+    (it culls signed area > 0 only, src/renderer.rs:677-682; 1/|A| is then
+    +inf), and win no pixel: a sample the rounded edge functions let through
+    gets a NaN depth.  Submitted in front of large triangles, with their
+    vertices at different depths, they leave the visibility buffer and the
+    pixels equal to the oracle's."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    w, h = 640, 360
+    back = big_front_triangles(rng, 30, w, h, 40, 220)
+    degen = degenerate_triangles(rng, 600, w, h)
+    for t in degen:
+        area = (t[1, 0] - t[0, 0]) * (t[2, 1] - t[0, 1]) - (t[2, 0] - t[0, 0]) * (t[1, 1] - t[0, 1])
+        assert area == 0.0
+    tris = np.concatenate([back, degen])
+    depths = np.concatenate([np.repeat(rng.uniform(20, 40, (len(back), 1)), 3, 1),
+                             rng.uniform(2, 10, (len(degen), 3))])
+    scene, cam_spec = screen_triangle_scene(sr, tris, depths, w, h)
+    attach_dummy(sr, scene)
+    cam = scenes.make_camera(cam_spec)
+    g = render_gpu(sr, sr.Scene(scene), cam, w, h)
+    o = render_oracle(orc, scene, cam, w, h)
+    hit = g["ids"] != sr.NO_ID
+    assert hit.sum() > 20000
+    assert np.all(g["ids"][hit] < len(back)), "a zero-area triangle won a pixel"
+    assert g["stats"]["setup_triangles"] == o["setup"]
+    compare(sr, g, o, "zero-area triangles")
 
 
 @pytest.mark.parametrize("case", ["backfacing", "looking_away"])
