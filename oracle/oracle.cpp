@@ -27,8 +27,11 @@
 //
 // Parity status: the oracle is pinned by the CPU tests in tests/ (exact
 // integer rasterizer on small viewports, watertightness, clipper
-// invariants, sampler closed forms, tonemap/quantize closed forms) and by
-// the FP64 shading restatement below.  Agreement with the Rust binary itself
+// invariants, sampler closed forms, tonemap/quantize closed forms; the
+// shader's direct, SH, split-sum and environment terms, the skybox
+// direction and the exposure step against the published models written
+// independently in float64; the translucent sort and blend) and by the FP64
+// shading restatement below.  Agreement with the Rust binary itself
 // is unpinned where it depends on glam's SIMD operation order, rayon's
 // scheduling or platform libm (no Rust toolchain exists here).
 
