@@ -93,6 +93,29 @@ def test_c1_odd_viewports(sr, orc, scenes, c1, size):
     compare(sr, render_gpu(sr, gscene, cam, w, h), render_oracle(orc, scene, cam, w, h), f"c1 {w}x{h}")
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_c1_random_cameras(sr, orc, scenes, c1, seed):
+    """Random poses around and inside the sphere cluster (positions in a
+    box twice the cluster's size, looking at a random point near its centre
+    give or take 0.3 rad of yaw and pitch, field of view 20-110 degrees, far
+    5-60, so near and far planes cut spheres) at random viewport sizes up to
+    400x300, 8-, 16- or 32-px tiles: visibility buffer, setup count and
+    pixels equal the oracle's in every case."""
+    scene, spec, gscene = c1
+    rng = np.random.Generator(np.random.PCG64(1000 + seed))
+    w, h = int(rng.integers(16, 401)), int(rng.integers(16, 301))
+    pos = rng.uniform(-9.0, 9.0, 3)
+    yaw, pitch = scenes.look_pose(pos, rng.uniform(-3.0, 3.0, 3))
+    s = scenes.camera_spec(pos, yaw + rng.uniform(-0.3, 0.3), float(np.clip(pitch + rng.uniform(-0.3, 0.3), -1.4, 1.4)),
+                           rng.uniform(np.radians(20), np.radians(110)), w, h, rng.uniform(5.0, 60.0))
+    cam = scenes.make_camera(s)
+    tile = int(rng.choice([8, 16, 32]))
+    g = render_gpu(sr, gscene, cam, w, h, tile=tile)
+    o = render_oracle(orc, scene, cam, w, h)
+    assert g["stats"]["setup_triangles"] == o["setup"]
+    compare(sr, g, o, f"c1 camera {seed} ({w}x{h}, tile {tile})")
+
+
 def test_c1_inside_cluster_clipping(sr, orc, scenes, c1):
     """Camera inside the sphere cluster: near/side-plane clipping and
     negative depths (view-space cull near plane sits behind the eye)."""
@@ -152,6 +175,37 @@ def test_c2_parity(sr, orc, scenes, env):
     g = render_gpu(sr, gscene, cam, w, h)
     o = render_oracle(orc, scene, cam, w, h, threads=16)
     compare(sr, g, o, "c2")
+
+
+@pytest.fixture(scope="module")
+def c2(sr, scenes, env):
+    scene, spec = scenes.make_c2()
+    scene.prepare(env, cache_key=scene.name)
+    return scene, spec, sr.Scene(scene)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_c2_random_cameras(sr, orc, scenes, c2, seed):
+    """Random poses over the heightfield (1,045,458 triangles, 2048^2
+    texture): 3-30 units above it, looking at a random point of it give or
+    take 0.2 rad, field of view 30-90 degrees, so grazing views clip the
+    terrain at the near plane and distant ones shade from small mips; at
+    random viewport sizes up to 480x270 and 8-, 16- or 32-px tiles the
+    visibility buffer, setup count and pixels equal the oracle's."""
+    scene, spec, gscene = c2
+    rng = np.random.Generator(np.random.PCG64(2000 + seed))
+    w, h = int(rng.integers(32, 481)), int(rng.integers(32, 271))
+    pos = np.array([rng.uniform(-45.0, 45.0), rng.uniform(3.0, 30.0), rng.uniform(-45.0, 45.0)])
+    target = np.array([rng.uniform(-40.0, 40.0), 0.0, rng.uniform(-40.0, 40.0)])
+    yaw, pitch = scenes.look_pose(pos, target)
+    s = scenes.camera_spec(pos, yaw + rng.uniform(-0.2, 0.2), float(np.clip(pitch + rng.uniform(-0.2, 0.2), -1.5, 1.5)),
+                           rng.uniform(np.radians(30), np.radians(90)), w, h, spec["far"])
+    cam = scenes.make_camera(s)
+    tile = int(rng.choice([8, 16, 32]))
+    g = render_gpu(sr, gscene, cam, w, h, tile=tile)
+    o = render_oracle(orc, scene, cam, w, h, threads=16)
+    assert g["stats"]["setup_triangles"] == o["setup"]
+    compare(sr, g, o, f"c2 camera {seed} ({w}x{h}, tile {tile})")
 
 
 def test_c3_parity(sr, orc, scenes, env):
