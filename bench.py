@@ -475,7 +475,7 @@ def main():
         o = orc.OracleRenderer(w, h)
         osc = orc.OracleScene(scene)
         o.render_scene(osc, cam, threads=cores, canonical=False)
-        times = []
+        times, stages = [], []
         t_end = time.perf_counter() + args.cpu_seconds
         while time.perf_counter() < t_end and len(times) < 50:
             t = time.perf_counter()
@@ -483,12 +483,16 @@ def main():
             o.update_auto_exposure(0.0)
             o.blit()
             times.append(time.perf_counter() - t)
+            stages.append(o.stage_ms())
         med = float(np.median(times))
         out["cpu_baseline"] = {"value": round(w * h / med / 1e9, 5), "unit": "Gpix/s", "cores": cores,
                                "kind": "port",
                                "sample": f"{len(times)} frames of the same workload after 1 warm-up "
                                          f"(median {med * 1e3:.1f} ms/frame, "
-                                         f"{scene.triangles / med / 1e6:.2f} Mtri/s; {share})"}
+                                         f"{scene.triangles / med / 1e6:.2f} Mtri/s; {share})",
+                               # the reference's two stage timers (src/renderer.rs:206-220), medians
+                               "stages_ms": {"clip_bin": round(float(np.median([s[0] for s in stages])), 3),
+                                             "raster_shade": round(float(np.median([s[1] for s in stages])), 3)}}
         out["gpu_vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
         # the checker's FP64 shader on the GPU's visibility buffer
         o.render_scene(osc, cam, threads=cores, fp64=True)
