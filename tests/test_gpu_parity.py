@@ -500,12 +500,10 @@ def test_server_translucent_requests(sr, scenes, c1, env):
     srv.close()
 
 
-def test_alpha_tested_materials(sr, orc, scenes, env):
-    """MASK materials (src/shader.rs:40-61, :311-329): the base texture's
-    nearest-sampled alpha against the cutoff decides coverage; alpha from
-    block noise so whole regions pass or fail and edges fall inside
-    triangles.  Half the spheres are masked, with different cutoffs."""
-    scene, spec = scenes.make_c1(width=640, height=360)
+def _alpha_c1(sr, scenes, W, H):
+    """C1 with half the spheres alpha-masked (different cutoffs) and the
+    shared texture's alpha made of 8x8-texel blocks of noise."""
+    scene, spec = scenes.make_c1(width=W, height=H)
     rng = np.random.Generator(np.random.PCG64(77))
     w, h, fmt, px = scene.textures[0]
     rgba = sr.unpack_rgba(np.asarray(px, np.uint32)).reshape(h, w, 4).copy()
@@ -517,6 +515,15 @@ def test_alpha_tested_materials(sr, orc, scenes, env):
             m["alpha_mode"] = sr.SR_ALPHA_MASK
             m["alpha_cutoff"] = 0.25 + 0.05 * i
     scene.rebuild()
+    return scene, spec
+
+
+def test_alpha_tested_materials(sr, orc, scenes, env):
+    """MASK materials (src/shader.rs:40-61, :311-329): the base texture's
+    nearest-sampled alpha against the cutoff decides coverage; alpha from
+    block noise so whole regions pass or fail and edges fall inside
+    triangles.  Half the spheres are masked, with different cutoffs."""
+    scene, spec = _alpha_c1(sr, scenes, 640, 360)
     scene.prepare(env, cache_key=None, grid=32)
     cam = scenes.make_camera(spec)
     g = render_gpu(sr, sr.Scene(scene), cam, 640, 360)

@@ -1,16 +1,17 @@
-"""Parity sweep over random poses: many small frames of the sphere cluster,
-the heightfield and a 20,000-triangle overdraw scene, each at a random
+"""Parity sweep over random poses: many small frames of the sphere cluster
+(also with alpha-masked and with translucent materials), the heightfield
+and a 20,000-triangle overdraw scene, each at a random
 viewport (8-640 x 8-400), tile size (8, 16, 32) and exposure step (0 or
 0.1 s), compared with the oracle: visibility buffer bit for bit, setup
 count, RGBA8 within 1 LSB.  SR_SWEEP_SCALE multiplies the number of frames
-(the default takes about 15 s; 5 gives about 2,000 frames)."""
+(the default takes about 20 s; 5 gives about 2,600 frames)."""
 import os
 
 import numpy as np
 import pytest
 
 from conftest import bits
-from test_gpu_parity import render_gpu, render_oracle
+from test_gpu_parity import _alpha_c1, _translucent_c1, render_gpu, render_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -18,7 +19,7 @@ SCALE = float(os.environ.get("SR_SWEEP_SCALE", "1"))
 
 
 def random_pose(scenes, cfg, spec, rng, w, h):
-    if cfg == "c1":       # around and inside the cluster, aimed near its centre
+    if cfg.startswith("c1"):      # around and inside the cluster, aimed near its centre
         pos = rng.uniform(-9.0, 9.0, 3)
         target = rng.uniform(-3.0, 3.0, 3)
         fov, far = rng.uniform(np.radians(15), np.radians(120)), rng.uniform(3.0, 80.0)
@@ -36,15 +37,23 @@ def random_pose(scenes, cfg, spec, rng, w, h):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("cfg,frames", [("c1", 300), ("c2", 40), ("c3_small", 60)])
+@pytest.mark.parametrize("cfg,frames", [("c1", 300), ("c2", 40), ("c3_small", 60), ("c1_alpha", 60),
+                                        ("c1_translucent", 60)])
 def test_random_frames_match_oracle(sr, orc, scenes, env, cfg, frames):
     if cfg == "c1":
         scene, spec = scenes.make_c1()
     elif cfg == "c2":
         scene, spec = scenes.make_c2()
-    else:
+    elif cfg == "c3_small":
         scene, spec = scenes.make_c3(width=640, height=360, triangles=20000, primitives=64)
-    scene.prepare(env, cache_key=scene.name)
+    elif cfg == "c1_alpha":
+        scene, spec = _alpha_c1(sr, scenes, 640, 360)
+    else:
+        scene, spec = _translucent_c1(sr, scenes, 640, 360)
+    if cfg in ("c1_alpha", "c1_translucent"):
+        scene.prepare(env, cache_key=None, grid=32)
+    else:
+        scene.prepare(env, cache_key=scene.name)
     gscene = sr.Scene(scene)
     bad, covered = [], 0
     for seed in range(int(frames * SCALE)):
