@@ -216,7 +216,8 @@ typedef struct sr_view {
     const struct sr_scene *scene;
     sr_camera camera;
     int32_t width, height;
-    float exposure;           /* exposure applied by the fused blit */
+    float exposure;           /* exposure applied by the fused blit (<= 0: the
+                                 reference's initial 2.0) */
 } sr_view;
 
 typedef struct sr_batch_opts {
