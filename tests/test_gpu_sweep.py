@@ -1,10 +1,12 @@
-"""Parity sweep over random poses: many small frames of the sphere cluster
+"""Parity sweeps over random poses.  Against the oracle: many small frames of the sphere cluster
 (also with alpha-masked and with translucent materials), the heightfield
 and a 20,000-triangle overdraw scene, each at a random
 viewport (8-640 x 8-400), tile size (8, 16, 32) and exposure step (0 or
 0.1 s), compared with the oracle: visibility buffer bit for bit, setup
 count, RGBA8 within 1 LSB.  SR_SWEEP_SCALE multiplies the number of frames
-(the default takes about 20 s; 5 gives about 2,600 frames)."""
+(the default takes about 20 s; 5 gives about 2,600 frames).  Against the
+plain renderer: random views over five scenes in one batch and as served
+requests."""
 import os
 
 import numpy as np
@@ -72,3 +74,89 @@ def test_random_frames_match_oracle(sr, orc, scenes, env, cfg, frames):
     print(f"[{cfg}] {int(frames * SCALE)} frames, {covered} with coverage, {len(bad)} differing")
     assert covered > frames * SCALE * 0.8
     assert not bad, "; ".join(bad[:5])
+
+
+def five_scenes(sr, scenes, env):
+    """The cluster (plain, alpha-masked, translucent), the heightfield and a
+    20,000-triangle overdraw scene, uploaded, with their camera specs."""
+    built = {}
+    for cfg in ("c1", "c2", "c3_small", "c1_alpha", "c1_translucent"):
+        if cfg == "c1":
+            scene, spec = scenes.make_c1()
+        elif cfg == "c2":
+            scene, spec = scenes.make_c2()
+        elif cfg == "c3_small":
+            scene, spec = scenes.make_c3(width=640, height=360, triangles=20000, primitives=64)
+        elif cfg == "c1_alpha":
+            scene, spec = _alpha_c1(sr, scenes, 640, 360)
+        else:
+            scene, spec = _translucent_c1(sr, scenes, 640, 360)
+        if cfg in ("c1_alpha", "c1_translucent"):
+            scene.prepare(env, cache_key=None, grid=32)
+        else:
+            scene.prepare(env, cache_key=scene.name)
+        built[cfg] = (sr.Scene(scene), spec)
+    return built
+
+
+def random_views(sr, scenes, built, seed, n):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    names = list(built)
+    views, plain = [], []
+    for _ in range(n):
+        cfg = names[int(rng.integers(len(names)))]
+        g, spec = built[cfg]
+        w, h = int(rng.integers(8, 641)), int(rng.integers(8, 401))
+        cam = random_pose(scenes, cfg, spec, rng, w, h)
+        views.append((cfg, g, cam, w, h))
+        plain.append(render_gpu(sr, g, cam, w, h))
+    assert sum(int((p["ids"] != sr.NO_ID).any()) for p in plain) > 0.75 * n
+    return views, plain
+
+
+@pytest.mark.timeout(900)
+def test_random_batch_matches_plain(sr, scenes, env):
+    """48 random views over five scenes (the cluster plain, alpha-masked and
+    translucent, the heightfield, the overdraw scene) at random sizes in one
+    batch: launched eagerly, as a hipGraph replayed twice, and with the
+    persistent raster, every view's visibility buffer and pixels equal the
+    plain renderer's."""
+    built = five_scenes(sr, scenes, env)
+    vs, plain = random_views(sr, scenes, built, 6000, 48)
+    views = [(g, cam, w, h, 2.0) for _, g, cam, w, h in vs]
+    for persistent, use_graph in ((-1, False), (-1, True), (0, False)):
+        b = sr.Batch(tile_size=16)
+        b.configure(persistent_blocks=persistent, use_graph=use_graph)
+        for rep in range(2 if use_graph else 1):
+            b.render(views)
+            for i, p in enumerate(plain):
+                ids, depth = b.read_vbuffer(i)
+                where = f"view {i} (persistent {persistent}, graph {use_graph}, replay {rep})"
+                assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"])), where
+                assert np.array_equal(b.read_pixels(i), p["px"]), where + " pixels"
+        b.close()
+
+
+@pytest.mark.timeout(900)
+def test_random_requests_served_match_plain(sr, scenes, env):
+    """48 random requests over the same five scenes submitted at once to the
+    render server (batches of at most 8 over two overlapping slots, with an
+    admission budget that splits them further): every request's visibility
+    buffer and pixels equal the plain renderer's."""
+    built = five_scenes(sr, scenes, env)
+    vs, plain = random_views(sr, scenes, built, 7000, 48)
+    srv = sr.Server(admission_bytes=256 << 20, max_batch_views=8, streams=2)
+    idx = {cfg: srv.add_scene(g) for cfg, (g, _) in built.items()}
+    rids = [srv.submit(idx[cfg], cam, w, h, tag=i) for i, (cfg, _, cam, w, h) in enumerate(vs)]
+    srv.drain()
+    done = srv.poll(64)
+    done += srv.poll(64)
+    assert sorted(d["request_id"] for d in done if d["request_id"] in rids) == sorted(rids)
+    assert all(d["status"] == 0 for d in done)
+    for i, rid in enumerate(rids):
+        (ids, depth), px = srv.read_vbuffer(rid), srv.read_pixels(rid)
+        p, h = plain[i], vs[i][4]
+        assert np.array_equal(ids, p["ids"]) and np.array_equal(bits(depth), bits(p["depth"])), f"request {i}"
+        assert np.array_equal(px[: h & ~1], p["px"][: h & ~1]), f"request {i} pixels"
+        srv.release(rid)
+    srv.close()
