@@ -331,16 +331,16 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
     if args.config == "c4":
         print(json.dumps(run_batched(args)))
         return
     if args.config == "c5":
         print(json.dumps(run_served(args)))
         return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
 
     import __graft_entry__ as entry
     sr = entry.load_package()
