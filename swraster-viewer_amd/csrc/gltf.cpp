@@ -60,9 +60,12 @@ struct JVal {
         const JVal *v = get(key);
         return v && v->type == Num ? v->n : def;
     }
+    // A number as an int; out of int's range (or NaN) it is -1, which every
+    // caller rejects as an index.
+    static int to_int(double x) { return x >= -2147483648.0 && x <= 2147483647.0 ? (int)x : -1; }
     int integer(const char *key, int def) const {
         const JVal *v = get(key);
-        return v && v->type == Num ? (int)v->n : def;
+        return v && v->type == Num ? to_int(v->n) : def;
     }
     std::string str(const char *key) const {
         const JVal *v = get(key);
@@ -336,8 +339,9 @@ struct Loader {
                 std::string path = dir + percent_decode(uri);
                 if (!read_file(path, data)) return fail(SR_ERR_NOT_FOUND, "cannot read glTF buffer '%s'", path.c_str());
             }
-            size_t need = (size_t)b.num("byteLength", 0.0);
-            if (data.size() < need) return bad("glTF buffer %s is shorter than its byteLength", std::to_string(i));
+            size_t need = 0;
+            if (!to_size(b.num("byteLength", 0.0), need) || data.size() < need)
+                return bad("glTF buffer %s is shorter than its byteLength", std::to_string(i));
             buffers.push_back(std::move(data));
         }
         return SR_OK;
@@ -353,6 +357,15 @@ struct Loader {
         size_t comp_size() const { return comp == 5126 || comp == 5125 ? 4 : (comp == 5123 || comp == 5122 ? 2 : 1); }
     };
 
+    // A JSON number used as a byte offset, length, stride or count: a
+    // non-negative value below 2^52 (so sums of two stay exact and cannot
+    // wrap), else false.
+    static bool to_size(double x, size_t &out) {
+        if (!(x >= 0.0 && x < 4503599627370496.0)) return false;
+        out = (size_t)x;
+        return true;
+    }
+
     int accessor(int index, Acc &A) {
         const JVal *as = doc.get("accessors");
         const JVal *a = as ? as->at((size_t)index) : nullptr;
@@ -367,22 +380,28 @@ struct Loader {
         A.comp = a->integer("componentType", 0);
         if (!(A.comp == 5120 || A.comp == 5121 || A.comp == 5122 || A.comp == 5123 || A.comp == 5125 || A.comp == 5126))
             return bad("unsupported glTF component type %s", std::to_string(A.comp));
-        A.count = (size_t)a->num("count", 0.0);
         A.normalized = a->get("normalized") && a->get("normalized")->b;
         const JVal *bv = a->get("bufferView");
         if (!bv) return fail(SR_ERR_UNSUPPORTED, "glTF accessor %d has no bufferView", index);
         const JVal *views = doc.get("bufferViews");
-        const JVal *v = views ? views->at((size_t)bv->n) : nullptr;
-        if (!v) return bad("glTF bufferView %s does not exist", std::to_string((int)bv->n));
+        size_t vi = 0;
+        const JVal *v = views && to_size(bv->n, vi) ? views->at(vi) : nullptr;
+        if (!v) return bad("glTF accessor %s refers to a missing bufferView", std::to_string(index));
         int buf = v->integer("buffer", -1);
         if (buf < 0 || (size_t)buf >= buffers.size()) return bad("glTF bufferView refers to a missing buffer");
-        size_t voff = (size_t)v->num("byteOffset", 0.0), vlen = (size_t)v->num("byteLength", 0.0);
-        size_t aoff = (size_t)a->num("byteOffset", 0.0);
+        size_t voff = 0, vlen = 0, aoff = 0;
+        if (!to_size(a->num("count", 0.0), A.count) || !to_size(v->num("byteOffset", 0.0), voff) ||
+            !to_size(v->num("byteLength", 0.0), vlen) || !to_size(a->num("byteOffset", 0.0), aoff) ||
+            !to_size(v->num("byteStride", 0.0), A.stride))
+            return bad("glTF accessor %s has a negative or oversized count, offset, length or stride",
+                       std::to_string(index));
         size_t elem = A.comp_size() * (size_t)A.ncomp;
-        A.stride = (size_t)v->num("byteStride", 0.0);
         if (A.stride == 0) A.stride = elem;
-        if (voff + vlen > buffers[buf].size()) return bad("glTF bufferView overruns its buffer");
-        if (A.count > 0 && aoff + (A.count - 1) * A.stride + elem > vlen) return bad("glTF accessor overruns its bufferView");
+        const size_t bsize = buffers[buf].size();
+        if (voff > bsize || vlen > bsize - voff) return bad("glTF bufferView overruns its buffer");
+        // the last element ends inside the view (no product or sum can wrap)
+        if (A.count > 0 && (aoff > vlen || elem > vlen - aoff || A.count - 1 > (vlen - aoff - elem) / A.stride))
+            return bad("glTF accessor overruns its bufferView");
         A.base = buffers[buf].data() + voff + aoff;
         return SR_OK;
     }
@@ -485,27 +504,27 @@ struct Loader {
                 if (!ia) return bad("glTF mesh %s has a primitive without indices", std::to_string(mi));
                 std::vector<float> pos, nrm, uv, tan;
                 std::vector<uint32_t> idx;
-                int rc = read_floats((int)pa->n, 3, pos, false);
-                if (!rc) rc = read_indices((int)ia->n, idx);
+                int rc = read_floats(JVal::to_int(pa->n), 3, pos, false);
+                if (!rc) rc = read_indices(JVal::to_int(ia->n), idx);
                 if (rc) return rc;
                 size_t nv = pos.size() / 3;
                 const JVal *ta = attr->get("TEXCOORD_0");
                 if (ta) {
-                    if ((rc = read_floats((int)ta->n, 2, uv, true))) return rc;
+                    if ((rc = read_floats(JVal::to_int(ta->n), 2, uv, true))) return rc;
                     if (uv.size() / 2 != nv) return bad("glTF TEXCOORD_0 count differs from POSITION in mesh %s", std::to_string(mi));
                 } else {
                     uv.assign(nv * 2, 0.0f);
                 }
                 const JVal *na = attr->get("NORMAL");
                 if (na) {
-                    if ((rc = read_floats((int)na->n, 3, nrm, false))) return rc;
+                    if ((rc = read_floats(JVal::to_int(na->n), 3, nrm, false))) return rc;
                     if (nrm.size() / 3 != nv) return bad("glTF NORMAL count differs from POSITION in mesh %s", std::to_string(mi));
                 } else {
                     smooth_normals(pos, idx, nrm);
                 }
                 const JVal *tga = attr->get("TANGENT");
                 if (tga) {
-                    if ((rc = read_floats((int)tga->n, 4, tan, false))) return rc;
+                    if ((rc = read_floats(JVal::to_int(tga->n), 4, tan, false))) return rc;
                     if (tan.size() / 4 != nv) return bad("glTF TANGENT count differs from POSITION in mesh %s", std::to_string(mi));
                 }
                 for (uint32_t i : idx)

@@ -46,6 +46,7 @@ struct Huff {
             valptr[l] = k;
             mincode[l] = code;
             for (int i = 0; i < bits[l - 1]; i++, k++, code++) {
+                if (code >= (1 << l)) return false;   // over-subscribed: no l-bit code left
                 if (l <= 9) {
                     int lo = code << (9 - l), n = 1 << (9 - l);
                     for (int j = 0; j < n; j++) {

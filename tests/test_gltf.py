@@ -394,6 +394,37 @@ def test_gltf_errors(sr, tmp_path):
         sr.SceneData.from_gltf(str(tmp_path / "bad.gltf"))
 
 
+@pytest.mark.parametrize("field,value", [("view_offset", -48.0), ("view_offset", 1.8e19), ("count", 1e15),
+                                         ("count", -1.0), ("accessor_offset", -12.0), ("stride", -4.0),
+                                         ("view_length", 1.8e19), ("buffer_view", -1.0)])
+def test_gltf_rejects_out_of_range_numbers(sr, tmp_path, field, value):
+    """Negative or huge offsets, lengths, strides, counts and indices are
+    refused before any read (a mutation fuzz of the loader under
+    AddressSanitizer found a negative byteOffset wrapping the bounds
+    check)."""
+    pos, idx = _quad()
+    acc = [_acc(0, 5126, 4, "VEC3"), _acc(1, 5123, 6, "SCALAR")]
+    doc = {"accessors": acc, "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "indices": 1}]}],
+           "nodes": [{"mesh": 0}]}
+    path = _write(tmp_path, doc, [pos, idx])
+    d = json.loads(open(path).read())
+    if field == "view_offset":
+        d["bufferViews"][0]["byteOffset"] = value
+    elif field == "view_length":
+        d["bufferViews"][0]["byteLength"] = value
+    elif field == "count":
+        d["accessors"][0]["count"] = value
+    elif field == "accessor_offset":
+        d["accessors"][0]["byteOffset"] = value
+    elif field == "stride":
+        d["bufferViews"][0]["byteStride"] = value
+    else:
+        d["accessors"][0]["bufferView"] = value
+    open(path, "w").write(json.dumps(d))
+    with pytest.raises(sr.SwrasterError):
+        sr.SceneData.from_gltf(path)
+
+
 def test_gltf_load_time_c2(sr, scenes, tmp_path):
     """The 1,045,458-triangle heightfield with its 2048^2 texture: loaded
     arrays identical, load time reported."""

@@ -95,6 +95,21 @@ def test_jpeg_rejects_bad_input(sr):
         sr.jpeg_decode(bytes(data))
 
 
+def test_jpeg_rejects_oversubscribed_huffman_lengths(sr):
+    """A DHT whose code-length counts ask for more codes of a length than
+    that many bits hold (three 1-bit codes) is refused: the table is never
+    built past its 9-bit lookup (a mutation fuzz of the decoder under
+    AddressSanitizer found the lookup written out of bounds here)."""
+    data = bytearray(encode(picture(16, 16), quality=80))
+    i = data.index(b"\xff\xc4")
+    counts = data[i + 5:i + 21]
+    nv = sum(counts)
+    assert nv >= 3
+    data[i + 5:i + 21] = bytes([3] + [0] * 14 + [nv - 3])
+    with pytest.raises(sr.SwrasterError, match="Huffman"):
+        sr.jpeg_decode(bytes(data))
+
+
 REF_CUBEMAP = "/root/reference/assets/cubemap.jpg"
 
 
