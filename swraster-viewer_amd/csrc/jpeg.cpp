@@ -176,6 +176,10 @@ struct Decoder {
         W = (p[3] << 8) | p[4];
         int n = p[5];
         if (W <= 0 || H <= 0) return fail("JPEG without a defined size (DNL) is not decoded");
+        // the header alone sizes the coefficient planes: refuse what a
+        // 512 MiB RGBA8 image would not hold (the image crate's default
+        // allocation limit) before allocating
+        if ((uint64_t)W * (uint64_t)H > (1ull << 27)) return fail("JPEG larger than 2^27 pixels");
         if (n != 1 && n != 3) return fail("only 1- and 3-component JPEGs are decoded (CMYK is not)");
         if (len < 6 + 3 * n) return fail("short SOF");
         comps.assign(n, Comp());

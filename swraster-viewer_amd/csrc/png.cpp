@@ -264,6 +264,8 @@ bool png_decode_rgba8(const uint8_t *data, size_t size, uint32_t &width, uint32_
     }
     if (!seen_end || ctype < 0) { err = "PNG has no IHDR or IEND"; return false; }
     if (w == 0 || h == 0 || w > 65536 || h > 65536) { err = "PNG size out of range"; return false; }
+    // at most a 512 MiB RGBA8 image (the image crate's default allocation limit)
+    if ((uint64_t)w * h > (1ull << 27)) { err = "PNG larger than 2^27 pixels"; return false; }
     if (interlace != 0) { err = "interlaced PNG is not supported"; return false; }
     int channels;
     switch (ctype) {

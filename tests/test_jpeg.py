@@ -110,6 +110,17 @@ def test_jpeg_rejects_oversubscribed_huffman_lengths(sr):
         sr.jpeg_decode(bytes(data))
 
 
+def test_jpeg_refuses_huge_frames_before_allocating(sr):
+    """A frame header claiming 65535 x 65535 samples is refused at once
+    (the header alone would size about 25 GB of coefficient planes); at
+    most 2^27 pixels, a 512 MiB RGBA8 image, are decoded."""
+    data = bytearray(encode(picture(16, 16), quality=80))
+    i = data.index(b"\xff\xc0")
+    data[i + 5:i + 9] = b"\xff\xff\xff\xff"
+    with pytest.raises(sr.SwrasterError, match="2\\^27"):
+        sr.jpeg_decode(bytes(data))
+
+
 REF_CUBEMAP = "/root/reference/assets/cubemap.jpg"
 
 
