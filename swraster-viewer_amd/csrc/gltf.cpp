@@ -583,7 +583,7 @@ struct Loader {
             if (d.mesh >= (int)g->meshes.size()) return bad("glTF node %s refers to a missing mesh", std::to_string(i));
             if (const JVal *c = n.get("children"))
                 for (size_t k = 0; k < c->size(); k++) {
-                    int ci = (int)c->at(k)->n;
+                    int ci = JVal::to_int(c->at(k)->n);
                     if (ci < 0 || (size_t)ci >= nn) return bad("glTF node %s has a missing child", std::to_string(i));
                     is_child[(size_t)ci] = 1;
                 }
@@ -604,7 +604,8 @@ struct Loader {
                 mat4_mul(it.parent.data(), local[it.node].data(), world.data());
                 memcpy(g->nodes[it.node].transform, world.data(), 64);
                 if (const JVal *c = ns->at(it.node)->get("children"))
-                    for (size_t k = c->size(); k-- > 0;) stack.push_back({(size_t)c->at(k)->n, world, it.depth + 1});
+                    for (size_t k = c->size(); k-- > 0;)
+                        stack.push_back({(size_t)JVal::to_int(c->at(k)->n), world, it.depth + 1});
             }
         }
         return SR_OK;
