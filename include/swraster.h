@@ -362,14 +362,16 @@ SR_API int sr_gltf_scene_desc(sr_gltf *gltf, sr_scene_desc *desc);
 SR_API int sr_gltf_destroy(sr_gltf *gltf);
 /* PNG decoding as used for glTF textures (non-interlaced; every colour type
  * and bit depth; tRNS): packed r<<24|g<<16|b<<8|a texels.  With texels NULL
- * only the size is returned. */
+ * only the size is returned.  Images over 2^27 pixels (512 MiB of RGBA8) are
+ * refused. */
 SR_API int sr_png_decode(const uint8_t *data, size_t size, uint32_t *width, uint32_t *height,
                          uint32_t *texels, size_t capacity);
 /* JPEG decoding as used for glTF textures (the reference's image crate,
  * src/texture.rs:897-1015): baseline, extended sequential and progressive
  * Huffman, 8-bit, grey or YCbCr (RGB with an Adobe marker), any sampling
  * factors, restart intervals; packed r<<24|g<<16|b<<8|0xFF texels.  With
- * texels NULL only the size is returned. */
+ * texels NULL only the size is returned.  Frames over 2^27 pixels are
+ * refused before anything is allocated. */
 SR_API int sr_jpeg_decode(const uint8_t *data, size_t size, uint32_t *width, uint32_t *height,
                           uint32_t *texels, size_t capacity);
 
