@@ -19,7 +19,9 @@ import pytest
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(REPO, "swraster-viewer_amd", "csrc")
 
-pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ is not installed")
+pytestmark = pytest.mark.skipif(shutil.which("g++") is None or
+                                not os.path.exists("/opt/rocm/include/hip/hip_runtime_api.h"),
+                                reason="g++ or the HIP headers (the loader includes the runtime's) are missing")
 
 ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:verify_asan_link_order=0:abort_on_error=1",
            UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
