@@ -116,6 +116,22 @@ def test_c1_random_cameras(sr, orc, scenes, c1, seed):
     compare(sr, g, o, f"c1 camera {seed} ({w}x{h}, tile {tile})")
 
 
+@pytest.mark.parametrize("name,over", [("fov 1 deg", {"fov": np.radians(1.0)}),
+                                       ("fov 179 deg", {"fov": np.radians(179.0)}),
+                                       ("far 1e6", {"far": 1e6}), ("far 1e30", {"far": 1e30}),
+                                       ("far 0.05", {"far": 0.05}), ("inside a sphere", {"position": [2.0, 1.0, 0.5]})])
+def test_c1_extreme_cameras(sr, orc, scenes, c1, name, over):
+    """Extreme fields of view (at 1 degree a sphere fills the frame, at 179
+    every sphere is below a pixel) and far planes (near = 0.01 far, from
+    5e-4 to 1e28), and a camera inside a sphere: the visibility buffer and
+    pixels still equal the oracle's."""
+    scene, spec, gscene = c1
+    s = dict(spec)
+    s.update(over)
+    cam = scenes.make_camera(s, 320, 180)
+    compare(sr, render_gpu(sr, gscene, cam, 320, 180), render_oracle(orc, scene, cam, 320, 180), f"c1 {name}")
+
+
 def test_c1_inside_cluster_clipping(sr, orc, scenes, c1):
     """Camera inside the sphere cluster: near/side-plane clipping and
     negative depths (view-space cull near plane sits behind the eye)."""
